@@ -1,0 +1,318 @@
+"""Host-side mirror of the reference interface of the plane-wave Davidson path.
+
+Names and argument meanings follow ABACUS:
+
+* :class:`HamiltPW` -- ``hamilt::HamiltPW`` with ``ops->hPsi`` (operator.cpp:39-88) and the
+  ``act`` of Ekinetic / Veff / Nonlocal (accumulate semantics);
+* :class:`DiagoDavid` -- ``hsolver::DiagoDavid`` (diago_david.h:21-84): constructor
+  ``(precondition, nband, dim, david_ndim, use_paw, ...)`` and
+  ``diag(hpsi_func, spsi_func, ldPsi, psi, eigenvalue, david_diag_thr, david_maxiter,
+  ntry_max=5, notconv_max=0)`` returning the total iteration count;
+* the device functors ``ekinetic_pw_op``, ``veff_pw_op``, ``nonlocal_pw_op``,
+  ``set_3d_fft_box_op``, ``set_recip_to_real_output_op``, ``set_real_to_recip_output_op``,
+  ``gemm_op``, ``dnevx_op`` and ``FFT::fft3D_forward/backward``.
+
+All compute goes through libapw.so (HIP, gfx950); device arrays are torch tensors used
+only as device memory.  Errors are raised as :class:`apw._lib.ApwError` (the reference
+aborts instead).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import APW_FP32_MIXED, APW_FP64, check
+
+_PREC = {"fp64": APW_FP64, "double": APW_FP64, "fp32_mixed": APW_FP32_MIXED, "mixed": APW_FP32_MIXED,
+         APW_FP64: APW_FP64, APW_FP32_MIXED: APW_FP32_MIXED}
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _ptr(t):
+    return C.c_void_p(t.data_ptr())
+
+
+def _dev_c128(t):
+    torch = _torch()
+    if t.dtype != torch.complex128 or not t.is_cuda or not t.is_contiguous():
+        raise TypeError("expected a contiguous complex128 CUDA tensor")
+    return t
+
+
+class Context:
+    """One device and one stream.  ``stream``: a HIP stream handle to share (e.g. a non-default
+    torch stream's ``cuda_stream``); None lets the library create its own.  Calls made
+    through this module wait for pending torch work on the device first, so torch-written
+    inputs are complete; results are complete after :meth:`synchronize`."""
+
+    def __init__(self, device: int = 0, stream=None):
+        _torch().cuda.init()
+        self.lib = _lib.load()
+        self.device = device
+        self.handle = C.c_void_p()
+        check(self.lib.apw_ctx_create(C.byref(self.handle), device, C.c_void_p(stream or 0)))
+
+    def pre(self):
+        _torch().cuda.synchronize(self.device)
+
+    def synchronize(self):
+        check(self.lib.apw_ctx_synchronize(self.handle))
+
+    def close(self):
+        if self.handle:
+            self.lib.apw_ctx_destroy(self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class HamiltPW:
+    """H of one k-point built from host arrays (see apw_ham_desc in include/apw.h)."""
+
+    def __init__(self, ctx: Context, system, precision="fp64"):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        s = system
+        self.npw, self.npwx = s.npw, s.npwx
+        self.nx, self.ny, self.nz = s.nx, s.ny, s.nz
+        self._keep = dict(
+            ig2ixyz=np.ascontiguousarray(s.ig2ixyz[: s.npw], dtype=np.int32),
+            gk2=np.ascontiguousarray(s.gk2[: s.npw], dtype=np.float64),
+            veff=np.ascontiguousarray(s.veff, dtype=np.float64),
+            vkb=np.ascontiguousarray(s.vkb, dtype=np.complex128),
+            na=np.ascontiguousarray(s.na, dtype=np.int32),
+            nh=np.ascontiguousarray(s.nh, dtype=np.int32),
+            deeq=np.ascontiguousarray(s.deeq[0], dtype=np.float64),
+        )
+        k = self._keep
+        d = _lib.HamDesc()
+        d.nx, d.ny, d.nz, d.npw, d.npwx = s.nx, s.ny, s.nz, s.npw, s.npwx
+        d.ig2ixyz = k["ig2ixyz"].ctypes.data_as(C.POINTER(C.c_int))
+        d.gk2 = k["gk2"].ctypes.data_as(C.POINTER(C.c_double))
+        d.tpiba2 = s.tpiba2
+        d.veff = k["veff"].ctypes.data_as(C.POINTER(C.c_double))
+        d.nkb = s.vkb.shape[0]
+        d.vkb = k["vkb"].ctypes.data_as(C.POINTER(C.c_double))
+        d.ntype = len(k["na"])
+        d.na = k["na"].ctypes.data_as(C.POINTER(C.c_int))
+        d.nh = k["nh"].ctypes.data_as(C.POINTER(C.c_int))
+        d.nhm = s.deeq.shape[-1]
+        d.deeq = k["deeq"].ctypes.data_as(C.POINTER(C.c_double))
+        d.precision = _PREC[precision]
+        self.handle = C.c_void_p()
+        check(self.lib.apw_ham_create(ctx.handle, C.byref(d), C.byref(self.handle)))
+
+    def set_precision(self, precision):
+        check(self.lib.apw_ham_set_precision(self.handle, _PREC[precision]))
+
+    def hPsi(self, psi, nbands, ldpsi=None, hpsi=None):
+        """Operator::hPsi: returns hpsi (nbands x ld) = H psi."""
+        torch = _torch()
+        ld = ldpsi or psi.shape[-1]
+        if hpsi is None:
+            hpsi = torch.empty((nbands, ld), dtype=torch.complex128, device=psi.device)
+        self.ctx.pre()
+        check(self.lib.apw_hpsi(self.handle, _ptr(_dev_c128(psi)), ld, _ptr(_dev_c128(hpsi)), hpsi.shape[-1], nbands))
+        return hpsi
+
+    def act_ekinetic(self, nbands, nbasis, psi, hpsi):
+        self.ctx.pre()
+        check(self.lib.apw_act_ekinetic(self.handle, nbands, nbasis, _ptr(psi), _ptr(hpsi)))
+
+    def act_veff(self, nbands, nbasis, psi, hpsi):
+        self.ctx.pre()
+        check(self.lib.apw_act_veff(self.handle, nbands, nbasis, _ptr(psi), _ptr(hpsi)))
+
+    def act_nonlocal(self, nbands, nbasis, psi, hpsi):
+        self.ctx.pre()
+        check(self.lib.apw_act_nonlocal(self.handle, nbands, nbasis, _ptr(psi), _ptr(hpsi)))
+
+    def recip_to_real(self, inp, out, add=False, factor=1.0):
+        self.ctx.pre()
+        check(self.lib.apw_recip_to_real(self.handle, _ptr(inp), _ptr(out), int(add), float(factor)))
+
+    def real_to_recip(self, inp, out, add=False, factor=1.0):
+        self.ctx.pre()
+        check(self.lib.apw_real_to_recip(self.handle, _ptr(inp), _ptr(out), int(add), float(factor)))
+
+    def close(self):
+        if self.handle:
+            self.lib.apw_ham_destroy(self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DiagoDavid:
+    """hsolver::DiagoDavid on the GPU; ``precondition`` is a host array (len >= dim)."""
+
+    def __init__(self, ctx: Context, precondition, nband: int, dim: int, david_ndim: int, use_paw: bool = False):
+        if use_paw:
+            raise NotImplementedError("PAW is outside the plane-wave norm-conserving path")
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.nband, self.dim = nband, dim
+        pre = np.ascontiguousarray(precondition[:dim], dtype=np.float64)
+        self.handle = C.c_void_p()
+        check(self.lib.apw_david_create(ctx.handle, pre.ctypes.data_as(C.POINTER(C.c_double)), nband, dim,
+                                        david_ndim, C.byref(self.handle)))
+        self.stats = None
+
+    def diag(self, hpsi_func, spsi_func, ldPsi, psi, eigenvalue, david_diag_thr, david_maxiter, ntry_max=5,
+             notconv_max=0, precision="fp64"):
+        """DiagoDavid::diag.  ``hpsi_func`` is a :class:`HamiltPW` (H applied inside the
+        library) or a callable ``f(X, HX, nvec, dim, id_start, id_end)`` on torch views
+        (X: nvec x dim head of the block, HX: the destination rows).  ``spsi_func``: None
+        for S = I or ``f(X, SX, nrow, npw, nbands)``.  ``psi``: CUDA complex128
+        (nband x ldPsi); ``eigenvalue``: host float64 array, filled in place."""
+        torch = _torch()
+        _dev_c128(psi)
+        eig = np.zeros(self.nband, dtype=np.float64)
+        stats = _lib.DiagStats()
+        ham = None
+        hcb = _lib.HPSI_CB()
+        scb = _lib.SPSI_CB()
+        errors = []
+        if isinstance(hpsi_func, HamiltPW):
+            ham = hpsi_func.handle
+        else:
+            dim = self.dim
+
+            def _h(user, X, HX, nvec, d, i0, i1):
+                try:
+                    xs = _wrap(X, nvec * d, psi.device).view(nvec, d)
+                    hs = _wrap(HX, (i1 - i0 + 1) * d, psi.device).view(i1 - i0 + 1, d)
+                    hpsi_func(xs, hs, nvec, d, i0, i1)
+                    torch.cuda.synchronize(psi.device)
+                    return 0
+                except Exception as exc:  # reported after the call returns
+                    errors.append(exc)
+                    return 1
+
+            hcb = _lib.HPSI_CB(_h)
+            if spsi_func is not None:
+                def _s(user, X, SX, nrow, npw, nb):
+                    try:
+                        xs = _wrap(X, nb * nrow, psi.device).view(nb, nrow)
+                        ss = _wrap(SX, nb * nrow, psi.device).view(nb, nrow)
+                        spsi_func(xs, ss, nrow, npw, nb)
+                        torch.cuda.synchronize(psi.device)
+                        return 0
+                    except Exception as exc:
+                        errors.append(exc)
+                        return 1
+
+                scb = _lib.SPSI_CB(_s)
+            del dim
+        torch.cuda.synchronize(psi.device)
+        rc = self.lib.apw_david_diag(self.handle, ham, hcb, scb, None, ldPsi, _ptr(psi),
+                                     eig.ctypes.data_as(C.POINTER(C.c_double)), float(david_diag_thr),
+                                     int(david_maxiter), int(ntry_max), int(notconv_max), _PREC[precision],
+                                     C.byref(stats))
+        if errors:
+            raise errors[0]
+        check(rc)
+        eigenvalue[: self.nband] = eig
+        self.stats = stats.as_dict()
+        return rc
+
+    def close(self):
+        if self.handle:
+            self.lib.apw_david_destroy(self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _wrap(ptr, count, device):
+    """A complex128 torch view of device memory owned by the library (no copy)."""
+    torch = _torch()
+
+    class _Arr:
+        pass
+
+    a = _Arr()
+    a.__cuda_array_interface__ = {"shape": (count,), "typestr": "<c16", "data": (int(ptr), False), "version": 2,
+                                  "strides": None}
+    return torch.as_tensor(a, device=device)
+
+
+# ---------------------------------------------------------------------------- functors
+def ekinetic_pw_op(ctx, nband, npw, max_npw, tpiba2, gk2, hpsi, psi):
+    ctx.pre()
+    check(ctx.lib.apw_ekinetic_pw_op(ctx.handle, nband, npw, max_npw, float(tpiba2), _ptr(gk2), _ptr(hpsi), _ptr(psi)))
+
+
+def veff_pw_op(ctx, size, out, inp):
+    ctx.pre()
+    check(ctx.lib.apw_veff_pw_op(ctx.handle, size, _ptr(out), _ptr(inp)))
+
+
+def nonlocal_pw_op(ctx, l1, l2, l3, sum_, iat, spin, nkb, deeq_x, deeq_y, deeq_z, deeq, ps, becp):
+    s = C.c_int(sum_)
+    ia = C.c_int(iat)
+    ctx.pre()
+    check(ctx.lib.apw_nonlocal_pw_op(ctx.handle, l1, l2, l3, C.byref(s), C.byref(ia), spin, nkb, deeq_x, deeq_y,
+                                     deeq_z, _ptr(deeq), _ptr(ps), _ptr(becp)))
+    return s.value, ia.value
+
+
+def set_3d_fft_box_op(ctx, npwk, box_index, inp, out):
+    ctx.pre()
+    check(ctx.lib.apw_set_3d_fft_box_op(ctx.handle, npwk, _ptr(box_index), _ptr(inp), _ptr(out)))
+
+
+def set_recip_to_real_output_op(ctx, nrxx, add, factor, inp, out):
+    ctx.pre()
+    check(ctx.lib.apw_set_recip_to_real_output_op(ctx.handle, nrxx, int(add), float(factor), _ptr(inp), _ptr(out)))
+
+
+def set_real_to_recip_output_op(ctx, npw_k, nxyz, add, factor, box_index, inp, out):
+    ctx.pre()
+    check(ctx.lib.apw_set_real_to_recip_output_op(ctx.handle, npw_k, nxyz, int(add), float(factor), _ptr(box_index),
+                                                  _ptr(inp), _ptr(out)))
+
+
+def fft3d(ctx, nx, ny, nz, box, direction):
+    """FFT::fft3D_backward (direction=+1) / fft3D_forward (-1), in place, unnormalised."""
+    ctx.pre()
+    check(ctx.lib.apw_fft3d(ctx.handle, nx, ny, nz, _ptr(box), int(direction)))
+
+
+def gemm_op(ctx, transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc):
+    al = (C.c_double * 2)(complex(alpha).real, complex(alpha).imag)
+    be = (C.c_double * 2)(complex(beta).real, complex(beta).imag)
+    ctx.pre()
+    check(ctx.lib.apw_zgemm(ctx.handle, transa.encode(), transb.encode(), m, n, k, al, _ptr(a), lda, _ptr(b), ldb,
+                            be, _ptr(c), ldc))
+
+
+def dnevx_op(ctx, nstart, ldh, hcc, nbands, eigenvalue, vcc):
+    ctx.pre()
+    check(ctx.lib.apw_dnevx_op(ctx.handle, nstart, ldh, _ptr(hcc), nbands, _ptr(eigenvalue), _ptr(vcc)))
+
+
+def probe_mfma_layout(ctx):
+    torch = _torch()
+    out = torch.zeros(4 * 64 * 4, dtype=torch.float64, device=f"cuda:{ctx.device}")
+    check(ctx.lib.apw_probe_mfma_layout(ctx.handle, _ptr(out)))
+    ctx.synchronize()
+    return out.view(4, 64, 4).cpu().numpy()

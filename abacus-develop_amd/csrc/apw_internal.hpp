@@ -1,0 +1,146 @@
+// Internal declarations of the apw library (HIP, gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "apw.h"
+
+namespace apw {
+
+// ---------------------------------------------------------------- errors
+void set_error(const char* fmt, ...);
+
+struct Error {
+    int code;
+};
+
+#define APW_HIP(call)                                                                         \
+    do {                                                                                      \
+        hipError_t e_ = (call);                                                               \
+        if (e_ != hipSuccess) {                                                               \
+            ::apw::set_error("%s:%d %s: %s", __FILE__, __LINE__, #call, hipGetErrorString(e_)); \
+            throw ::apw::Error{APW_EHIP};                                                     \
+        }                                                                                     \
+    } while (0)
+
+#define APW_CHECK(cond, ...)                      \
+    do {                                          \
+        if (!(cond)) {                            \
+            ::apw::set_error(__VA_ARGS__);        \
+            throw ::apw::Error{APW_EINVAL};       \
+        }                                         \
+    } while (0)
+
+#define APW_LAUNCH_CHECK() APW_HIP(hipGetLastError())
+
+// Runs f() translating exceptions into status codes (for the extern "C" layer).
+template <class F>
+int guarded(F&& f) {
+    try {
+        return f();
+    } catch (const Error& e) {
+        return e.code;
+    } catch (...) {
+        set_error("unexpected C++ exception");
+        return APW_EINVAL;
+    }
+}
+
+// ---------------------------------------------------------------- device memory
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    // grow-only; contents are not preserved
+    void reserve(size_t count) {
+        if (count <= n) return;
+        release();
+        hipError_t e = hipMalloc(&p, count * sizeof(T) + 256);
+        if (e != hipSuccess) {
+            p = nullptr;
+            set_error("hipMalloc of %zu bytes failed: %s", count * sizeof(T), hipGetErrorString(e));
+            throw Error{APW_ENOMEM};
+        }
+        n = count;
+    }
+    T* get() const { return p; }
+};
+
+struct GemmWorkspace {
+    DevBuf<double> part;
+};
+
+}  // namespace apw
+
+struct apw_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    int num_cu = 0;
+    // scratch shared by the functor-level entry points
+    apw::DevBuf<double> scratch;
+    apw::GemmWorkspace gws;
+};
+
+namespace apw {
+
+struct DeviceGuard {
+    int prev = 0;
+    explicit DeviceGuard(int dev) {
+        (void)hipGetDevice(&prev);
+        if (prev != dev) APW_HIP(hipSetDevice(dev));
+    }
+    ~DeviceGuard() {
+        int cur = 0;
+        (void)hipGetDevice(&cur);
+        if (cur != prev) (void)hipSetDevice(prev);
+    }
+};
+
+// ---------------------------------------------------------------- GEMM
+// Column-major complex GEMM, C = alpha op(A) op(B) + beta C.  op: 'N', 'T', 'C'.
+// R = double (c128 data) or float (c64 data).  ws: scratch for split-K partials.
+template <class R>
+void gemm(hipStream_t s, GemmWorkspace& ws, int num_cu, char ta, char tb, int m, int n, int k,
+          double alpha_re, double alpha_im, const R* A, int lda, const R* B, int ldb,
+          double beta_re, double beta_im, R* C, int ldc);
+
+// ---------------------------------------------------------------- eigensolver
+struct EigWorkspace {
+    DevBuf<double> a;      // n x n c128 working matrix
+    DevBuf<double> v;      // n x n c128 Householder vectors
+    DevBuf<double> tau;    // n c128
+    DevBuf<double> d, e;   // tridiagonal
+    DevBuf<double> p;      // n c128
+    DevBuf<double> z;      // n x nband real eigenvectors of T
+    DevBuf<unsigned> bar;  // grid barrier counter + abort flag
+    DevBuf<double> lu;     // inverse-iteration LU factors, 4 n per eigenvalue
+    DevBuf<int> piv;
+};
+// Lowest nband eigenpairs of Hermitian hcc (n x n, lower triangle, ld ldh).
+// eig: device double[nband]; vcc: device c128 (ld ldv), columns 0..nband-1 rows 0..n-1.
+void heevx_lowest(hipStream_t s, EigWorkspace& ws, int num_cu, int n, int ldh, const double* hcc,
+                  int nband, double* eig, double* vcc, int ldv);
+
+// ---------------------------------------------------------------- small dense helpers
+// Cholesky of Hermitian positive definite G (n x n, ld) and inverse of the upper factor:
+// G = R^H R, Rinv written to rinv (ld n).  status[0] != 0 on breakdown (pivot <= tiny).
+void chol_inv_upper(hipStream_t s, int n, const double* g, int ldg, double* rinv, int* status,
+                    double min_norm);
+
+}  // namespace apw

@@ -1,0 +1,481 @@
+// Block Davidson on the GPU: the DiagoDavid<complex<double>, DEVICE_GPU> call surface
+// (diago_david.h:21-84) with the algorithm of diago_david.cpp:151-1240.
+//
+// The host drives the iteration; every vector and subspace operation runs on the device:
+//   * the per-vector classical Gram-Schmidt of SchmidtOrth (diago_david.cpp:950-1075,
+//     one host sync per vector) is done as one block: L = Q^H S V, G = V^H S V - L^H L,
+//     G = R^H R (Cholesky), V <- (V - Q L) R^-1.  In exact arithmetic this is the same
+//     vectors as the reference's Gram-Schmidt with Pythagorean norms (both are the QR
+//     factor with a positive diagonal), and it breaks down (norm < 1e-12) in the same place;
+//   * the subspace eigenproblem (dnevx_op, zheevx on the host in the ROCm reference) is the
+//     GPU solver of eigh.hip;
+//   * refresh() sets hcc/vcc on the device instead of a host round trip
+//     (diago_david.cpp:882-920), keeping the reference's behaviour of clearing hcc and scc
+//     but only setting the diagonal of vcc.
+// One host sync per iteration remains: the nband eigenvalues come back for the
+// convergence test |e_new - e_old| < thr (diago_david.cpp:270-280).
+//
+// FP32-mixed form: a first diag_once runs with the H|psi> chain in FP32 (subspace work in
+// FP64) until the residual norms of the corrected bands drop below 1e-5 or no band is
+// left unconverged; its Ritz vectors then restart the plain FP64 iteration, which alone
+// decides convergence.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <vector>
+
+#include "apw_ham.hpp"
+#include "apw_internal.hpp"
+
+namespace apw {
+namespace {
+
+int gsz(size_t n) { return (int)std::max<size_t>(1, std::min<size_t>((n + 255) / 256, 8192)); }
+
+__global__ void k_copy_cols(int rows, int ncols, const double2* __restrict__ src, int lds, double2* __restrict__ dst,
+                            int ldd) {
+    const size_t total = (size_t)rows * ncols;
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        const int r = (int)(t % rows), c = (int)(t / rows);
+        dst[(size_t)c * ldd + r] = src[(size_t)c * lds + r];
+    }
+}
+
+__global__ void k_zero_rows(int r0, int r1, int ncols, double2* __restrict__ x, int ld) {
+    const int rows = r1 - r0;
+    const size_t total = (size_t)rows * ncols;
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        const int r = r0 + (int)(t % rows), c = (int)(t / rows);
+        x[(size_t)c * ld + r] = make_double2(0.0, 0.0);
+    }
+}
+
+__global__ void k_zero(size_t n, double* __restrict__ x) {
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) x[t] = 0.0;
+}
+
+// vc[i + m*nbase] = vcc[i + unconv[m]*nbx]     (cal_grad, diago_david.cpp:408-415)
+__global__ void k_gather_vc(int nbase, int nc, const double2* __restrict__ vcc, int nbx, const int* __restrict__ unconv,
+                            double2* __restrict__ vc) {
+    const size_t total = (size_t)nbase * nc;
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        const int i = (int)(t % nbase), m = (int)(t / nbase);
+        vc[t] = vcc[(size_t)unconv[m] * nbx + i];
+    }
+}
+
+// vc[:, m] *= -eig[unconv[m]]                  (diago_david.cpp:449-475)
+__global__ void k_scale_vc(int nbase, int nc, double2* __restrict__ vc, const double* __restrict__ eig,
+                           const int* __restrict__ unconv) {
+    const size_t total = (size_t)nbase * nc;
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        const int m = (int)(t / nbase);
+        const double f = -1.0 * eig[unconv[m]];
+        double2 v = vc[t];
+        v.x *= f;
+        v.y *= f;
+        vc[t] = v;
+    }
+}
+
+// X[i, m] /= pre[i]                            (vector_div_vector_op, diago_david.cpp:505-532)
+__global__ void k_precond(int dim, int nc, double2* __restrict__ x, const double* __restrict__ pre) {
+    const size_t total = (size_t)dim * nc;
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        const int i = (int)(t % dim);
+        double2 v = x[t];
+        v.x = v.x / pre[i];
+        v.y = v.y / pre[i];
+        x[t] = v;
+    }
+}
+
+// out[m] = || x[:, m] - lam[m] * y[:, m] ||^2  (one workgroup per column; y may be null)
+__global__ void k_colres2(int rows, const double2* __restrict__ x, int ldx, const double2* __restrict__ y, int ldy,
+                          const double* __restrict__ lam, double* __restrict__ out) {
+    __shared__ double red[4];
+    const int m = blockIdx.x;
+    const double l = (y && lam) ? lam[m] : 0.0;
+    double s = 0;
+    for (int i = threadIdx.x; i < rows; i += blockDim.x) {
+        double2 v = x[(size_t)m * ldx + i];
+        if (y) {
+            const double2 w = y[(size_t)m * ldy + i];
+            v.x -= l * w.x;
+            v.y -= l * w.y;
+        }
+        s += v.x * v.x + v.y * v.y;
+    }
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) out[m] = red[0] + red[1] + red[2] + red[3];
+}
+
+// refresh(): hcc = diag(eig[0..nband)), vcc[i][i] = 1 (off-diagonal entries of vcc kept)
+__global__ void k_refresh_diag(int nband, double2* __restrict__ hcc, double2* __restrict__ vcc, int nbx,
+                               const double* __restrict__ eig) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nband; i += gridDim.x * blockDim.x) {
+        hcc[(size_t)i * nbx + i] = make_double2(eig[i], 0.0);
+        vcc[(size_t)i * nbx + i] = make_double2(1.0, 0.0);
+    }
+}
+
+}  // namespace
+
+struct David {
+    apw_ctx* ctx = nullptr;
+    int nband = 0, dim = 0, ndim = 0, nbx = 0;
+    DevBuf<double> pre, basis, hp, sp, hcc, vcc, eig, tmp, vc, lmat, gmat, rinv, resid;
+    DevBuf<int> unconv_d, status;
+    EigWorkspace ews;
+    GemmWorkspace gws;
+
+    // per call
+    Ham* ham = nullptr;
+    apw_hpsi_cb hcb = nullptr;
+    apw_spsi_cb scb = nullptr;
+    void* user = nullptr;
+    int prec = APW_FP64;
+    int hpsi_calls = 0;
+    long long hpsi_bands = 0;
+    double last_resid = -1.0;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> hev;
+
+    hipStream_t st() const { return ctx->stream; }
+    double2* B(double* p, int col) const { return reinterpret_cast<double2*>(p) + (size_t)col * dim; }
+    double* Bd(double* p, int col) const { return p + (size_t)2 * col * dim; }
+
+    void create(apw_ctx* c, const double* precondition, int nband_, int dim_, int ndim_) {
+        ctx = c;
+        nband = nband_;
+        dim = dim_;
+        ndim = ndim_;
+        APW_CHECK(precondition, "DiagoDavid: null precondition");
+        APW_CHECK(nband > 0 && dim > 0, "DiagoDavid: nband and dim must be positive");
+        APW_CHECK(ndim > 1, "DiagoDavid: david_ndim must be > 1");                 // diago_david.cpp:60
+        APW_CHECK((long)ndim * nband < (long)dim, "DiagoDavid: david_ndim*nband must be < dim");  // :61
+        nbx = ndim * nband;
+        APW_CHECK(nbx <= 2000, "DiagoDavid: david_ndim*nband = %d exceeds the subspace solver limit", nbx);
+        pre.reserve(dim);
+        APW_HIP(hipMemcpy(pre.get(), precondition, dim * sizeof(double), hipMemcpyHostToDevice));
+        basis.reserve((size_t)2 * dim * nbx);
+        hp.reserve((size_t)2 * dim * nbx);
+        hcc.reserve((size_t)2 * nbx * nbx);
+        vcc.reserve((size_t)2 * nbx * nbx);
+        eig.reserve(nbx);
+        tmp.reserve((size_t)2 * dim * nband);
+        vc.reserve((size_t)2 * nbx * nband);
+        lmat.reserve((size_t)2 * nbx * nband);
+        gmat.reserve((size_t)2 * nband * nband);
+        rinv.reserve((size_t)4 * nband * nband);
+        resid.reserve(nbx);
+        unconv_d.reserve(nband);
+        status.reserve(4);
+        hipLaunchKernelGGL(k_zero, dim3(gsz((size_t)2 * nbx * nbx)), dim3(256), 0, st(), (size_t)2 * nbx * nbx, vcc.get());
+        hipLaunchKernelGGL(k_zero, dim3(gsz((size_t)2 * nbx * nbx)), dim3(256), 0, st(), (size_t)2 * nbx * nbx, hcc.get());
+        APW_LAUNCH_CHECK();
+        APW_HIP(hipStreamSynchronize(st()));
+    }
+
+    double* S() { return scb ? sp.get() : basis.get(); }
+
+    void gemm(char ta, char tb, int m, int n, int k, double ar, const double* A, int lda, const double* Bm, int ldb,
+              double br, double* C, int ldc) {
+        apw::gemm<double>(st(), gws, ctx->num_cu, ta, tb, m, n, k, ar, 0.0, A, lda, Bm, ldb, br, 0.0, C, ldc);
+    }
+
+    // HX = H X[:, i0..i1]  (hpsi_func semantics: X = head of basis, HX = exact destination)
+    void apply_h(double* HX, int i0, int i1, int p) {
+        const int nb = i1 - i0 + 1;
+        hipEvent_t e0, e1;
+        APW_HIP(hipEventCreate(&e0));
+        APW_HIP(hipEventCreate(&e1));
+        APW_HIP(hipEventRecord(e0, st()));
+        if (ham) {
+            ham->hpsi(Bd(basis.get(), i0), dim, HX, dim, nb, p);
+        } else {
+            APW_HIP(hipStreamSynchronize(st()));
+            const int rc = hcb(user, basis.get(), HX, nbx, dim, i0, i1);
+            if (rc != 0) {
+                set_error("DiagoDavid: hpsi callback returned %d", rc);
+                throw Error{APW_ECALLBACK};
+            }
+        }
+        APW_HIP(hipEventRecord(e1, st()));
+        hev.emplace_back(e0, e1);
+        ++hpsi_calls;
+        hpsi_bands += nb;
+    }
+
+    void apply_s(int c0, int nc) {
+        if (!scb) return;
+        APW_HIP(hipStreamSynchronize(st()));
+        for (int m = 0; m < nc; ++m) {   // called as spsi(in, out, dim, dim, 1) per band (diago_david.cpp:189-228)
+            const int rc = scb(user, Bd(basis.get(), c0 + m), Bd(sp.get(), c0 + m), dim, dim, 1);
+            if (rc != 0) {
+                set_error("DiagoDavid: spsi callback returned %d", rc);
+                throw Error{APW_ECALLBACK};
+            }
+        }
+    }
+
+    // S-orthonormalise basis[:, c0 .. c0+nc) against basis[:, 0 .. c0) and among themselves
+    void orthonormalize(int c0, int nc) {
+        double* V = Bd(basis.get(), c0);
+        apply_s(c0, nc);
+        double* SV = Bd(S(), c0);
+        double* G = gmat.get();
+        gemm('C', 'N', nc, nc, dim, 1.0, V, dim, SV, dim, 0.0, G, nc);
+        if (c0 > 0) {
+            double* L = lmat.get();
+            gemm('C', 'N', c0, nc, dim, 1.0, basis.get(), dim, SV, dim, 0.0, L, c0);   // L = Q^H S V
+            gemm('C', 'N', nc, nc, c0, -1.0, L, c0, L, c0, 1.0, G, nc);                // G -= L^H L
+            gemm('N', 'N', dim, nc, c0, -1.0, basis.get(), dim, L, c0, 1.0, V, dim);   // V -= Q L
+        }
+        hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, st(), (size_t)2, reinterpret_cast<double*>(status.get()));
+        APW_LAUNCH_CHECK();
+        chol_inv_upper(st(), nc, G, nc, rinv.get(), status.get(), 1.0e-12);
+        gemm('N', 'N', dim, nc, nc, 1.0, V, dim, rinv.get(), nc, 0.0, tmp.get(), dim);   // tmp = V R^-1
+        hipLaunchKernelGGL(k_copy_cols, dim3(gsz((size_t)dim * nc)), dim3(256), 0, st(), dim, nc,
+                           reinterpret_cast<const double2*>(tmp.get()), dim, reinterpret_cast<double2*>(V), dim);
+        APW_LAUNCH_CHECK();
+        apply_s(c0, nc);
+    }
+
+    void check_status() {
+        int h[2] = {0, 0};
+        APW_HIP(hipMemcpyAsync(h, status.get(), sizeof(h), hipMemcpyDeviceToHost, st()));
+        APW_HIP(hipStreamSynchronize(st()));
+        if (h[0] != 0) {
+            set_error("DiagoDavid::SchmidtOrth: psi_norm < 1.0e-12 (vector %d of the block)", h[0] - 1);
+            throw Error{APW_EBREAKDOWN};
+        }
+    }
+
+    void solve_subspace(int nbase) {
+        heevx_lowest(st(), ews, ctx->num_cu, nbase, nbx, hcc.get(), nband, eig.get(), vcc.get(), nbx);
+    }
+
+    // hcc[nbase .. nbase+nc, 0 .. nbase+nc) = basis_new^H hpsi   (cal_elem, diago_david.cpp:626-717)
+    void cal_elem(int nbase, int nc) {
+        gemm('C', 'N', nc, nbase + nc, dim, 1.0, Bd(basis.get(), nbase), dim, hp.get(), dim, 0.0,
+             reinterpret_cast<double*>(reinterpret_cast<double2*>(hcc.get()) + nbase), nbx);
+    }
+
+    // one diag_once (diago_david.cpp:151-341).  switch_thr > 0: FP32 pre-phase that stops once
+    // the corrected bands' residual norms are all below switch_thr (or nothing is unconverged).
+    int diag_once(int ldpsi, double* psi, double* eig_host, double thr, int maxiter, int& notconv_out,
+                  double switch_thr) {
+        std::vector<int> unconv(nband);
+        for (int m = 0; m < nband; ++m) unconv[m] = m;
+        int notconv = nband;
+        // initial orthonormalisation of the start vectors (:186-228)
+        hipLaunchKernelGGL(k_copy_cols, dim3(gsz((size_t)dim * nband)), dim3(256), 0, st(), dim, nband,
+                           reinterpret_cast<const double2*>(psi), ldpsi, reinterpret_cast<double2*>(basis.get()), dim);
+        APW_LAUNCH_CHECK();
+        orthonormalize(0, nband);
+        apply_h(hp.get(), 0, nband - 1, prec);                                    // :233
+        cal_elem(0, nband);
+        int nbase = nband;
+        solve_subspace(nbase);
+        check_status();
+        APW_HIP(hipMemcpyAsync(eig_host, eig.get(), nband * sizeof(double), hipMemcpyDeviceToHost, st()));
+        APW_HIP(hipStreamSynchronize(st()));
+        std::vector<double> ehost(nband), res(nband);
+        int it = 0;
+        while (true) {
+            ++it;
+            // ---- cal_grad (:359-611) ----
+            APW_HIP(hipMemcpyAsync(unconv_d.get(), unconv.data(), notconv * sizeof(int), hipMemcpyHostToDevice, st()));
+            double2* vcp = reinterpret_cast<double2*>(vc.get());
+            hipLaunchKernelGGL(k_gather_vc, dim3(gsz((size_t)nbase * notconv)), dim3(256), 0, st(), nbase, notconv,
+                               reinterpret_cast<const double2*>(vcc.get()), nbx, unconv_d.get(), vcp);
+            double* NB = Bd(basis.get(), nbase);
+            gemm('N', 'N', dim, notconv, nbase, 1.0, hp.get(), dim, vc.get(), nbase, 0.0, NB, dim);
+            hipLaunchKernelGGL(k_scale_vc, dim3(gsz((size_t)nbase * notconv)), dim3(256), 0, st(), nbase, notconv, vcp,
+                               eig.get(), unconv_d.get());
+            gemm('N', 'N', dim, notconv, nbase, 1.0, S(), dim, vc.get(), nbase, 1.0, NB, dim);
+            if (switch_thr > 0) {
+                hipLaunchKernelGGL(k_colres2, dim3(notconv), dim3(256), 0, st(), dim,
+                                   reinterpret_cast<const double2*>(NB), dim, (const double2*)nullptr, 0,
+                                   (const double*)nullptr, resid.get());
+            }
+            hipLaunchKernelGGL(k_precond, dim3(gsz((size_t)dim * notconv)), dim3(256), 0, st(), dim, notconv,
+                               reinterpret_cast<double2*>(NB), pre.get());
+            APW_LAUNCH_CHECK();
+            orthonormalize(nbase, notconv);
+            apply_h(Bd(hp.get(), nbase), nbase, nbase + notconv - 1, prec);      // :604
+            // ---- cal_elem, diag_zhegvx ----
+            cal_elem(nbase, notconv);
+            const int nc_grad = notconv;
+            nbase += notconv;
+            solve_subspace(nbase);
+            check_status();
+            APW_HIP(hipMemcpyAsync(ehost.data(), eig.get(), nband * sizeof(double), hipMemcpyDeviceToHost, st()));
+            double rmax = 0;
+            if (switch_thr > 0) {
+                APW_HIP(hipMemcpyAsync(res.data(), resid.get(), nc_grad * sizeof(double), hipMemcpyDeviceToHost, st()));
+            }
+            APW_HIP(hipStreamSynchronize(st()));
+            for (int m = 0; m < nc_grad && switch_thr > 0; ++m) rmax = std::max(rmax, std::sqrt(res[m]));
+            // ---- convergence (:270-280) ----
+            notconv = 0;
+            for (int m = 0; m < nband; ++m) {
+                const bool conv = std::abs(ehost[m] - eig_host[m]) < thr;
+                if (!conv) unconv[notconv++] = m;
+                eig_host[m] = ehost[m];
+            }
+            const bool leave_mixed = switch_thr > 0 && (rmax < switch_thr || notconv == 0);
+            if (!notconv || nbase + notconv > nbx || it == maxiter || leave_mixed) {
+                // psi = basis vcc (:292-306)
+                gemm('N', 'N', dim, nband, nbase, 1.0, basis.get(), dim, vcc.get(), nbx, 0.0, psi, ldpsi);
+                if (ldpsi > dim) {
+                    hipLaunchKernelGGL(k_zero_rows, dim3(gsz((size_t)(ldpsi - dim) * nband)), dim3(256), 0, st(), dim,
+                                       ldpsi, nband, reinterpret_cast<double2*>(psi), ldpsi);
+                    APW_LAUNCH_CHECK();
+                }
+                if (!notconv || it == maxiter || leave_mixed) {
+                    // residual norms ||H psi_m - e_m psi_m|| of the returned vectors, from the
+                    // subspace (hpsi vcc - e psi): one GEMM, no extra H application
+                    gemm('N', 'N', dim, nband, nbase, 1.0, hp.get(), dim, vcc.get(), nbx, 0.0, tmp.get(), dim);
+                    hipLaunchKernelGGL(k_colres2, dim3(nband), dim3(256), 0, st(), dim,
+                                       reinterpret_cast<const double2*>(tmp.get()), dim,
+                                       reinterpret_cast<const double2*>(psi), ldpsi, eig.get(), resid.get());
+                    APW_LAUNCH_CHECK();
+                    std::vector<double> r2(nband);
+                    APW_HIP(hipMemcpyAsync(r2.data(), resid.get(), nband * sizeof(double), hipMemcpyDeviceToHost, st()));
+                    APW_HIP(hipStreamSynchronize(st()));
+                    last_resid = 0;
+                    for (double v : r2) last_resid = std::max(last_resid, std::sqrt(v));
+                    break;
+                }
+                // refresh (:797-934)
+                gemm('N', 'N', dim, nband, nbase, 1.0, hp.get(), dim, vcc.get(), nbx, 0.0, tmp.get(), dim);
+                hipLaunchKernelGGL(k_copy_cols, dim3(gsz((size_t)dim * nband)), dim3(256), 0, st(), dim, nband,
+                                   reinterpret_cast<const double2*>(tmp.get()), dim,
+                                   reinterpret_cast<double2*>(hp.get()), dim);
+                if (scb) {
+                    gemm('N', 'N', dim, nband, nbase, 1.0, sp.get(), dim, vcc.get(), nbx, 0.0, tmp.get(), dim);
+                    hipLaunchKernelGGL(k_copy_cols, dim3(gsz((size_t)dim * nband)), dim3(256), 0, st(), dim, nband,
+                                       reinterpret_cast<const double2*>(tmp.get()), dim,
+                                       reinterpret_cast<double2*>(sp.get()), dim);
+                }
+                hipLaunchKernelGGL(k_copy_cols, dim3(gsz((size_t)dim * nband)), dim3(256), 0, st(), dim, nband,
+                                   reinterpret_cast<const double2*>(psi), ldpsi,
+                                   reinterpret_cast<double2*>(basis.get()), dim);
+                hipLaunchKernelGGL(k_zero, dim3(gsz((size_t)2 * nbx * nbx)), dim3(256), 0, st(), (size_t)2 * nbx * nbx,
+                                   hcc.get());
+                hipLaunchKernelGGL(k_refresh_diag, dim3(1), dim3(256), 0, st(), nband,
+                                   reinterpret_cast<double2*>(hcc.get()), reinterpret_cast<double2*>(vcc.get()), nbx,
+                                   eig.get());
+                APW_LAUNCH_CHECK();
+                nbase = nband;
+            }
+        }
+        notconv_out = notconv;
+        return it;
+    }
+
+};
+
+}  // namespace apw
+
+struct apw_david {
+    apw::David d;
+};
+
+using apw::guarded;
+
+extern "C" {
+
+int apw_david_create(apw_ctx* ctx, const double* precondition, int nband, int dim, int david_ndim, apw_david** out) {
+    return guarded([&] {
+        APW_CHECK(ctx && out, "apw_david_create: null argument");
+        apw::DeviceGuard g(ctx->device);
+        auto* p = new apw_david();
+        try {
+            p->d.create(ctx, precondition, nband, dim, david_ndim);
+        } catch (...) {
+            delete p;
+            throw;
+        }
+        *out = p;
+        return APW_OK;
+    });
+}
+
+int apw_david_destroy(apw_david* dav) {
+    return guarded([&] {
+        if (!dav) return APW_OK;
+        apw::DeviceGuard g(dav->d.ctx->device);
+        (void)hipStreamSynchronize(dav->d.ctx->stream);
+        delete dav;
+        return APW_OK;
+    });
+}
+
+int apw_david_diag(apw_david* dav, apw_ham* ham, apw_hpsi_cb hpsi_cb, apw_spsi_cb spsi_cb, void* user, int ldpsi,
+                   double* psi, double* eig, double david_diag_thr, int david_maxiter, int ntry_max, int notconv_max,
+                   int precision, apw_diag_stats* stats) {
+    return guarded([&] {
+        APW_CHECK(dav && psi && eig, "apw_david_diag: null argument");
+        apw::David& D = dav->d;
+        APW_CHECK(ham || hpsi_cb, "apw_david_diag: need a Hamiltonian or an hpsi callback");
+        APW_CHECK(ldpsi >= D.dim, "apw_david_diag: ldpsi < dim");
+        APW_CHECK(precision == APW_FP64 || precision == APW_FP32_MIXED, "apw_david_diag: bad precision");
+        APW_CHECK(david_maxiter >= 1, "apw_david_diag: david_maxiter < 1");
+        if (ham) APW_CHECK(ham->h.npw == D.dim, "apw_david_diag: ham npw %d != dim %d", ham->h.npw, D.dim);
+        apw::DeviceGuard g(D.ctx->device);
+        D.ham = ham ? &ham->h : nullptr;
+        D.hcb = hpsi_cb;
+        D.scb = ham ? nullptr : spsi_cb;
+        D.user = user;
+        D.hpsi_calls = 0;
+        D.hpsi_bands = 0;
+        D.hev.clear();
+        if (D.scb) D.sp.reserve((size_t)2 * D.dim * D.nbx);
+        const auto t0 = std::chrono::steady_clock::now();
+        int total = 0, ntry = 0, notconv = 0, switch_iter = -1;
+        int rc = APW_OK;
+        try {
+            if (precision == APW_FP32_MIXED && ham) {
+                D.prec = APW_FP32_MIXED;
+                total += D.diag_once(ldpsi, psi, eig, david_diag_thr, david_maxiter, notconv, 1.0e-5);
+                switch_iter = total;
+            }
+            D.prec = APW_FP64;
+            do {
+                total += D.diag_once(ldpsi, psi, eig, david_diag_thr, david_maxiter, notconv, 0.0);
+                ++ntry;
+            } while (!(ntry > ntry_max || notconv <= notconv_max));   // check_block_conv (:1223-1240)
+            APW_HIP(hipStreamSynchronize(D.ctx->stream));
+        } catch (const apw::Error& e) {
+            rc = e.code;
+        }
+        const auto t1 = std::chrono::steady_clock::now();
+        double th = 0;
+        for (auto& pr : D.hev) {
+            float ms = 0;
+            if (rc == APW_OK && hipEventElapsedTime(&ms, pr.first, pr.second) == hipSuccess) th += ms * 1e-3;
+            (void)hipEventDestroy(pr.first);
+            (void)hipEventDestroy(pr.second);
+        }
+        D.hev.clear();
+        if (rc != APW_OK) return rc;
+        if (stats) {
+            stats->iterations = total;
+            stats->ntry = ntry;
+            stats->notconv = notconv;
+            stats->hpsi_calls = D.hpsi_calls;
+            stats->hpsi_bands = D.hpsi_bands;
+            stats->fp64_switch_iter = switch_iter;
+            stats->max_residual = D.last_resid;
+            stats->t_hpsi_s = th;
+            stats->t_total_s = std::chrono::duration<double>(t1 - t0).count();
+        }
+        return total;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,348 @@
+// Complex GEMM on CDNA4 matrix cores (v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32).
+//
+// Replaces gemm_op<std::complex<T>, DEVICE_GPU> (hipBLAS/cuBLAS Zgemm/Cgemm in the
+// reference, rocm/math_kernel_op.hip.cu:798) for the shapes of the path:
+//   * inner-product shapes  C(m x n) = A^H B with k = npw (up to ~2.3e5) and m, n <= ~2e3
+//     (becp = vkb^H psi, cal_elem, Gram matrices)  -> split-K with a deterministic reduction;
+//   * update shapes         C(npw x n) += A(npw x k) B(k x n) with k <= ~2e3
+//     (back-projection, cal_grad, refresh, final psi) -> plain 2D tiling.
+//
+// Tile: 64 (m) x 64 (n) per workgroup, k step 16, 4 waves in a 2 x 2 arrangement, each
+// wave owning a 32 x 32 block = 2 x 2 MFMA 16x16 sub-tiles with separate real and
+// imaginary accumulators (a complex product = 4 real MFMAs).  Operand roles are swapped
+// (MFMA "A" <- op(B) indexed by n, MFMA "B" <- op(A) indexed by m) so that the 16 lanes of
+// a group hold 16 consecutive rows m of one column n: the column-major stores of C are then
+// 256-byte coalesced runs.  Accumulator register r of lane l holds D row drow(l, r) and
+// column l % 16; drow is (l / 16) + 4 r for the f64 instruction and 4 (l / 16) + r for the
+// f32 one (checked on the device by the layout probe below and its unit test).
+#include "apw_internal.hpp"
+
+namespace apw {
+namespace {
+
+typedef double d4_t __attribute__((ext_vector_type(4)));
+typedef float f4_t __attribute__((ext_vector_type(4)));
+
+template <class R>
+struct Mfma;
+template <>
+struct Mfma<double> {
+    using acc_t = d4_t;
+    using c_t = double2;
+    __device__ static inline acc_t mma(double a, double b, acc_t c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    // row of D held in accumulator register r of a lane (column = lane % 16)
+    __device__ static inline int drow(int lane, int r) { return (lane >> 4) + 4 * r; }
+};
+template <>
+struct Mfma<float> {
+    using acc_t = f4_t;
+    using c_t = float2;
+    __device__ static inline acc_t mma(float a, float b, acc_t c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    __device__ static inline int drow(int lane, int r) { return 4 * (lane >> 4) + r; }
+};
+
+constexpr int BM = 64, BN = 64, BK = 16, NT = 256;
+
+// op codes: 0 = N, 1 = T, 2 = C
+template <class R, int TA, int TB>
+__global__ __launch_bounds__(NT) void zgemm_kernel(int M, int N, int K, int kchunk,
+                                                   const typename Mfma<R>::c_t* __restrict__ A, int lda,
+                                                   const typename Mfma<R>::c_t* __restrict__ B, int ldb,
+                                                   typename Mfma<R>::c_t* __restrict__ C, int ldc,
+                                                   double ar, double ai, double br, double bi,
+                                                   typename Mfma<R>::c_t* __restrict__ part) {
+    using c_t = typename Mfma<R>::c_t;
+    using acc_t = typename Mfma<R>::acc_t;
+    __shared__ R As_re[BK][BM], As_im[BK][BM], Bs_re[BK][BN], Bs_im[BK][BN];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
+    const int kbeg = blockIdx.z * kchunk;
+    const int kend = min(K, kbeg + kchunk);
+
+    acc_t acc_re[2][2], acc_im[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            acc_re[a][b] = acc_t{0, 0, 0, 0};
+            acc_im[a][b] = acc_t{0, 0, 0, 0};
+        }
+
+    for (int k0 = kbeg; k0 < kend; k0 += BK) {
+        c_t ra[4], rb[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int e = tid + NT * i;
+            int mm, kk;
+            if (TA == 0) { mm = e & 63; kk = e >> 6; } else { kk = e & 15; mm = e >> 4; }
+            const int gm = m0 + mm, gk = k0 + kk;
+            c_t v = c_t{0, 0};
+            if (gm < M && gk < kend) {
+                v = (TA == 0) ? A[(size_t)gm + (size_t)gk * lda] : A[(size_t)gk + (size_t)gm * lda];
+                if (TA == 2) v.y = -v.y;
+            }
+            ra[i] = v;
+            int nn;
+            if (TB == 0) { kk = e & 15; nn = e >> 4; } else { nn = e & 63; kk = e >> 6; }
+            const int gn = n0 + nn;
+            const int gk2 = k0 + kk;
+            c_t w = c_t{0, 0};
+            if (gn < N && gk2 < kend) {
+                w = (TB == 0) ? B[(size_t)gk2 + (size_t)gn * ldb] : B[(size_t)gn + (size_t)gk2 * ldb];
+                if (TB == 2) w.y = -w.y;
+            }
+            rb[i] = w;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int e = tid + NT * i;
+            int mm, kk;
+            if (TA == 0) { mm = e & 63; kk = e >> 6; } else { kk = e & 15; mm = e >> 4; }
+            As_re[kk][mm] = ra[i].x;
+            As_im[kk][mm] = ra[i].y;
+            int nn;
+            if (TB == 0) { kk = e & 15; nn = e >> 4; } else { nn = e & 63; kk = e >> 6; }
+            Bs_re[kk][nn] = rb[i].x;
+            Bs_im[kk][nn] = rb[i].y;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < BK / 4; ++ks) {
+            const int kk = ks * 4 + (lane >> 4);
+            R xr[2], xi[2], yr[2], yi[2];
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                const int nn = wn * 32 + t * 16 + (lane & 15);
+                xr[t] = Bs_re[kk][nn];
+                xi[t] = Bs_im[kk][nn];
+                const int mm = wm * 32 + t * 16 + (lane & 15);
+                yr[t] = As_re[kk][mm];
+                yi[t] = As_im[kk][mm];
+            }
+#pragma unroll
+            for (int tn = 0; tn < 2; ++tn)
+#pragma unroll
+                for (int tm = 0; tm < 2; ++tm) {
+                    acc_re[tn][tm] = Mfma<R>::mma(xr[tn], yr[tm], acc_re[tn][tm]);
+                    acc_re[tn][tm] = Mfma<R>::mma(-xi[tn], yi[tm], acc_re[tn][tm]);
+                    acc_im[tn][tm] = Mfma<R>::mma(xr[tn], yi[tm], acc_im[tn][tm]);
+                    acc_im[tn][tm] = Mfma<R>::mma(xi[tn], yr[tm], acc_im[tn][tm]);
+                }
+        }
+    }
+
+    // epilogue: lane holds D[n = drow(lane, r)][m = lane%16] of each sub-tile
+#pragma unroll
+    for (int tn = 0; tn < 2; ++tn)
+#pragma unroll
+        for (int tm = 0; tm < 2; ++tm)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int gm = m0 + wm * 32 + tm * 16 + (lane & 15);
+                const int gn = n0 + wn * 32 + tn * 16 + Mfma<R>::drow(lane, r);
+                if (gm >= M || gn >= N) continue;
+                const double xr = (double)acc_re[tn][tm][r];
+                const double xi = (double)acc_im[tn][tm][r];
+                if (part) {
+                    c_t v;
+                    v.x = (R)xr;
+                    v.y = (R)xi;
+                    part[(size_t)blockIdx.z * M * N + gm + (size_t)gn * M] = v;
+                } else {
+                    double cr = ar * xr - ai * xi;
+                    double ci = ar * xi + ai * xr;
+                    c_t* cp = C + gm + (size_t)gn * ldc;
+                    if (br != 0.0 || bi != 0.0) {
+                        const c_t c = *cp;
+                        cr += br * (double)c.x - bi * (double)c.y;
+                        ci += br * (double)c.y + bi * (double)c.x;
+                    }
+                    c_t v;
+                    v.x = (R)cr;
+                    v.y = (R)ci;
+                    *cp = v;
+                }
+            }
+}
+
+// C = alpha * sum_s part[s] + beta * C  (fixed summation order: deterministic)
+template <class R>
+__global__ void splitk_reduce(int M, int N, int S, const typename Mfma<R>::c_t* __restrict__ part,
+                              typename Mfma<R>::c_t* __restrict__ C, int ldc, double ar, double ai,
+                              double br, double bi) {
+    using c_t = typename Mfma<R>::c_t;
+    const size_t total = (size_t)M * N;
+    for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < total;
+         idx += (size_t)gridDim.x * blockDim.x) {
+        R sr = 0, si = 0;
+        for (int s = 0; s < S; ++s) {
+            const c_t v = part[(size_t)s * total + idx];
+            sr += v.x;
+            si += v.y;
+        }
+        const int m = (int)(idx % M);
+        const int n = (int)(idx / M);
+        double cr = ar * (double)sr - ai * (double)si;
+        double ci = ar * (double)si + ai * (double)sr;
+        c_t* cp = C + m + (size_t)n * ldc;
+        if (br != 0.0 || bi != 0.0) {
+            const c_t c = *cp;
+            cr += br * (double)c.x - bi * (double)c.y;
+            ci += br * (double)c.y + bi * (double)c.x;
+        }
+        c_t v;
+        v.x = (R)cr;
+        v.y = (R)ci;
+        *cp = v;
+    }
+}
+
+template <class R>
+__global__ void scale_c(int M, int N, typename Mfma<R>::c_t* __restrict__ C, int ldc, double br, double bi) {
+    using c_t = typename Mfma<R>::c_t;
+    const size_t total = (size_t)M * N;
+    for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < total;
+         idx += (size_t)gridDim.x * blockDim.x) {
+        const int m = (int)(idx % M), n = (int)(idx / M);
+        c_t* cp = C + m + (size_t)n * ldc;
+        c_t v;
+        if (br == 0.0 && bi == 0.0) {
+            v.x = 0;
+            v.y = 0;
+        } else {
+            const c_t c = *cp;
+            v.x = (R)(br * c.x - bi * c.y);
+            v.y = (R)(br * c.y + bi * c.x);
+        }
+        *cp = v;
+    }
+}
+
+int opcode(char t) {
+    switch (t) {
+        case 'N': case 'n': return 0;
+        case 'T': case 't': return 1;
+        case 'C': case 'c': return 2;
+        default: return -1;
+    }
+}
+
+template <class R, int TA, int TB>
+void launch(hipStream_t s, dim3 grid, int M, int N, int K, int kchunk, const R* A, int lda, const R* B,
+            int ldb, R* C, int ldc, double ar, double ai, double br, double bi, R* part) {
+    using c_t = typename Mfma<R>::c_t;
+    hipLaunchKernelGGL((zgemm_kernel<R, TA, TB>), grid, dim3(NT), 0, s, M, N, K, kchunk,
+                       reinterpret_cast<const c_t*>(A), lda, reinterpret_cast<const c_t*>(B), ldb,
+                       reinterpret_cast<c_t*>(C), ldc, ar, ai, br, bi, reinterpret_cast<c_t*>(part));
+}
+
+}  // namespace
+
+template <class R>
+void gemm(hipStream_t s, GemmWorkspace& ws, int num_cu, char ta, char tb, int m, int n, int k,
+          double alpha_re, double alpha_im, const R* A, int lda, const R* B, int ldb,
+          double beta_re, double beta_im, R* C, int ldc) {
+    using c_t = typename Mfma<R>::c_t;
+    const int oa = opcode(ta), ob = opcode(tb);
+    APW_CHECK(oa >= 0 && ob >= 0, "gemm: bad trans '%c' '%c'", ta, tb);
+    APW_CHECK(m >= 0 && n >= 0 && k >= 0, "gemm: negative size");
+    if (m == 0 || n == 0) return;
+    APW_CHECK(ldc >= m, "gemm: ldc < m");
+    APW_CHECK(lda >= (oa == 0 ? m : k) || k == 0, "gemm: lda too small");
+    APW_CHECK(ldb >= (ob == 0 ? k : n) || k == 0, "gemm: ldb too small");
+    if (k == 0 || (alpha_re == 0.0 && alpha_im == 0.0)) {
+        const size_t total = (size_t)m * n;
+        const int blocks = (int)std::min<size_t>((total + 255) / 256, 65536);
+        hipLaunchKernelGGL(scale_c<R>, dim3(blocks), dim3(256), 0, s, m, n, reinterpret_cast<c_t*>(C), ldc,
+                           beta_re, beta_im);
+        APW_LAUNCH_CHECK();
+        return;
+    }
+    const int tiles_m = (m + BM - 1) / BM, tiles_n = (n + BN - 1) / BN;
+    const long tiles = (long)tiles_m * tiles_n;
+    const int target = 2 * std::max(num_cu, 64);
+    int splits = 1;
+    if (tiles < target) {
+        splits = (int)((target + tiles - 1) / tiles);
+        const int kmax_splits = std::max(1, k / 512);
+        splits = std::min(splits, kmax_splits);
+        splits = std::min(splits, 256);
+    }
+    int kchunk = (k + splits - 1) / splits;
+    kchunk = (kchunk + BK - 1) / BK * BK;
+    splits = (k + kchunk - 1) / kchunk;
+    R* part = nullptr;
+    if (splits > 1) {
+        const size_t need = (size_t)splits * m * n * 2 * sizeof(R) / sizeof(double) + 2;
+        ws.part.reserve(need);
+        part = reinterpret_cast<R*>(ws.part.get());
+    }
+    dim3 grid(tiles_m, tiles_n, splits);
+#define APW_GEMM_CASE(X, Y)                                                                            \
+    if (oa == X && ob == Y) {                                                                         \
+        launch<R, X, Y>(s, grid, m, n, k, kchunk, A, lda, B, ldb, C, ldc, alpha_re, alpha_im, beta_re, \
+                        beta_im, part);                                                               \
+    }
+    APW_GEMM_CASE(0, 0) APW_GEMM_CASE(0, 1) APW_GEMM_CASE(0, 2)
+    APW_GEMM_CASE(1, 0) APW_GEMM_CASE(1, 1) APW_GEMM_CASE(1, 2)
+    APW_GEMM_CASE(2, 0) APW_GEMM_CASE(2, 1) APW_GEMM_CASE(2, 2)
+#undef APW_GEMM_CASE
+    APW_LAUNCH_CHECK();
+    if (splits > 1) {
+        const size_t total = (size_t)m * n;
+        const int blocks = (int)std::min<size_t>((total + 255) / 256, 65536);
+        hipLaunchKernelGGL(splitk_reduce<R>, dim3(blocks), dim3(256), 0, s, m, n, splits,
+                           reinterpret_cast<const c_t*>(part), reinterpret_cast<c_t*>(C), ldc, alpha_re,
+                           alpha_im, beta_re, beta_im);
+        APW_LAUNCH_CHECK();
+    }
+}
+
+template void gemm<double>(hipStream_t, GemmWorkspace&, int, char, char, int, int, int, double, double,
+                           const double*, int, const double*, int, double, double, double*, int);
+template void gemm<float>(hipStream_t, GemmWorkspace&, int, char, char, int, int, int, double, double,
+                          const float*, int, const float*, int, double, double, float*, int);
+
+// ---------------------------------------------------------------- layout probe
+// Records, for every (lane, accumulator register) of one v_mfma_f64_16x16x4f64 and one
+// v_mfma_f32_16x16x4f32, the (row, column) of D it holds.  Used by the unit tests to
+// check the operand/accumulator layout the GEMM relies on.
+__global__ void probe_mfma_kernel(double* out) {
+    const int l = threadIdx.x;
+    const double row_sel = (l >> 4) == 0 ? 1.0 : 0.0;
+    d4_t z{0, 0, 0, 0};
+    // D[i][j] = A[i][0]: A = row index, B[0][j] = 1
+    d4_t r = __builtin_amdgcn_mfma_f64_16x16x4f64((double)(l & 15), row_sel, z, 0, 0, 0);
+    // D[i][j] = B[0][j]: A[i][0] = 1, B = column index
+    d4_t c = __builtin_amdgcn_mfma_f64_16x16x4f64(row_sel, (double)(l & 15), z, 0, 0, 0);
+    f4_t zf{0, 0, 0, 0};
+    f4_t rf = __builtin_amdgcn_mfma_f32_16x16x4f32((float)(l & 15), (float)row_sel, zf, 0, 0, 0);
+    f4_t cf = __builtin_amdgcn_mfma_f32_16x16x4f32((float)row_sel, (float)(l & 15), zf, 0, 0, 0);
+    for (int q = 0; q < 4; ++q) {
+        out[(0 * 64 + l) * 4 + q] = r[q];
+        out[(1 * 64 + l) * 4 + q] = c[q];
+        out[(2 * 64 + l) * 4 + q] = rf[q];
+        out[(3 * 64 + l) * 4 + q] = cf[q];
+    }
+}
+
+}  // namespace apw
+
+extern "C" int apw_probe_mfma_layout(apw_ctx* ctx, double* out_dev) {
+    return apw::guarded([&] {
+        APW_CHECK(ctx && out_dev, "apw_probe_mfma_layout: null argument");
+        apw::DeviceGuard g(ctx->device);
+        hipLaunchKernelGGL(apw::probe_mfma_kernel, dim3(1), dim3(64), 0, ctx->stream, out_dev);
+        APW_LAUNCH_CHECK();
+        return APW_OK;
+    });
+}

@@ -1,0 +1,605 @@
+// Plane-wave 3D FFT passes for the H|psi> path (gfx950).
+//
+// Replaces, per band, the reference GPU sequence (pw_transform_k.cpp:336-476):
+//   memset(box) -> set_3d_fft_box -> hipfftExecZ2Z(BACKWARD) -> copy -> veff_pw_op ->
+//   copy -> hipfftExecZ2Z(FORWARD) -> set_real_to_recip_output
+// with five batched passes that only touch the part of the box the sphere reaches:
+//   1. z-backward on the ns sticks, gathering psi straight from the sphere;
+//   2. y-backward on the x columns that hold sticks (nxr of nx), scattering the sticks;
+//   3. x-backward, multiply by veff, x-forward, in one kernel (the only pass over the
+//      full grid; the real-space box never goes to HBM);
+//   4. y-forward, keeping the stick rows;
+//   5. z-forward on the sticks, scaled by 1/nxyz and gathered into hpsi, with the kinetic
+//      term gk2*tpiba2*psi fused in.
+// Each pass is a batch of 1D transforms done in LDS by a mixed-radix (4, 2, 3, 5)
+// Stockham FFT (self-sorting, so no bit reversal); lines are interleaved in LDS so the
+// global loads/stores are runs of consecutive z (16 x 16 B) or whole sticks.
+// Twiddles exp(-2 pi i k / n) are computed on the host in long double.
+#include "pw_fft.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <type_traits>
+
+namespace apw {
+
+FftPlan make_plan(int n) {
+    FftPlan p;
+    p.n = n;
+    int m = n;
+    while (m % 4 == 0) { p.rad[p.nst++] = 4; m /= 4; }
+    while (m % 2 == 0) { p.rad[p.nst++] = 2; m /= 2; }
+    while (m % 3 == 0) { p.rad[p.nst++] = 3; m /= 3; }
+    while (m % 5 == 0) { p.rad[p.nst++] = 5; m /= 5; }
+    APW_CHECK(m == 1 && n > 0, "FFT length %d is not 2^a 3^b 5^c", n);
+    APW_CHECK(p.nst <= 16, "FFT length %d: too many stages", n);
+    return p;
+}
+
+namespace {
+
+template <class R>
+using cplx = typename std::conditional<std::is_same<R, double>::value, double2, float2>::type;
+
+template <class C>
+__device__ inline C cmul(C a, C b) {
+    C r;
+    r.x = a.x * b.x - a.y * b.y;
+    r.y = a.x * b.y + a.y * b.x;
+    return r;
+}
+template <class C>
+__device__ inline C cadd(C a, C b) { C r; r.x = a.x + b.x; r.y = a.y + b.y; return r; }
+template <class C>
+__device__ inline C csub(C a, C b) { C r; r.x = a.x - b.x; r.y = a.y - b.y; return r; }
+// multiply by sgn * i
+template <class C>
+__device__ inline C cmuli(C a, int sgn) {
+    C r;
+    r.x = -sgn * a.y;
+    r.y = sgn * a.x;
+    return r;
+}
+
+template <class C, class R>
+__device__ inline void dft_radix(C* v, int rdx, int sgn) {
+    if (rdx == 2) {
+        C a = v[0], b = v[1];
+        v[0] = cadd(a, b);
+        v[1] = csub(a, b);
+    } else if (rdx == 4) {
+        C t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+        C t2 = cadd(v[1], v[3]), t3 = cmuli(csub(v[1], v[3]), sgn);
+        v[0] = cadd(t0, t2);
+        v[2] = csub(t0, t2);
+        v[1] = cadd(t1, t3);
+        v[3] = csub(t1, t3);
+    } else if (rdx == 3) {
+        const R h = (R)0.86602540378443864676;   // sqrt(3)/2
+        C t = cadd(v[1], v[2]);
+        C u = csub(v[1], v[2]);
+        C a = v[0];
+        C m;
+        m.x = a.x - (R)0.5 * t.x;
+        m.y = a.y - (R)0.5 * t.y;
+        C w = cmuli(u, sgn);
+        w.x *= h;
+        w.y *= h;
+        v[0] = cadd(a, t);
+        v[1] = cadd(m, w);
+        v[2] = csub(m, w);
+    } else {  // 5
+        const R c1 = (R)0.30901699437494742410, c2 = (R)-0.80901699437494742410;
+        const R s1 = (R)0.95105651629515357212, s2 = (R)0.58778525229247312917;
+        C a = v[0];
+        C t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
+        C u1 = csub(v[1], v[4]), u2 = csub(v[2], v[3]);
+        C m1, m2, q1, q2;
+        m1.x = a.x + c1 * t1.x + c2 * t2.x;
+        m1.y = a.y + c1 * t1.y + c2 * t2.y;
+        m2.x = a.x + c2 * t1.x + c1 * t2.x;
+        m2.y = a.y + c2 * t1.y + c1 * t2.y;
+        q1.x = s1 * u1.x + s2 * u2.x;
+        q1.y = s1 * u1.y + s2 * u2.y;
+        q2.x = s2 * u1.x - s1 * u2.x;
+        q2.y = s2 * u1.y - s1 * u2.y;
+        q1 = cmuli(q1, sgn);
+        q2 = cmuli(q2, sgn);
+        v[0] = cadd(a, cadd(t1, t2));
+        v[1] = cadd(m1, q1);
+        v[4] = csub(m1, q1);
+        v[2] = cadd(m2, q2);
+        v[3] = csub(m2, q2);
+    }
+}
+
+// In-LDS Stockham FFT of L interleaved lines of length p.n: element k of line l at
+// x[k * ld + l].  sgn = +1 backward (exp +i), -1 forward.  tw: exp(-2 pi i j / n), j < n.
+// Returns the buffer (x or y) that holds the result.  All threads of the block call it.
+template <class R>
+__device__ cplx<R>* lds_fft(cplx<R>* x, cplx<R>* y, const cplx<R>* tw, const FftPlan& p, int L, int ld, int sgn) {
+    using C = cplx<R>;
+    const int n = p.n;
+    int Ns = 1;
+    for (int st = 0; st < p.nst; ++st) {
+        const int rdx = p.rad[st];
+        const int nb = n / rdx;
+        const int stride = n / (Ns * rdx);
+        for (int b = threadIdx.x; b < nb * L; b += blockDim.x) {
+            const int l = b % L;
+            const int j = b / L;
+            const int jm = j % Ns;
+            C v[5];
+            for (int r = 0; r < rdx; ++r) v[r] = x[(j + r * nb) * ld + l];
+            if (Ns > 1) {
+                for (int r = 1; r < rdx; ++r) {
+                    C t = tw[(r * jm * stride) % n];
+                    if (sgn > 0) t.y = -t.y;
+                    v[r] = cmul(v[r], t);
+                }
+            }
+            dft_radix<C, R>(v, rdx, sgn);
+            const int base = (j / Ns) * Ns * rdx + jm;
+            for (int r = 0; r < rdx; ++r) y[(base + r * Ns) * ld + l] = v[r];
+        }
+        __syncthreads();
+        C* t = x;
+        x = y;
+        y = t;
+        Ns *= rdx;
+    }
+    return x;
+}
+
+template <class R>
+__device__ inline void load_tw(cplx<R>* tws, const cplx<R>* tw, int n) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) tws[i] = tw[i];
+}
+
+template <class R>
+__device__ inline cplx<R> to_c(double2 v) {
+    cplx<R> r;
+    r.x = (R)v.x;
+    r.y = (R)v.y;
+    return r;
+}
+
+constexpr int NT = 256;
+
+// ---- pass 1: z-backward on sticks, gathered from the sphere --------------------------
+// grid (ceil(ns/L), nb); LDS: 2 * nz * (L+1) + nz complex
+template <class R>
+__global__ __launch_bounds__(NT) void k_zbwd(const double2* __restrict__ psi, int ld, const int* __restrict__ stick_ig,
+                                             int ns, cplx<R>* __restrict__ S, FftPlan pz,
+                                             const cplx<R>* __restrict__ tw, int L) {
+    using C = cplx<R>;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int nz = pz.n;
+    const int ldl = L + 1;
+    C* x = reinterpret_cast<C*>(smem);
+    C* y = x + nz * ldl;
+    C* tws = y + nz * ldl;
+    const int band = blockIdx.y;
+    const int s0 = blockIdx.x * L;
+    const double2* pb = psi + (size_t)band * ld;
+    load_tw<R>(tws, tw, nz);
+    for (int t = threadIdx.x; t < L * nz; t += blockDim.x) {
+        const int l = t / nz, iz = t - (t / nz) * nz;
+        const int s = s0 + l;
+        C v;
+        v.x = 0;
+        v.y = 0;
+        if (s < ns) {
+            const int ig = stick_ig[(size_t)s * nz + iz];
+            if (ig >= 0) v = to_c<R>(pb[ig]);
+        }
+        x[iz * ldl + l] = v;
+    }
+    __syncthreads();
+    C* res = lds_fft<R>(x, y, tws, pz, L, ldl, +1);
+    C* Sb = S + (size_t)band * ns * nz;
+    for (int t = threadIdx.x; t < L * nz; t += blockDim.x) {
+        const int l = t / nz, iz = t - (t / nz) * nz;
+        const int s = s0 + l;
+        if (s < ns) Sb[(size_t)s * nz + iz] = res[iz * ldl + l];
+    }
+}
+
+// ---- pass 2: y-backward on the x slots -----------------------------------------------
+// grid (nxr, ceil(nz/L), nb); LDS 2 * ny * (L+1) + ny
+template <class R>
+__global__ __launch_bounds__(NT) void k_ybwd(const cplx<R>* __restrict__ S, int ns, int nz, int nxr,
+                                             const int* __restrict__ slot_start, const int* __restrict__ slot_cnt,
+                                             const int* __restrict__ stick_iy, cplx<R>* __restrict__ X, FftPlan py,
+                                             const cplx<R>* __restrict__ tw, int L) {
+    using C = cplx<R>;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int ny = py.n;
+    const int ldl = L + 1;
+    C* x = reinterpret_cast<C*>(smem);
+    C* y = x + ny * ldl;
+    C* tws = y + ny * ldl;
+    const int slot = blockIdx.x;
+    const int iz0 = blockIdx.y * L;
+    const int band = blockIdx.z;
+    load_tw<R>(tws, tw, ny);
+    for (int t = threadIdx.x; t < ny * ldl; t += blockDim.x) {
+        C z;
+        z.x = 0;
+        z.y = 0;
+        x[t] = z;
+    }
+    __syncthreads();
+    const int st0 = slot_start[slot], cnt = slot_cnt[slot];
+    const C* Sb = S + (size_t)band * ns * nz;
+    for (int t = threadIdx.x; t < cnt * L; t += blockDim.x) {
+        const int q = t / L, dz = t - (t / L) * L;
+        const int iz = iz0 + dz;
+        const int s = st0 + q;
+        if (iz < nz) x[stick_iy[s] * ldl + dz] = Sb[(size_t)s * nz + iz];
+    }
+    __syncthreads();
+    C* res = lds_fft<R>(x, y, tws, py, L, ldl, +1);
+    C* Xb = X + ((size_t)band * nxr + slot) * ny * nz;
+    for (int t = threadIdx.x; t < ny * L; t += blockDim.x) {
+        const int iy = t / L, dz = t - (t / L) * L;
+        const int iz = iz0 + dz;
+        if (iz < nz) Xb[(size_t)iy * nz + iz] = res[iy * ldl + dz];
+    }
+}
+
+// ---- pass 3: x pass.  MODE 0: bwd, *veff, fwd (X -> X); 1: bwd, write full box; 2: read full box, fwd
+// grid (ny, ceil(nz/L), nb); LDS 2 * nx * (L+1) + nx
+template <class R, int MODE>
+__global__ __launch_bounds__(NT) void k_xpass(cplx<R>* __restrict__ X, int nxr, int ny, int nz,
+                                              const int* __restrict__ slot_ix, const R* __restrict__ veff,
+                                              double2* __restrict__ box, int add, double factor, FftPlan px,
+                                              const cplx<R>* __restrict__ tw, int L) {
+    using C = cplx<R>;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int nx = px.n;
+    const int ldl = L + 1;
+    C* x = reinterpret_cast<C*>(smem);
+    C* y = x + nx * ldl;
+    C* tws = y + nx * ldl;
+    const int iy = blockIdx.x;
+    const int iz0 = blockIdx.y * L;
+    const int band = blockIdx.z;
+    load_tw<R>(tws, tw, nx);
+    C* Xb = X + (size_t)band * nxr * ny * nz;
+    const size_t nxyz = (size_t)nx * ny * nz;
+    double2* boxb = box ? box + (size_t)band * nxyz : nullptr;
+    if (MODE == 2) {
+        for (int t = threadIdx.x; t < nx * L; t += blockDim.x) {
+            const int ix = t / L, dz = t - (t / L) * L;
+            const int iz = iz0 + dz;
+            C v;
+            v.x = 0;
+            v.y = 0;
+            if (iz < nz) v = to_c<R>(boxb[((size_t)ix * ny + iy) * nz + iz]);
+            x[ix * ldl + dz] = v;
+        }
+    } else {
+        for (int t = threadIdx.x; t < nx * ldl; t += blockDim.x) {
+            C z;
+            z.x = 0;
+            z.y = 0;
+            x[t] = z;
+        }
+        __syncthreads();
+        for (int t = threadIdx.x; t < nxr * L; t += blockDim.x) {
+            const int q = t / L, dz = t - (t / L) * L;
+            const int iz = iz0 + dz;
+            if (iz < nz) x[slot_ix[q] * ldl + dz] = Xb[((size_t)q * ny + iy) * nz + iz];
+        }
+    }
+    __syncthreads();
+    C* res = x;
+    if (MODE != 2) res = lds_fft<R>(x, y, tws, px, L, ldl, +1);
+    if (MODE == 1) {
+        for (int t = threadIdx.x; t < nx * L; t += blockDim.x) {
+            const int ix = t / L, dz = t - (t / L) * L;
+            const int iz = iz0 + dz;
+            if (iz < nz) {
+                const C v = res[ix * ldl + dz];
+                double2* bp = boxb + ((size_t)ix * ny + iy) * nz + iz;
+                double2 o;
+                if (add) {
+                    o = *bp;
+                    o.x += factor * (double)v.x;
+                    o.y += factor * (double)v.y;
+                } else {
+                    o.x = (double)v.x;
+                    o.y = (double)v.y;
+                }
+                *bp = o;
+            }
+        }
+        return;
+    }
+    if (MODE == 0) {
+        for (int t = threadIdx.x; t < nx * L; t += blockDim.x) {
+            const int ix = t / L, dz = t - (t / L) * L;
+            const int iz = iz0 + dz;
+            if (iz < nz) {
+                const R vr = veff[((size_t)ix * ny + iy) * nz + iz];
+                C v = res[ix * ldl + dz];
+                v.x *= vr;
+                v.y *= vr;
+                res[ix * ldl + dz] = v;
+            }
+        }
+        __syncthreads();
+    }
+    C* other = (res == x) ? y : x;
+    C* out = lds_fft<R>(res, other, tws, px, L, ldl, -1);
+    for (int t = threadIdx.x; t < nxr * L; t += blockDim.x) {
+        const int q = t / L, dz = t - (t / L) * L;
+        const int iz = iz0 + dz;
+        if (iz < nz) Xb[((size_t)q * ny + iy) * nz + iz] = out[slot_ix[q] * ldl + dz];
+    }
+}
+
+// ---- pass 4: y-forward, keeping the stick rows -----------------------------------------
+template <class R>
+__global__ __launch_bounds__(NT) void k_yfwd(const cplx<R>* __restrict__ X, int ns, int nz, int nxr,
+                                             const int* __restrict__ slot_start, const int* __restrict__ slot_cnt,
+                                             const int* __restrict__ stick_iy, cplx<R>* __restrict__ S, FftPlan py,
+                                             const cplx<R>* __restrict__ tw, int L) {
+    using C = cplx<R>;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int ny = py.n;
+    const int ldl = L + 1;
+    C* x = reinterpret_cast<C*>(smem);
+    C* y = x + ny * ldl;
+    C* tws = y + ny * ldl;
+    const int slot = blockIdx.x;
+    const int iz0 = blockIdx.y * L;
+    const int band = blockIdx.z;
+    load_tw<R>(tws, tw, ny);
+    const C* Xb = X + ((size_t)band * nxr + slot) * ny * nz;
+    for (int t = threadIdx.x; t < ny * L; t += blockDim.x) {
+        const int iy = t / L, dz = t - (t / L) * L;
+        const int iz = iz0 + dz;
+        C v;
+        v.x = 0;
+        v.y = 0;
+        if (iz < nz) v = Xb[(size_t)iy * nz + iz];
+        x[iy * ldl + dz] = v;
+    }
+    __syncthreads();
+    C* res = lds_fft<R>(x, y, tws, py, L, ldl, -1);
+    const int st0 = slot_start[slot], cnt = slot_cnt[slot];
+    C* Sb = S + (size_t)band * ns * nz;
+    for (int t = threadIdx.x; t < cnt * L; t += blockDim.x) {
+        const int q = t / L, dz = t - (t / L) * L;
+        const int iz = iz0 + dz;
+        const int s = st0 + q;
+        if (iz < nz) Sb[(size_t)s * nz + iz] = res[stick_iy[s] * ldl + dz];
+    }
+}
+
+// ---- pass 5: z-forward on sticks, gathered into hpsi ----------------------------------
+template <class R>
+__global__ __launch_bounds__(NT) void k_zfwd(const cplx<R>* __restrict__ S, const int* __restrict__ stick_ig, int ns,
+                                             double2* __restrict__ hpsi, int ldh, const double2* __restrict__ psi,
+                                             int ld, const double* __restrict__ gk2, double tpiba2, int mode,
+                                             double scale, FftPlan pz, const cplx<R>* __restrict__ tw, int L) {
+    using C = cplx<R>;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int nz = pz.n;
+    const int ldl = L + 1;
+    C* x = reinterpret_cast<C*>(smem);
+    C* y = x + nz * ldl;
+    C* tws = y + nz * ldl;
+    const int band = blockIdx.y;
+    const int s0 = blockIdx.x * L;
+    load_tw<R>(tws, tw, nz);
+    const C* Sb = S + (size_t)band * ns * nz;
+    for (int t = threadIdx.x; t < L * nz; t += blockDim.x) {
+        const int l = t / nz, iz = t - (t / nz) * nz;
+        const int s = s0 + l;
+        C v;
+        v.x = 0;
+        v.y = 0;
+        if (s < ns) v = Sb[(size_t)s * nz + iz];
+        x[iz * ldl + l] = v;
+    }
+    __syncthreads();
+    C* res = lds_fft<R>(x, y, tws, pz, L, ldl, -1);
+    double2* hb = hpsi + (size_t)band * ldh;
+    const double2* pb = psi ? psi + (size_t)band * ld : nullptr;
+    for (int t = threadIdx.x; t < L * nz; t += blockDim.x) {
+        const int l = t / nz, iz = t - (t / nz) * nz;
+        const int s = s0 + l;
+        if (s >= ns) continue;
+        const int ig = stick_ig[(size_t)s * nz + iz];
+        if (ig < 0) continue;
+        const C v = res[iz * ldl + l];
+        double2 o;
+        o.x = scale * (double)v.x;
+        o.y = scale * (double)v.y;
+        if (mode == GATHER_ADD) {
+            const double2 h = hb[ig];
+            o.x += h.x;
+            o.y += h.y;
+        } else if (mode == GATHER_SET_KIN) {
+            const double kin = gk2[ig] * tpiba2;
+            const double2 p = pb[ig];
+            o.x += kin * p.x;
+            o.y += kin * p.y;
+        }
+        hb[ig] = o;
+    }
+}
+
+int lines_for(int n) {
+    // 2 buffers of n * (L+1) complex doubles within ~64 KB of LDS
+    int L = 16;
+    while (L > 1 && 2 * (size_t)n * (L + 1) * 16 > 66000) L /= 2;
+    return L;
+}
+
+template <class R>
+size_t lds_bytes(int n, int L) {
+    return (2 * (size_t)n * (L + 1) + n) * sizeof(cplx<R>);
+}
+
+template <class R>
+const cplx<R>* tw_ptr(const PwGrid& g, int off) {
+    if (std::is_same<R, double>::value) return reinterpret_cast<const cplx<R>*>(g.tw64.get() + 2 * off);
+    return reinterpret_cast<const cplx<R>*>(g.tw32.get() + 2 * off);
+}
+
+template <class R>
+void pass_forward_half(hipStream_t s, const PwGrid& g, R* S, R* X, int nb, double2* hpsi, int ldh,
+                       const double2* psi, int ld, const double* gk2, double tpiba2, GatherMode mode, double scale) {
+    using C = cplx<R>;
+    const int Ly = lines_for(g.ny), Lz = lines_for(g.nz);
+    hipLaunchKernelGGL((k_yfwd<R>), dim3(g.nxr, (g.nz + Ly - 1) / Ly, nb), dim3(NT), lds_bytes<R>(g.ny, Ly), s,
+                       reinterpret_cast<const C*>(X), g.ns, g.nz, g.nxr, g.slot_start.get(), g.slot_cnt.get(),
+                       g.stick_iy.get(), reinterpret_cast<C*>(S), g.py, tw_ptr<R>(g, g.twoff_y), Ly);
+    APW_LAUNCH_CHECK();
+    hipLaunchKernelGGL((k_zfwd<R>), dim3((g.ns + Lz - 1) / Lz, nb), dim3(NT), lds_bytes<R>(g.nz, Lz), s,
+                       reinterpret_cast<const C*>(S), g.stick_ig.get(), g.ns, hpsi, ldh, psi, ld, gk2, tpiba2,
+                       (int)mode, scale, g.pz, tw_ptr<R>(g, g.twoff_z), Lz);
+    APW_LAUNCH_CHECK();
+}
+
+template <class R>
+void pass_backward_half(hipStream_t s, const PwGrid& g, const double2* psi, int ld, R* S, R* X, int nb) {
+    using C = cplx<R>;
+    const int Ly = lines_for(g.ny), Lz = lines_for(g.nz);
+    hipLaunchKernelGGL((k_zbwd<R>), dim3((g.ns + Lz - 1) / Lz, nb), dim3(NT), lds_bytes<R>(g.nz, Lz), s, psi, ld,
+                       g.stick_ig.get(), g.ns, reinterpret_cast<C*>(S), g.pz, tw_ptr<R>(g, g.twoff_z), Lz);
+    APW_LAUNCH_CHECK();
+    hipLaunchKernelGGL((k_ybwd<R>), dim3(g.nxr, (g.nz + Ly - 1) / Ly, nb), dim3(NT), lds_bytes<R>(g.ny, Ly), s,
+                       reinterpret_cast<const C*>(S), g.ns, g.nz, g.nxr, g.slot_start.get(), g.slot_cnt.get(),
+                       g.stick_iy.get(), reinterpret_cast<C*>(X), g.py, tw_ptr<R>(g, g.twoff_y), Ly);
+    APW_LAUNCH_CHECK();
+}
+
+}  // namespace
+
+void PwGrid::build(int nx_, int ny_, int nz_, int npw_, const int* ig2ixyz) {
+    nx = nx_;
+    ny = ny_;
+    nz = nz_;
+    npw = npw_;
+    APW_CHECK(nx > 0 && ny > 0 && nz > 0 && npw > 0, "PwGrid: bad sizes");
+    px = make_plan(nx);
+    py = make_plan(ny);
+    pz = make_plan(nz);
+    const long nxyz = (long)nx * ny * nz;
+    // sticks keyed by ix*ny + iy, ordered by (ix, iy)
+    std::map<long, int> key2s;
+    for (int ig = 0; ig < npw; ++ig) {
+        const long idx = ig2ixyz[ig];
+        APW_CHECK(idx >= 0 && idx < nxyz, "ig2ixyz[%d] = %ld outside the %dx%dx%d box", ig, idx, nx, ny, nz);
+        key2s[idx / nz] = 0;
+    }
+    ns = 0;
+    std::vector<int> h_iy, h_ix_of_s;
+    for (auto& kv : key2s) {
+        kv.second = ns++;
+        h_ix_of_s.push_back((int)(kv.first / ny));
+        h_iy.push_back((int)(kv.first % ny));
+    }
+    std::vector<int> h_ig((size_t)ns * nz, -1);
+    for (int ig = 0; ig < npw; ++ig) {
+        const long idx = ig2ixyz[ig];
+        const int s = key2s[idx / nz];
+        const int iz = (int)(idx % nz);
+        APW_CHECK(h_ig[(size_t)s * nz + iz] < 0, "ig2ixyz maps two plane waves to box point %ld", idx);
+        h_ig[(size_t)s * nz + iz] = ig;
+    }
+    std::vector<int> h_slot_ix, h_slot_start, h_slot_cnt;
+    for (int s = 0; s < ns; ++s) {
+        if (h_slot_ix.empty() || h_slot_ix.back() != h_ix_of_s[s]) {
+            h_slot_ix.push_back(h_ix_of_s[s]);
+            h_slot_start.push_back(s);
+            h_slot_cnt.push_back(0);
+        }
+        h_slot_cnt.back()++;
+    }
+    nxr = (int)h_slot_ix.size();
+    auto up = [](DevBuf<int>& d, const std::vector<int>& h) {
+        d.reserve(h.size());
+        APW_HIP(hipMemcpy(d.get(), h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice));
+    };
+    up(stick_ig, h_ig);
+    up(stick_iy, h_iy);
+    up(slot_ix, h_slot_ix);
+    up(slot_start, h_slot_start);
+    up(slot_cnt, h_slot_cnt);
+    // twiddles exp(-2 pi i k / n) in long double
+    std::vector<double> t64;
+    std::vector<float> t32;
+    auto add_tw = [&](int n) {
+        const int off = (int)(t64.size() / 2);
+        const long double pi = 3.141592653589793238462643383279502884L;
+        for (int k = 0; k < n; ++k) {
+            const long double a = -2.0L * pi * (long double)k / (long double)n;
+            t64.push_back((double)cosl(a));
+            t64.push_back((double)sinl(a));
+            t32.push_back((float)cosl(a));
+            t32.push_back((float)sinl(a));
+        }
+        return off;
+    };
+    twoff_x = add_tw(nx);
+    twoff_y = add_tw(ny);
+    twoff_z = add_tw(nz);
+    tw64.reserve(t64.size());
+    tw32.reserve(t32.size());
+    APW_HIP(hipMemcpy(tw64.get(), t64.data(), t64.size() * sizeof(double), hipMemcpyHostToDevice));
+    APW_HIP(hipMemcpy(tw32.get(), t32.data(), t32.size() * sizeof(float), hipMemcpyHostToDevice));
+}
+
+template <class R>
+void local_hpsi(hipStream_t s, const PwGrid& g, const double* psi, int ld, double* hpsi, int ldh, int nb,
+                const R* veff, const double* gk2, double tpiba2, GatherMode mode, double scale, R* S, R* X) {
+    using C = cplx<R>;
+    if (nb <= 0) return;
+    const double2* p2 = reinterpret_cast<const double2*>(psi);
+    pass_backward_half<R>(s, g, p2, ld, S, X, nb);
+    const int Lx = lines_for(g.nx);
+    hipLaunchKernelGGL((k_xpass<R, 0>), dim3(g.ny, (g.nz + Lx - 1) / Lx, nb), dim3(NT), lds_bytes<R>(g.nx, Lx), s,
+                       reinterpret_cast<C*>(X), g.nxr, g.ny, g.nz, g.slot_ix.get(), veff, (double2*)nullptr, 0, 0.0,
+                       g.px, tw_ptr<R>(g, g.twoff_x), Lx);
+    APW_LAUNCH_CHECK();
+    pass_forward_half<R>(s, g, S, X, nb, reinterpret_cast<double2*>(hpsi), ldh, p2, ld, gk2, tpiba2, mode, scale);
+}
+
+template void local_hpsi<double>(hipStream_t, const PwGrid&, const double*, int, double*, int, int, const double*,
+                                 const double*, double, GatherMode, double, double*, double*);
+template void local_hpsi<float>(hipStream_t, const PwGrid&, const double*, int, double*, int, int, const float*,
+                                const double*, double, GatherMode, double, float*, float*);
+
+void sphere_to_box(hipStream_t s, const PwGrid& g, const double* psi, int ld, double* box, int nb, bool add,
+                   double factor, double* S, double* X) {
+    using C = cplx<double>;
+    pass_backward_half<double>(s, g, reinterpret_cast<const double2*>(psi), ld, S, X, nb);
+    const int Lx = lines_for(g.nx);
+    hipLaunchKernelGGL((k_xpass<double, 1>), dim3(g.ny, (g.nz + Lx - 1) / Lx, nb), dim3(NT),
+                       lds_bytes<double>(g.nx, Lx), s, reinterpret_cast<C*>(X), g.nxr, g.ny, g.nz, g.slot_ix.get(),
+                       (const double*)nullptr, reinterpret_cast<double2*>(box), add ? 1 : 0, factor, g.px,
+                       tw_ptr<double>(g, g.twoff_x), Lx);
+    APW_LAUNCH_CHECK();
+}
+
+void box_to_sphere(hipStream_t s, const PwGrid& g, const double* box, double* hpsi, int ldh, int nb,
+                   GatherMode mode, double scale, double* S, double* X) {
+    using C = cplx<double>;
+    const int Lx = lines_for(g.nx);
+    hipLaunchKernelGGL((k_xpass<double, 2>), dim3(g.ny, (g.nz + Lx - 1) / Lx, nb), dim3(NT),
+                       lds_bytes<double>(g.nx, Lx), s, reinterpret_cast<C*>(X), g.nxr, g.ny, g.nz, g.slot_ix.get(),
+                       (const double*)nullptr, const_cast<double2*>(reinterpret_cast<const double2*>(box)), 0, 0.0,
+                       g.px, tw_ptr<double>(g, g.twoff_x), Lx);
+    APW_LAUNCH_CHECK();
+    pass_forward_half<double>(s, g, S, X, nb, reinterpret_cast<double2*>(hpsi), ldh, nullptr, 0, nullptr, 0.0, mode,
+                              scale);
+}
+
+}  // namespace apw

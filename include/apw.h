@@ -1,0 +1,184 @@
+/*
+ * apw.h -- C ABI of the MI355X-native plane-wave Davidson path.
+ *
+ * Replaces, for one k-point / one GPU, the device path of ABACUS's
+ *   hsolver::DiagoDavid<T, DEVICE_GPU>          (source/module_hsolver/diago_david.h:21-84)
+ * driving the HamiltPW operator chain
+ *   Operator<T, DEVICE_GPU>::hPsi / act          (source/module_hamilt_general/operator.h:48-65)
+ *   Ekinetic / Veff / Nonlocal <OperatorPW>      (source/module_hamilt_pw/hamilt_pwdft/operator_pw/)
+ * and the device functors they call.
+ *
+ * Conventions
+ *  - Every function returns APW_OK (0) or a negative APW_E* status; the message of the
+ *    last failure on the calling thread is apw_last_error().  Nothing calls exit().
+ *  - Complex arrays are interleaved (re, im) pairs: "c128" = 2 doubles, "c64" = 2 floats.
+ *  - Matrices are column-major, as in the reference (psi: ld x nbands, band b at psi + b*ld).
+ *  - Pointers documented "device" are HIP device pointers owned by the caller; "host"
+ *    pointers are read during the call only.
+ *  - All work of a context is ordered on the context's stream.  Device outputs are
+ *    ready after apw_ctx_synchronize() (host outputs are ready on return).
+ */
+#ifndef APW_H
+#define APW_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    APW_OK = 0,
+    APW_EINVAL = -1,      /* bad argument / shape                        */
+    APW_EHIP = -2,        /* HIP runtime error                           */
+    APW_ENOMEM = -3,      /* device allocation failed                    */
+    APW_EBREAKDOWN = -4,  /* Gram-Schmidt norm < 1e-12 (reference aborts) */
+    APW_EEIG = -5,        /* subspace eigensolver did not converge        */
+    APW_ECALLBACK = -6    /* a user callback returned non-zero            */
+};
+
+enum { APW_FP64 = 0, APW_FP32_MIXED = 1 };
+
+typedef struct apw_ctx apw_ctx;
+typedef struct apw_ham apw_ham;
+typedef struct apw_david apw_david;
+
+const char* apw_last_error(void);
+
+/* Context: one device, one stream.  stream == NULL: the library creates its own
+ * (default priority) stream.  No counterpart in the reference, which uses the
+ * default stream and global hipBLAS handles (rocm/math_kernel_op.hip.cu:40,52). */
+int apw_ctx_create(apw_ctx** out, int device, void* hip_stream);
+int apw_ctx_destroy(apw_ctx* ctx);
+int apw_ctx_synchronize(apw_ctx* ctx);
+void* apw_ctx_stream(apw_ctx* ctx);
+
+/* ------------------------------------------------------------------------------
+ * Hamiltonian of one k-point (HamiltPW + PW_Basis_K + pseudopot_cell_vnl data).
+ * Host arrays, uploaded once:
+ *   ig2ixyz[npw]     iz + iy*nz + ix*ny*nz     (PW_Basis_K::ig2ixyz_k, pw_basis_k.cpp:335)
+ *   gk2[npw]         |k+G|^2 in tpiba^2 units   (pw_basis_k.cpp:247)
+ *   veff[nx*ny*nz]   local potential, z fastest (Veff::act, veff_pw.cpp:63)
+ *   vkb              c128, nkb rows of npwx (row-major nkb x npwx = col-major npwx x nkb,
+ *                    VNL_in_pw.cpp:226), projectors ordered type -> atom -> ih
+ *   deeq[nat*nhm*nhm] real, spin 0, deeq[(iat*nhm + x)*nhm + k] (nonlocal_op.cpp:33-37)
+ *   na[ntype], nh[ntype]
+ * Replaces HamiltPW construction (hamilt_pw.cpp:22-114) + Nonlocal::init/getvnl.
+ * ------------------------------------------------------------------------------ */
+typedef struct apw_ham_desc {
+    int nx, ny, nz;
+    int npw, npwx;
+    const int* ig2ixyz;
+    const double* gk2;
+    double tpiba2;
+    const double* veff;
+    int nkb;
+    const double* vkb;
+    int ntype;
+    const int* na;
+    const int* nh;
+    int nhm;
+    const double* deeq;
+    int precision;      /* APW_FP64 or APW_FP32_MIXED: arithmetic of FFT/Veff/nonlocal */
+} apw_ham_desc;
+
+int apw_ham_create(apw_ctx* ctx, const apw_ham_desc* desc, apw_ham** out);
+int apw_ham_destroy(apw_ham* ham);
+/* Switch the H|psi> arithmetic (APW_FP64 / APW_FP32_MIXED); psi and hpsi stay c128. */
+int apw_ham_set_precision(apw_ham* ham, int precision);
+
+/* Operator::hPsi (operator.cpp:39-88): hpsi[:, b] = H psi[:, b] for b < nbands
+ * (overwrite; the reference zeroes then accumulates every act()).  Device c128. */
+int apw_hpsi(apw_ham* ham, const double* psi, int ldpsi, double* hpsi, int ldhpsi, int nbands);
+
+/* Operator::act of each operator (ACCUMULATE into hpsi), nbasis = leading dimension:
+ * Ekinetic  ekinetic_pw.cpp:34-58; Veff veff_pw.cpp:38-105; Nonlocal nonlocal_pw.cpp:211-304 */
+int apw_act_ekinetic(apw_ham* ham, int nbands, int nbasis, const double* psi, double* hpsi);
+int apw_act_veff(apw_ham* ham, int nbands, int nbasis, const double* psi, double* hpsi);
+int apw_act_nonlocal(apw_ham* ham, int nbands, int nbasis, const double* psi, double* hpsi);
+
+/* PW_Basis_K::recip_to_real / real_to_recip, GPU forms (pw_transform_k.cpp:336-476):
+ * recip_to_real: box = FFT^+ (scatter(in)); out = box, or out += factor*box if add.
+ * real_to_recip: out[ig] = box'[ig]/nxyz, or out[ig] += factor/nxyz * box'[ig] if add,
+ *                box' = FFT^- (in).   in/out device c128 (box: nx*ny*nz). */
+int apw_recip_to_real(apw_ham* ham, const double* in, double* out, int add, double factor);
+int apw_real_to_recip(apw_ham* ham, const double* in, double* out, int add, double factor);
+
+/* ------------------------------------------------------------------------------
+ * Block Davidson (DiagoDavid, diago_david.cpp).
+ * ------------------------------------------------------------------------------ */
+/* DiagoDavid constructor (diago_david.cpp:34-121): precondition is HOST (len >= dim). */
+int apw_david_create(apw_ctx* ctx, const double* precondition, int nband, int dim,
+                     int david_ndim, apw_david** out);
+int apw_david_destroy(apw_david* dav);
+
+/* HPsiFunc (diago_david.h:56): HX = H * X[:, id_start..id_end], X = head of the block
+ * (no offset, ld = dim), HX = exact destination.  Device c128.  Return 0 on success. */
+typedef int (*apw_hpsi_cb)(void* user, const double* X, double* HX, int nvec, int dim,
+                           int id_start, int id_end);
+/* SPsiFunc (diago_david.h:75): SX = S X for nbands vectors of length npw (ld nrow). */
+typedef int (*apw_spsi_cb)(void* user, const double* X, double* SX, int nrow, int npw, int nbands);
+
+typedef struct apw_diag_stats {
+    int iterations;        /* return value of DiagoDavid::diag: sum of dav_iter */
+    int ntry;              /* diag_once calls                                    */
+    int notconv;           /* bands unconverged at exit                          */
+    int hpsi_calls;        /* H|psi> block applications                          */
+    long long hpsi_bands;  /* bands pushed through H|psi>                        */
+    int fp64_switch_iter;  /* FP32-mixed: iteration at which FP64 H|psi> took over (-1 = never) */
+    double max_residual;   /* max ||H psi - e psi|| over bands at exit (FP64)    */
+    double t_hpsi_s;       /* seconds inside H|psi> (device-event timed)         */
+    double t_total_s;      /* seconds inside diag()                              */
+} apw_diag_stats;
+
+/* DiagoDavid::diag (diago_david.cpp:1173-1200).
+ *   ham != NULL: H|psi> applied by the library (S = I); hpsi_cb/spsi_cb ignored.
+ *   ham == NULL: hpsi_cb required; spsi_cb NULL means S = I.
+ *   psi: device c128, ldpsi x nband, start vectors in, eigenvectors out.
+ *   eig: HOST double[nband] out.  precision: APW_FP64 or APW_FP32_MIXED (library H only).
+ * Returns the total Davidson iteration count (>= 0) or a negative status. */
+int apw_david_diag(apw_david* dav, apw_ham* ham, apw_hpsi_cb hpsi_cb, apw_spsi_cb spsi_cb,
+                   void* user, int ldpsi, double* psi, double* eig, double david_diag_thr,
+                   int david_maxiter, int ntry_max, int notconv_max, int precision,
+                   apw_diag_stats* stats);
+
+/* ------------------------------------------------------------------------------
+ * Device functors of the path (parity entry points; same argument meaning as the
+ * reference functor named beside each).  Device pointers unless noted.
+ * ------------------------------------------------------------------------------ */
+/* ekinetic_pw_op<double, DEVICE_GPU> (kernels/ekinetic_op.h; cuda/ekinetic_op.cu:13-48) */
+int apw_ekinetic_pw_op(apw_ctx* ctx, int nband, int npw, int max_npw, double tpiba2,
+                       const double* gk2, double* hpsi, const double* psi);
+/* veff_pw_op<double, DEVICE_GPU> (cuda/veff_op.cu:14-23): out[ir] *= in[ir] */
+int apw_veff_pw_op(apw_ctx* ctx, int size, double* out, const double* in);
+/* nonlocal_pw_op<double, DEVICE_GPU> (cuda/nonlocal_op.cu:14-38,95); sum and iat are
+ * HOST in/out counters advanced as the reference does. */
+int apw_nonlocal_pw_op(apw_ctx* ctx, int l1, int l2, int l3, int* sum, int* iat, int spin,
+                       int nkb, int deeq_x, int deeq_y, int deeq_z, const double* deeq,
+                       double* ps, const double* becp);
+/* set_3d_fft_box_op / set_recip_to_real_output_op / set_real_to_recip_output_op
+ * (module_pw/kernels/pw_op.h:9-48, cuda/pw_op.cu:12-122) */
+int apw_set_3d_fft_box_op(apw_ctx* ctx, int npwk, const int* box_index, const double* in,
+                          double* out);
+int apw_set_recip_to_real_output_op(apw_ctx* ctx, int nrxx, int add, double factor,
+                                    const double* in, double* out);
+int apw_set_real_to_recip_output_op(apw_ctx* ctx, int npw_k, int nxyz, int add, double factor,
+                                    const int* box_index, const double* in, double* out);
+/* FFT::fft3D_backward / fft3D_forward (fft.cpp:773-821): in-place, unnormalised,
+ * backward = exponent +i.  Device c128 box of nx*ny*nz, z fastest.  dir: +1 / -1. */
+int apw_fft3d(apw_ctx* ctx, int nx, int ny, int nz, double* box, int dir);
+/* gemm_op<complex<double>, DEVICE_GPU> (math_kernel_op.h:86-120; column-major zgemm
+ * semantics; trans in 'N','T','C').  alpha/beta HOST c128 (2 doubles each). */
+int apw_zgemm(apw_ctx* ctx, char transa, char transb, int m, int n, int k,
+              const double* alpha, const double* a, int lda, const double* b, int ldb,
+              const double* beta, double* c, int ldc);
+/* dnevx_op<complex<double>, DEVICE_GPU> (dngvd_op.h:85-107): lowest nbands eigenpairs
+ * of the Hermitian nstart x nstart matrix hcc (lower triangle, ld ldh).  eigenvalue:
+ * device double[nbands]; vcc: device c128, ld ldh, eigenvector j in column j. */
+int apw_dnevx_op(apw_ctx* ctx, int nstart, int ldh, const double* hcc, int nbands,
+                 double* eigenvalue, double* vcc);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* APW_H */

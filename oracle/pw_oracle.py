@@ -205,11 +205,64 @@ class DavidAbort(RuntimeError):
     pass
 
 
+def householder_tridiag_lower(A):
+    """Reflectors of the reduction of Hermitian A to real tridiagonal form, lower
+    triangle, LAPACK zhetrd('L')/zlarfg convention: P_k = I - tau_k v_k v_k^H, v_k[k+1] = 1,
+    P_k^H x = beta e_1 with beta = -sign(Re alpha) * ||x||.  Returns (V, tau)."""
+    A = np.array(A, dtype=np.complex128, copy=True)
+    n = A.shape[0]
+    V = np.zeros((n, n), dtype=np.complex128)
+    tau = np.zeros(n, dtype=np.complex128)
+    for k in range(n - 1):
+        x = A[k + 1:, k].copy()
+        alpha = x[0]
+        xn2 = float(np.sum(np.abs(x[1:]) ** 2))
+        v = np.zeros(n - k - 1, dtype=np.complex128)
+        v[0] = 1.0
+        if xn2 == 0.0 and alpha.imag == 0.0:
+            t = 0.0
+        else:
+            beta = -np.copysign(np.sqrt(abs(alpha) ** 2 + xn2), alpha.real)
+            t = (beta - alpha) / beta
+            v[1:] = x[1:] / (alpha - beta)
+        V[k + 1:, k] = v
+        tau[k] = t
+        if t != 0.0:
+            At = A[k + 1:, k + 1:]
+            p = At @ v
+            mu = np.vdot(v, p).real
+            y = t * p - (abs(t) ** 2 * mu / 2.0) * v
+            A[k + 1:, k + 1:] = At - np.outer(y, v.conj()) - np.outer(v, y.conj())
+    return V, tau
+
+
+def _apply_qh(V, tau, Z):
+    Z = np.array(Z, dtype=np.complex128, copy=True)
+    for k in range(V.shape[0] - 1):
+        v = V[k + 1:, k]
+        Z[k + 1:, :] -= np.conj(tau[k]) * np.outer(v, v.conj() @ Z[k + 1:, :])
+    return Z
+
+
 def dnevx(nbase, hcc, nband):
     """Lowest nband eigenpairs of the Hermitian nbase x nbase leading block, lower
-    triangle referenced, LAPACK zheevx (dngvd_op.cpp:142-235)."""
+    triangle referenced (dnevx_op -> zheevx, dngvd_op.cpp:142-235).
+
+    Eigenvector phases follow LAPACK's reduction + inverse-iteration path of zheevx
+    (zhetrd 'L', dstein): with A = Q T Q^H, each vector is Q z with z real and its
+    largest-magnitude component positive.  zheevx uses that convention whenever fewer
+    than all eigenpairs are requested; for the all-eigenpairs case (nbase == nband) its QR
+    path leaves implementation-dependent signs, which the rule above replaces."""
     a = hcc[:nbase, :nbase]
-    w, v = sla.eigh(a, lower=True, subset_by_index=[0, nband - 1], driver="evx")
+    A = np.tril(a) + np.tril(a, -1).conj().T
+    A[np.diag_indices(nbase)] = A.diagonal().real
+    w, v = sla.eigh(A, subset_by_index=[0, nband - 1])
+    V, tau = householder_tridiag_lower(A)
+    z = _apply_qh(V, tau, v)
+    for j in range(nband):
+        i = int(np.argmax(np.abs(z[:, j])))
+        ph = z[i, j] / abs(z[i, j])
+        v[:, j] = v[:, j] / ph
     return w, v
 
 

@@ -1,0 +1,336 @@
+"""GPU parity tests: the HIP library (through the C ABI) against the oracle and the
+reference's golden vectors.  Tolerances: FP64 kernels 1e-12 relative unless stated;
+golden vectors at the reference's own tolerances (they carry 6 significant digits)."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from apw import configs
+from oracle import pw_oracle as O
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as t
+    assert t.cuda.is_available(), "GPU tests need a GPU"
+    return t
+
+
+@pytest.fixture(scope="module")
+def ctx(torch):
+    from apw import pw
+    c = pw.Context(0)
+    yield c
+    c.close()
+
+
+def _load(name):
+    with open(os.path.join(GOLD, name)) as f:
+        return json.load(f)
+
+
+def _c(d):
+    return np.array(d["re"]) + 1j * np.array(d["im"])
+
+
+def dev(torch, a, dtype=None):
+    a = np.ascontiguousarray(a)
+    t = torch.from_numpy(a)
+    if dtype is not None:
+        t = t.to(dtype)
+    return t.cuda()
+
+
+def host(ctx, t):
+    ctx.synchronize()
+    return t.cpu().numpy()
+
+
+# ------------------------------------------------------------------ MFMA layout
+def test_mfma_layout(ctx):
+    from apw import pw
+    out = pw.probe_mfma_layout(ctx)
+    lane = np.arange(64)[:, None]
+    reg = np.arange(4)[None, :]
+    col = np.broadcast_to(lane % 16, (64, 4))
+    rows = {0: lane // 16 + 4 * reg,      # v_mfma_f64_16x16x4f64
+            2: 4 * (lane // 16) + reg}    # v_mfma_f32_16x16x4f32
+    for k in (0, 2):
+        assert np.array_equal(out[k + 1], col), f"col layout {k}: {out[k + 1].tolist()}"
+        assert np.array_equal(out[k], rows[k]), f"row layout {k}: {out[k].tolist()}"
+
+
+# ------------------------------------------------------------------ functors vs golden
+def test_ekinetic_golden(torch, ctx):
+    from apw import pw
+    d = _load("ref_ekinetic_op.json")
+    psi = dev(torch, _c(d["psi"]))
+    hpsi = torch.zeros_like(psi)
+    gk2 = dev(torch, np.array(d["gk2"]))
+    pw.ekinetic_pw_op(ctx, d["band"], d["dim"], d["dim"], d["tpiba2"], gk2, hpsi, psi)
+    got = host(ctx, hpsi)
+    assert np.abs(got - _c(d["expected_hpsi"])).max() < 1e-6
+    # bit-level parity with the oracle restatement (same expression, FP64)
+    ref = O.ekinetic_pw_op(d["band"], d["dim"], d["dim"], d["tpiba2"], np.array(d["gk2"]),
+                           np.zeros(len(d["gk2"]) * d["band"], complex), _c(d["psi"]))
+    assert np.abs(got - ref).max() <= 1e-15 * np.abs(ref).max()
+
+
+def test_veff_golden(torch, ctx):
+    from apw import pw
+    d = _load("ref_veff_op.json")
+    out = dev(torch, _c(d["out"]))
+    pw.veff_pw_op(ctx, d["size"], out, dev(torch, np.array(d["in"])))
+    assert np.abs(host(ctx, out) - _c(d["expected_out"])).max() < 6e-5
+
+
+def test_nonlocal_golden(torch, ctx):
+    from apw import pw
+    d = _load("ref_nonlocal_op.json")
+    ps = torch.zeros(len(d["expected_ps"]["re"]), dtype=torch.complex128, device="cuda")
+    s, iat = pw.nonlocal_pw_op(ctx, d["l1"], d["l2"], d["l3"], 0, 0, d["spin"], d["nkb"], d["deeq_x"], d["deeq_y"],
+                               d["deeq_z"], dev(torch, np.array(d["deeq"])), ps, dev(torch, _c(d["becp"])))
+    assert np.abs(host(ctx, ps) - _c(d["expected_ps"])).max() < 5e-6
+    assert (s, iat) == (d["expected_sum"], d["expected_iat"])
+
+
+def test_pw_op_golden(torch, ctx):
+    from apw import pw
+    d = _load("ref_pw_op.json")
+    idx = dev(torch, np.array(d["box_index"], dtype=np.int32))
+    box = torch.zeros(d["nxyz"], dtype=torch.complex128, device="cuda")
+    pw.set_3d_fft_box_op(ctx, d["npwk"], idx, dev(torch, _c(d["in_1"])), box)
+    assert np.abs(host(ctx, box) - _c(d["out_1"])).max() < 1e-12
+    out = torch.zeros(d["nxyz"], dtype=torch.complex128, device="cuda")
+    pw.set_recip_to_real_output_op(ctx, d["nxyz"], False, 1.0, dev(torch, _c(d["in_2"])), out)
+    assert np.abs(host(ctx, out) - _c(d["out_2"])).max() < 1e-12
+    o3 = dev(torch, _c(d["out_3_init"]))
+    pw.set_real_to_recip_output_op(ctx, d["npwk"], d["nxyz"], True, 1.0, idx, dev(torch, _c(d["in_3"])), o3)
+    assert np.abs(host(ctx, o3) - _c(d["out_3"])).max() < 5e-6
+
+
+# ------------------------------------------------------------------ FFT
+@pytest.mark.parametrize("shape", [(12, 12, 12), (2, 3, 5), (18, 20, 24), (36, 36, 36), (48, 48, 96), (45, 15, 30),
+                                   (160, 8, 16)])
+def test_fft3d_vs_numpy(torch, ctx, shape):
+    from apw import pw
+    rng = np.random.default_rng(sum(shape))
+    x = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    box = dev(torch, x.reshape(-1))
+    pw.fft3d(ctx, *shape, box, +1)
+    ref = O.fft3d_backward(x.reshape(-1), *shape)
+    got = host(ctx, box)
+    assert np.abs(got - ref).max() < 1e-13 * np.abs(ref).max()
+    pw.fft3d(ctx, *shape, box, -1)
+    ref2 = O.fft3d_forward(ref, *shape)
+    assert np.abs(host(ctx, box) - ref2).max() < 1e-13 * np.abs(ref2).max()
+
+
+def test_fft_chain_reference_fixture(torch, ctx):
+    """pw_op_test box -> backward FFT reproduces veff_op_test's input (12^3 grid)."""
+    from apw import pw
+    p = _load("ref_pw_op.json")
+    box = dev(torch, _c(p["out_1"]))
+    pw.fft3d(ctx, 12, 12, 12, box, +1)
+    assert np.abs(host(ctx, box) - _c(p["in_2"])).max() < 2e-5
+
+
+# ------------------------------------------------------------------ GEMM
+@pytest.mark.parametrize("ta,tb,m,n,k", [("N", "N", 70, 33, 19), ("C", "N", 64, 48, 3001), ("N", "T", 300, 40, 64),
+                                         ("T", "C", 17, 65, 129), ("C", "N", 160, 640, 51627),
+                                         ("N", "N", 51627, 160, 640), ("N", "C", 5, 7, 0)])
+def test_zgemm_vs_numpy(torch, ctx, ta, tb, m, n, k):
+    from apw import pw
+    rng = np.random.default_rng(m * 7 + n * 3 + k)
+
+    def rnd(r, c):
+        return rng.standard_normal((r, c)) + 1j * rng.standard_normal((r, c))
+
+    A = rnd(m, k) if ta == "N" else rnd(k, m)
+    Bm = rnd(k, n) if tb == "N" else rnd(n, k)
+    Cm = rnd(m, n)
+    op = {"N": lambda x: x, "T": lambda x: x.T, "C": lambda x: x.conj().T}
+    alpha, beta = 0.7 - 0.2j, 0.3 + 0.1j
+    ref = alpha * op[ta](A) @ op[tb](Bm) + beta * Cm
+    # column-major device storage = row-major transpose
+    dA, dB, dC = dev(torch, A.T.copy()), dev(torch, Bm.T.copy()), dev(torch, Cm.T.copy())
+    pw.gemm_op(ctx, ta, tb, m, n, k, alpha, dA, A.shape[0], dB, Bm.shape[0], beta, dC, m)
+    got = host(ctx, dC).T
+    scale = np.abs(ref).max() + np.sqrt(max(k, 1)) * 2
+    assert np.abs(got - ref).max() < 1e-13 * scale
+
+
+# ------------------------------------------------------------------ eigensolver
+@pytest.mark.parametrize("n,nband,ld", [(1, 1, 1), (2, 1, 3), (48, 24, 48), (200, 60, 256), (640, 160, 640)])
+def test_dnevx_vs_lapack(torch, ctx, n, nband, ld):
+    from apw import pw
+    rng = np.random.default_rng(n)
+    H = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    H = H + H.conj().T
+    w = np.linalg.eigvalsh(H)
+    hcc = np.zeros((ld, ld), complex)
+    hcc[:n, :n] = np.tril(H)          # lower triangle only, as cal_elem fills it
+    dh = dev(torch, hcc.T.copy())      # column-major
+    de = torch.zeros(nband, dtype=torch.float64, device="cuda")
+    dv = torch.zeros((ld, ld), dtype=torch.complex128, device="cuda")
+    pw.dnevx_op(ctx, n, ld, dh, nband, de, dv)
+    e = host(ctx, de)
+    V = host(ctx, dv).T[:n, :nband]
+    assert np.abs(e - w[:nband]).max() < 1e-12 * max(1.0, np.abs(w).max())
+    assert np.abs(H @ V - V * e).max() < 1e-11 * max(1.0, np.abs(w).max())
+    assert np.abs(V.conj().T @ V - np.eye(nband)).max() < 1e-11
+
+
+def test_dnevx_degenerate(torch, ctx):
+    """Exactly degenerate eigenvalues (a cluster) still give orthonormal eigenvectors."""
+    from apw import pw
+    rng = np.random.default_rng(5)
+    n, nband = 40, 12
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))
+    lam = np.sort(np.concatenate([np.full(4, -1.0), np.full(3, 0.5), rng.uniform(1, 3, n - 7)]))
+    H = (Q * lam) @ Q.conj().T
+    H = 0.5 * (H + H.conj().T)
+    dh = dev(torch, np.tril(H).T.copy())
+    de = torch.zeros(nband, dtype=torch.float64, device="cuda")
+    dv = torch.zeros((n, n), dtype=torch.complex128, device="cuda")
+    pw.dnevx_op(ctx, n, n, dh, nband, de, dv)
+    e = host(ctx, de)
+    V = host(ctx, dv).T[:, :nband]
+    assert np.abs(e - lam[:nband]).max() < 1e-12
+    assert np.abs(V.conj().T @ V - np.eye(nband)).max() < 1e-10
+    assert np.abs(H @ V - V * e).max() < 1e-10
+
+
+# ------------------------------------------------------------------ H|psi>
+@pytest.fixture(scope="module")
+def si8():
+    return configs.make("si8")
+
+
+def _rand_psi(s, nb, seed):
+    rng = np.random.default_rng(seed)
+    X = np.zeros((nb, s.npwx), complex)
+    X[:, : s.npw] = rng.standard_normal((nb, s.npw)) + 1j * rng.standard_normal((nb, s.npw))
+    return X
+
+
+@pytest.mark.parametrize("name", ["tiny", "si8"])
+def test_hpsi_vs_oracle(torch, ctx, name):
+    from apw import pw
+    s = configs.make(name)
+    ham = pw.HamiltPW(ctx, s)
+    nb = 5
+    X = _rand_psi(s, nb, 3)
+    out = ham.hPsi(dev(torch, X), nb)
+    got = host(ctx, out)[:, : s.npw]
+    ref = O.HamiltOracle(s).hpsi(X[:, : s.npw].reshape(-1), nb, s.npw).reshape(nb, s.npw)
+    assert np.abs(got - ref).max() < 1e-12 * np.abs(ref).max()
+    ham.close()
+
+
+def test_act_accumulate(torch, ctx, si8):
+    """Each act() accumulates; their sum equals hPsi (operator.cpp:134-167)."""
+    from apw import pw
+    s = si8
+    ham = pw.HamiltPW(ctx, s)
+    nb = 3
+    X = _rand_psi(s, nb, 4)
+    dX = dev(torch, X)
+    acc = dev(torch, np.full((nb, s.npwx), 0.25 + 0.5j))
+    ham.act_ekinetic(nb, s.npwx, dX, acc)
+    ham.act_veff(nb, s.npwx, dX, acc)
+    ham.act_nonlocal(nb, s.npwx, dX, acc)
+    full = ham.hPsi(dX, nb)
+    got = host(ctx, acc)[:, : s.npw] - (0.25 + 0.5j)
+    assert np.abs(got - host(ctx, full)[:, : s.npw]).max() < 1e-12 * np.abs(got).max()
+    ham.close()
+
+
+def test_recip_real_roundtrip(torch, ctx, si8):
+    from apw import pw
+    s = si8
+    ham = pw.HamiltPW(ctx, s)
+    x = _rand_psi(s, 1, 5)[0, : s.npw]
+    box = torch.zeros(s.nxyz, dtype=torch.complex128, device="cuda")
+    ham.recip_to_real(dev(torch, x), box)
+    b = np.zeros(s.nxyz, complex)
+    O.set_3d_fft_box_op(s.npw, s.ig2ixyz, x, b)
+    ref = O.fft3d_backward(b, s.nx, s.ny, s.nz)
+    assert np.abs(host(ctx, box) - ref).max() < 1e-12 * np.abs(ref).max()
+    back = torch.zeros(s.npw, dtype=torch.complex128, device="cuda")
+    ham.real_to_recip(box, back)
+    assert np.abs(host(ctx, back) - x).max() < 1e-12 * np.abs(x).max()
+    ham.close()
+
+
+# ------------------------------------------------------------------ Davidson
+def _run_david(torch, ctx, s, precision="fp64", thr=1e-9):
+    from apw import pw
+    ham = pw.HamiltPW(ctx, s)
+    dav = pw.DiagoDavid(ctx, s.precondition, s.nband, s.npw, s.david_ndim)
+    psi = dev(torch, s.psi0)
+    eig = np.zeros(s.nband)
+    it = dav.diag(ham, None, s.npwx, psi, eig, thr, 500, 5, 0, precision=precision)
+    res = (eig, host(ctx, psi), it, dav.stats)
+    dav.close()
+    ham.close()
+    return res
+
+
+def test_david_tiny_matches_oracle_and_dense(torch, ctx):
+    s = configs.make("tiny")
+    eig, psi, it, st = _run_david(torch, ctx, s)
+    oe, opsi, oit, odav = O.david_solve(s, thr=1e-9)
+    w = np.linalg.eigvalsh(O.HamiltOracle(s).dense())
+    assert np.abs(eig - oe).max() < 1e-10
+    assert np.abs(eig - w[: s.nband]).max() < 1e-9
+    assert it == oit
+
+
+def test_david_si8_fp64_parity(torch, ctx, si8):
+    s = si8
+    eig, psi, it, st = _run_david(torch, ctx, s)
+    oe, opsi, oit, odav = O.david_solve(s, thr=1e-9)
+    print("gpu iters", it, "oracle iters", oit, "max |de|", np.abs(eig - oe).max(), st)
+    assert np.abs(eig - oe).max() < 1e-10
+    P = psi[:, : s.npw]
+    assert np.abs(P.conj() @ P.T - np.eye(s.nband)).max() < 1e-9
+    assert st["max_residual"] < 1e-3
+
+
+def test_david_si8_mixed(torch, ctx, si8):
+    s = si8
+    eig, psi, it, st = _run_david(torch, ctx, s, precision="fp32_mixed")
+    oe, _, _, _ = O.david_solve(s, thr=1e-9)
+    print("mixed iters", it, "switch at", st["fp64_switch_iter"], "max |de|", np.abs(eig - oe).max())
+    assert np.abs(eig - oe).max() < 1e-9
+
+
+def test_david_callback_dense(torch, ctx):
+    """Reference-style test (diago_david_test.cpp): a dense random Hermitian H applied by a
+    user hpsi callback; eigenvalues vs LAPACK."""
+    from apw import pw
+    rng = np.random.default_rng(100)
+    n, nband = 200, 8
+    H = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    H = 0.5 * (H + H.conj().T) + np.diag(np.arange(n) * 0.5)
+    w = np.linalg.eigvalsh(H)
+    dH = dev(torch, H)
+    pre = np.real(np.diag(H)).copy()
+    pre = np.where(np.abs(pre) < 1.0, 1.0, pre)
+
+    def hpsi(X, HX, nvec, dim, i0, i1):
+        HX.copy_((dH @ X[i0:i1 + 1].T).T)
+
+    dav = pw.DiagoDavid(ctx, pre, nband, n, 4)
+    psi0 = rng.standard_normal((nband, n)) + 1j * rng.standard_normal((nband, n))
+    psi = dev(torch, psi0)
+    eig = np.zeros(nband)
+    dav.diag(hpsi, None, n, psi, eig, 1e-10, 200)
+    assert np.abs(eig - w[:nband]).max() < 1e-8
+    dav.close()
