@@ -52,6 +52,10 @@ PROTOTYPES = {
     "apw_ctx_destroy": (_i, [_vp]),
     "apw_ctx_synchronize": (_i, [_vp]),
     "apw_ctx_stream": (_vp, [_vp]),
+    "apw_ctx_set_profiling": (_i, [_vp, _i]),
+    "apw_ctx_profile_count": (_i, [_vp]),
+    "apw_ctx_profile_entry": (_i, [_vp, _i, C.c_char_p, _i, C.POINTER(C.c_longlong), C.POINTER(_d), C.POINTER(_d),
+                                   C.POINTER(_d)]),
     "apw_ham_create": (_i, [_vp, C.POINTER(HamDesc), C.POINTER(_vp)]),
     "apw_ham_destroy": (_i, [_vp]),
     "apw_ham_set_precision": (_i, [_vp, _i]),

@@ -61,6 +61,21 @@ class Context:
     def pre(self):
         _torch().cuda.synchronize(self.device)
 
+    def set_profiling(self, on: bool):
+        check(self.lib.apw_ctx_set_profiling(self.handle, int(on)))
+
+    def profile(self):
+        """Per-kernel totals {name: dict(launches, ms, bytes, flops)} of the profiler."""
+        n = check(self.lib.apw_ctx_profile_count(self.handle))
+        out = {}
+        buf = C.create_string_buffer(128)
+        for i in range(n):
+            la, ms, by, fl = C.c_longlong(), C.c_double(), C.c_double(), C.c_double()
+            check(self.lib.apw_ctx_profile_entry(self.handle, i, buf, 128, C.byref(la), C.byref(ms), C.byref(by),
+                                                 C.byref(fl)))
+            out[buf.value.decode()] = dict(launches=la.value, ms=ms.value, bytes=by.value, flops=fl.value)
+        return out
+
     def synchronize(self):
         check(self.lib.apw_ctx_synchronize(self.handle))
 

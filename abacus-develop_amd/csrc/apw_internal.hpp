@@ -85,6 +85,37 @@ struct GemmWorkspace {
     DevBuf<double> part;
 };
 
+// ---------------------------------------------------------------- kernel profiler
+// When enabled, instrumented launches are bracketed by HIP events recorded on the launch
+// stream; per kernel name the launch count, the device time between the events and the
+// algorithmic bytes / flops of each launch are accumulated (resolved lazily).
+struct ProfEntry {
+    std::string name;
+    long long launches = 0;
+    double bytes = 0, flops = 0, ms = 0;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+};
+
+struct Prof {
+    bool on = false;
+    std::vector<ProfEntry> entries;
+    std::vector<hipEvent_t> pool;
+    ProfEntry& entry(const char* name);
+    hipEvent_t event();
+    void resolve();
+    void reset();
+    ~Prof();
+};
+
+struct ProfScope {
+    Prof* p = nullptr;
+    hipStream_t s = nullptr;
+    ProfEntry* e = nullptr;
+    hipEvent_t a = nullptr, b = nullptr;
+    ProfScope(Prof* prof, hipStream_t st, const char* name, double bytes, double flops);
+    ~ProfScope();
+};
+
 }  // namespace apw
 
 struct apw_ctx {
@@ -95,6 +126,7 @@ struct apw_ctx {
     // scratch shared by the functor-level entry points
     apw::DevBuf<double> scratch;
     apw::GemmWorkspace gws;
+    apw::Prof prof;
 };
 
 namespace apw {
@@ -118,7 +150,7 @@ struct DeviceGuard {
 template <class R>
 void gemm(hipStream_t s, GemmWorkspace& ws, int num_cu, char ta, char tb, int m, int n, int k,
           double alpha_re, double alpha_im, const R* A, int lda, const R* B, int ldb,
-          double beta_re, double beta_im, R* C, int ldc);
+          double beta_re, double beta_im, R* C, int ldc, Prof* prof = nullptr);
 
 // ---------------------------------------------------------------- eigensolver
 struct EigWorkspace {
@@ -135,7 +167,7 @@ struct EigWorkspace {
 // Lowest nband eigenpairs of Hermitian hcc (n x n, lower triangle, ld ldh).
 // eig: device double[nband]; vcc: device c128 (ld ldv), columns 0..nband-1 rows 0..n-1.
 void heevx_lowest(hipStream_t s, EigWorkspace& ws, int num_cu, int n, int ldh, const double* hcc,
-                  int nband, double* eig, double* vcc, int ldv);
+                  int nband, double* eig, double* vcc, int ldv, Prof* prof = nullptr);
 
 // ---------------------------------------------------------------- small dense helpers
 // Cholesky of Hermitian positive definite G (n x n, ld) and inverse of the upper factor:

@@ -14,6 +14,71 @@ void set_error(const char* fmt, ...) {
     va_end(ap);
 }
 
+ProfEntry& Prof::entry(const char* name) {
+    for (auto& e : entries)
+        if (e.name == name) return e;
+    entries.emplace_back();
+    entries.back().name = name;
+    return entries.back();
+}
+
+hipEvent_t Prof::event() {
+    if (!pool.empty()) {
+        hipEvent_t e = pool.back();
+        pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    APW_HIP(hipEventCreate(&e));
+    return e;
+}
+
+void Prof::resolve() {
+    for (auto& e : entries) {
+        for (auto& pr : e.pending) {
+            float ms = 0;
+            APW_HIP(hipEventSynchronize(pr.second));
+            APW_HIP(hipEventElapsedTime(&ms, pr.first, pr.second));
+            e.ms += ms;
+            pool.push_back(pr.first);
+            pool.push_back(pr.second);
+        }
+        e.pending.clear();
+    }
+}
+
+void Prof::reset() {
+    resolve();
+    entries.clear();
+}
+
+Prof::~Prof() {
+    for (auto& e : entries)
+        for (auto& pr : e.pending) {
+            (void)hipEventDestroy(pr.first);
+            (void)hipEventDestroy(pr.second);
+        }
+    for (auto ev : pool) (void)hipEventDestroy(ev);
+}
+
+ProfScope::ProfScope(Prof* prof, hipStream_t st, const char* name, double bytes, double flops) {
+    if (!prof || !prof->on) return;
+    p = prof;
+    s = st;
+    e = &prof->entry(name);
+    e->launches += 1;
+    e->bytes += bytes;
+    e->flops += flops;
+    a = prof->event();
+    b = prof->event();
+    APW_HIP(hipEventRecord(a, s));
+}
+
+ProfScope::~ProfScope() {
+    if (!p) return;
+    if (hipEventRecord(b, s) == hipSuccess) e->pending.emplace_back(a, b);
+}
+
 }  // namespace apw
 
 using apw::guarded;
@@ -77,6 +142,40 @@ int apw_ctx_synchronize(apw_ctx* ctx) {
 
 void* apw_ctx_stream(apw_ctx* ctx) { return ctx ? reinterpret_cast<void*>(ctx->stream) : nullptr; }
 
+int apw_ctx_set_profiling(apw_ctx* ctx, int on) {
+    return guarded([&] {
+        APW_CHECK(ctx, "apw_ctx_set_profiling: null ctx");
+        apw::DeviceGuard g(ctx->device);
+        ctx->prof.reset();
+        ctx->prof.on = on != 0;
+        return (int)APW_OK;
+    });
+}
+
+int apw_ctx_profile_count(apw_ctx* ctx) {
+    return guarded([&] {
+        APW_CHECK(ctx, "apw_ctx_profile_count: null ctx");
+        apw::DeviceGuard g(ctx->device);
+        ctx->prof.resolve();
+        return (int)ctx->prof.entries.size();
+    });
+}
+
+int apw_ctx_profile_entry(apw_ctx* ctx, int i, char* name, int name_cap, long long* launches, double* ms,
+                          double* bytes, double* flops) {
+    return guarded([&] {
+        APW_CHECK(ctx && name && name_cap > 0 && launches && ms && bytes && flops, "apw_ctx_profile_entry: null argument");
+        APW_CHECK(i >= 0 && i < (int)ctx->prof.entries.size(), "apw_ctx_profile_entry: index %d out of range", i);
+        const apw::ProfEntry& e = ctx->prof.entries[i];
+        snprintf(name, name_cap, "%s", e.name.c_str());
+        *launches = e.launches;
+        *ms = e.ms;
+        *bytes = e.bytes;
+        *flops = e.flops;
+        return (int)APW_OK;
+    });
+}
+
 int apw_zgemm(apw_ctx* ctx, char transa, char transb, int m, int n, int k, const double* alpha, const double* a,
               int lda, const double* b, int ldb, const double* beta, double* c, int ldc) {
     return guarded([&] {
@@ -84,7 +183,7 @@ int apw_zgemm(apw_ctx* ctx, char transa, char transb, int m, int n, int k, const
         APW_CHECK(k == 0 || (a && b), "apw_zgemm: null A/B");
         apw::DeviceGuard g(ctx->device);
         apw::gemm<double>(ctx->stream, ctx->gws, ctx->num_cu, transa, transb, m, n, k, alpha[0], alpha[1], a, lda, b, ldb,
-                          beta[0], beta[1], c, ldc);
+                          beta[0], beta[1], c, ldc, &ctx->prof);
         APW_HIP(hipStreamSynchronize(ctx->stream));
         return (int)APW_OK;
     });

@@ -182,7 +182,7 @@ struct David {
 
     void gemm(char ta, char tb, int m, int n, int k, double ar, const double* A, int lda, const double* Bm, int ldb,
               double br, double* C, int ldc) {
-        apw::gemm<double>(st(), gws, ctx->num_cu, ta, tb, m, n, k, ar, 0.0, A, lda, Bm, ldb, br, 0.0, C, ldc);
+        apw::gemm<double>(st(), gws, ctx->num_cu, ta, tb, m, n, k, ar, 0.0, A, lda, Bm, ldb, br, 0.0, C, ldc, &ctx->prof);
     }
 
     // HX = H X[:, i0..i1]  (hpsi_func semantics: X = head of basis, HX = exact destination)
@@ -254,7 +254,7 @@ struct David {
     }
 
     void solve_subspace(int nbase) {
-        heevx_lowest(st(), ews, ctx->num_cu, nbase, nbx, hcc.get(), nband, eig.get(), vcc.get(), nbx);
+        heevx_lowest(st(), ews, ctx->num_cu, nbase, nbx, hcc.get(), nband, eig.get(), vcc.get(), nbx, &ctx->prof);
     }
 
     // hcc[nbase .. nbase+nc, 0 .. nbase+nc) = basis_new^H hpsi   (cal_elem, diago_david.cpp:626-717)

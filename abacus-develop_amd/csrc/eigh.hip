@@ -512,7 +512,7 @@ __global__ __launch_bounds__(1024) void k_chol_inv(int n, const double2* __restr
 }  // namespace
 
 void heevx_lowest(hipStream_t s, EigWorkspace& ws, int num_cu, int n, int ldh, const double* hcc, int nband,
-                  double* eig, double* vcc, int ldv) {
+                  double* eig, double* vcc, int ldv, Prof* prof) {
     APW_CHECK(n >= 1 && nband >= 1 && nband <= n && ldh >= n && ldv >= n, "heevx: bad sizes n=%d nband=%d", n, nband);
     APW_CHECK(n <= 2000, "heevx: n=%d too large (LDS of the reduction kernel)", n);
     ws.a.reserve((size_t)2 * n * n);
@@ -542,12 +542,16 @@ void heevx_lowest(hipStream_t s, EigWorkspace& ws, int num_cu, int n, int ldh, c
         double2* Pp = reinterpret_cast<double2*>(ws.p.get());
         unsigned* barp = ws.bar.get();
         void* args[] = {&A, &n, &Vp, &taup, &dp, &ep, &Pp, &barp};
+        // algorithmic: per column a hemv read and a rank-2 read+write of the trailing matrix
+        ProfScope ps(prof, s, "eig_tridiag", 16.0 * 3.0 * n * (double)n * n / 3.0, 16.0 / 3.0 * n * (double)n * n);
         APW_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_tridiag), dim3(nwg), dim3(NT), args,
                                            (unsigned)lds, s));
     } else {
         // n == 1: T = [Re a00]; the real part is the first double of A
         APW_HIP(hipMemcpyAsync(ws.d.get(), A, sizeof(double), hipMemcpyDeviceToDevice, s));
     }
+    {
+    ProfScope ps(prof, s, "eig_bisect_invit", 0.0, 0.0);
     hipLaunchKernelGGL(k_bisect, dim3((nband + 63) / 64), dim3(64), 0, s, ws.d.get(), ws.e.get(), n, nband, eig);
     APW_LAUNCH_CHECK();
     const size_t lds_inv = (size_t)n * sizeof(double) + 64 * sizeof(double);
@@ -556,7 +560,10 @@ void heevx_lowest(hipStream_t s, EigWorkspace& ws, int num_cu, int n, int ldh, c
     hipLaunchKernelGGL(k_invit, dim3(nband), dim3(NT), lds_inv, s, ws.d.get(), ws.e.get(), n, nband, eig, ws.z.get(),
                        1e-7, ws.lu.get(), ws.piv.get());
     APW_LAUNCH_CHECK();
+    }
     const size_t lds_bt = (size_t)n * sizeof(double2) + 64 * sizeof(double);
+    ProfScope pb(prof, s, "eig_backtransform", 16.0 * n * (double)n / 2.0 + 16.0 * n * nband,
+                 8.0 * n * (double)n * nband);
     hipLaunchKernelGGL(k_backtransform, dim3(nband), dim3(NT), lds_bt, s, n,
                        reinterpret_cast<const double2*>(ws.v.get()), reinterpret_cast<const double2*>(ws.tau.get()),
                        ws.z.get(), reinterpret_cast<double2*>(vcc), ldv);

@@ -15,6 +15,8 @@
 // 256-byte coalesced runs.  Accumulator register r of lane l holds D row drow(l, r) and
 // column l % 16; drow is (l / 16) + 4 r for the f64 instruction and 4 (l / 16) + r for the
 // f32 one (checked on the device by the layout probe below and its unit test).
+#include <type_traits>
+
 #include "apw_internal.hpp"
 
 namespace apw {
@@ -250,7 +252,7 @@ void launch(hipStream_t s, dim3 grid, int M, int N, int K, int kchunk, const R* 
 template <class R>
 void gemm(hipStream_t s, GemmWorkspace& ws, int num_cu, char ta, char tb, int m, int n, int k,
           double alpha_re, double alpha_im, const R* A, int lda, const R* B, int ldb,
-          double beta_re, double beta_im, R* C, int ldc) {
+          double beta_re, double beta_im, R* C, int ldc, Prof* prof) {
     using c_t = typename Mfma<R>::c_t;
     const int oa = opcode(ta), ob = opcode(tb);
     APW_CHECK(oa >= 0 && ob >= 0, "gemm: bad trans '%c' '%c'", ta, tb);
@@ -287,6 +289,11 @@ void gemm(hipStream_t s, GemmWorkspace& ws, int num_cu, char ta, char tb, int m,
         part = reinterpret_cast<R*>(ws.part.get());
     }
     dim3 grid(tiles_m, tiles_n, splits);
+    const double esz = 2.0 * sizeof(R);
+    const bool beta_nz = (beta_re != 0.0 || beta_im != 0.0);
+    ProfScope ps(prof, s, std::is_same<R, double>::value ? "zgemm_f64" : "cgemm_f32",
+                 esz * ((double)m * k + (double)k * n + (double)m * n * (splits > 1 ? splits : (beta_nz ? 2 : 1))),
+                 8.0 * m * n * (double)k);
 #define APW_GEMM_CASE(X, Y)                                                                            \
     if (oa == X && ob == Y) {                                                                         \
         launch<R, X, Y>(s, grid, m, n, k, kchunk, A, lda, B, ldb, C, ldc, alpha_re, alpha_im, beta_re, \
@@ -298,6 +305,8 @@ void gemm(hipStream_t s, GemmWorkspace& ws, int num_cu, char ta, char tb, int m,
 #undef APW_GEMM_CASE
     APW_LAUNCH_CHECK();
     if (splits > 1) {
+        ProfScope pr(prof, s, "gemm_splitk_reduce", esz * (double)m * n * (splits + (beta_nz ? 2 : 1)),
+                     2.0 * m * n * (double)splits);
         const size_t total = (size_t)m * n;
         const int blocks = (int)std::min<size_t>((total + 255) / 256, 65536);
         hipLaunchKernelGGL(splitk_reduce<R>, dim3(blocks), dim3(256), 0, s, m, n, splits,
@@ -308,9 +317,9 @@ void gemm(hipStream_t s, GemmWorkspace& ws, int num_cu, char ta, char tb, int m,
 }
 
 template void gemm<double>(hipStream_t, GemmWorkspace&, int, char, char, int, int, int, double, double,
-                           const double*, int, const double*, int, double, double, double*, int);
+                           const double*, int, const double*, int, double, double, double*, int, Prof*);
 template void gemm<float>(hipStream_t, GemmWorkspace&, int, char, char, int, int, int, double, double,
-                          const float*, int, const float*, int, double, double, float*, int);
+                          const float*, int, const float*, int, double, double, float*, int, Prof*);
 
 // ---------------------------------------------------------------- layout probe
 // Records, for every (lane, accumulator register) of one v_mfma_f64_16x16x4f64 and one

@@ -434,6 +434,8 @@ __global__ __launch_bounds__(NT) void k_zfwd(const cplx<R>* __restrict__ S, cons
     }
 }
 
+double fft_flops(int n) { return 5.0 * n * std::log2((double)n); }
+
 int lines_for(int n) {
     // 2 buffers of n * (L+1) complex doubles within ~64 KB of LDS
     int L = 16;
@@ -454,13 +456,21 @@ const cplx<R>* tw_ptr(const PwGrid& g, int off) {
 
 template <class R>
 void pass_forward_half(hipStream_t s, const PwGrid& g, R* S, R* X, int nb, double2* hpsi, int ldh,
-                       const double2* psi, int ld, const double* gk2, double tpiba2, GatherMode mode, double scale) {
+                       const double2* psi, int ld, const double* gk2, double tpiba2, GatherMode mode, double scale,
+                       Prof* prof) {
     using C = cplx<R>;
     const int Ly = lines_for(g.ny), Lz = lines_for(g.nz);
+    const double cs = sizeof(C);
+    const double nSz = (double)g.ns * g.nz, nX = (double)g.nxr * g.ny * g.nz;
+    {
+    ProfScope ps(prof, s, "fft_y_fwd", nb * cs * (nX + nSz), nb * fft_flops(g.ny) * g.nxr * g.nz);
     hipLaunchKernelGGL((k_yfwd<R>), dim3(g.nxr, (g.nz + Ly - 1) / Ly, nb), dim3(NT), lds_bytes<R>(g.ny, Ly), s,
                        reinterpret_cast<const C*>(X), g.ns, g.nz, g.nxr, g.slot_start.get(), g.slot_cnt.get(),
                        g.stick_iy.get(), reinterpret_cast<C*>(S), g.py, tw_ptr<R>(g, g.twoff_y), Ly);
     APW_LAUNCH_CHECK();
+    }
+    const double pwio = 16.0 * g.npw * ((mode == GATHER_SET) ? 1 : 2);
+    ProfScope ps(prof, s, "fft_z_fwd_gather", nb * (cs * nSz + pwio), nb * (fft_flops(g.nz) * g.ns + 4.0 * g.npw));
     hipLaunchKernelGGL((k_zfwd<R>), dim3((g.ns + Lz - 1) / Lz, nb), dim3(NT), lds_bytes<R>(g.nz, Lz), s,
                        reinterpret_cast<const C*>(S), g.stick_ig.get(), g.ns, hpsi, ldh, psi, ld, gk2, tpiba2,
                        (int)mode, scale, g.pz, tw_ptr<R>(g, g.twoff_z), Lz);
@@ -468,12 +478,18 @@ void pass_forward_half(hipStream_t s, const PwGrid& g, R* S, R* X, int nb, doubl
 }
 
 template <class R>
-void pass_backward_half(hipStream_t s, const PwGrid& g, const double2* psi, int ld, R* S, R* X, int nb) {
+void pass_backward_half(hipStream_t s, const PwGrid& g, const double2* psi, int ld, R* S, R* X, int nb, Prof* prof) {
     using C = cplx<R>;
     const int Ly = lines_for(g.ny), Lz = lines_for(g.nz);
+    const double cs = sizeof(C);
+    const double nSz = (double)g.ns * g.nz, nX = (double)g.nxr * g.ny * g.nz;
+    {
+    ProfScope ps(prof, s, "fft_z_bwd_scatter", nb * (16.0 * g.npw + cs * nSz), nb * fft_flops(g.nz) * g.ns);
     hipLaunchKernelGGL((k_zbwd<R>), dim3((g.ns + Lz - 1) / Lz, nb), dim3(NT), lds_bytes<R>(g.nz, Lz), s, psi, ld,
                        g.stick_ig.get(), g.ns, reinterpret_cast<C*>(S), g.pz, tw_ptr<R>(g, g.twoff_z), Lz);
     APW_LAUNCH_CHECK();
+    }
+    ProfScope ps(prof, s, "fft_y_bwd", nb * cs * (nSz + nX), nb * fft_flops(g.ny) * g.nxr * g.nz);
     hipLaunchKernelGGL((k_ybwd<R>), dim3(g.nxr, (g.nz + Ly - 1) / Ly, nb), dim3(NT), lds_bytes<R>(g.ny, Ly), s,
                        reinterpret_cast<const C*>(S), g.ns, g.nz, g.nxr, g.slot_start.get(), g.slot_cnt.get(),
                        g.stick_iy.get(), reinterpret_cast<C*>(X), g.py, tw_ptr<R>(g, g.twoff_y), Ly);
@@ -559,28 +575,34 @@ void PwGrid::build(int nx_, int ny_, int nz_, int npw_, const int* ig2ixyz) {
 
 template <class R>
 void local_hpsi(hipStream_t s, const PwGrid& g, const double* psi, int ld, double* hpsi, int ldh, int nb,
-                const R* veff, const double* gk2, double tpiba2, GatherMode mode, double scale, R* S, R* X) {
+                const R* veff, const double* gk2, double tpiba2, GatherMode mode, double scale, R* S, R* X,
+                Prof* prof) {
     using C = cplx<R>;
     if (nb <= 0) return;
     const double2* p2 = reinterpret_cast<const double2*>(psi);
-    pass_backward_half<R>(s, g, p2, ld, S, X, nb);
+    pass_backward_half<R>(s, g, p2, ld, S, X, nb, prof);
     const int Lx = lines_for(g.nx);
+    {
+    const double nX = (double)g.nxr * g.ny * g.nz, nxyz = (double)g.nx * g.ny * g.nz;
+    ProfScope ps(prof, s, "fft_x_veff_fused", nb * (2.0 * sizeof(C) * nX + sizeof(R) * nxyz),
+                 nb * (2.0 * fft_flops(g.nx) * g.ny * g.nz + 2.0 * nxyz));
     hipLaunchKernelGGL((k_xpass<R, 0>), dim3(g.ny, (g.nz + Lx - 1) / Lx, nb), dim3(NT), lds_bytes<R>(g.nx, Lx), s,
                        reinterpret_cast<C*>(X), g.nxr, g.ny, g.nz, g.slot_ix.get(), veff, (double2*)nullptr, 0, 0.0,
                        g.px, tw_ptr<R>(g, g.twoff_x), Lx);
     APW_LAUNCH_CHECK();
-    pass_forward_half<R>(s, g, S, X, nb, reinterpret_cast<double2*>(hpsi), ldh, p2, ld, gk2, tpiba2, mode, scale);
+    }
+    pass_forward_half<R>(s, g, S, X, nb, reinterpret_cast<double2*>(hpsi), ldh, p2, ld, gk2, tpiba2, mode, scale, prof);
 }
 
 template void local_hpsi<double>(hipStream_t, const PwGrid&, const double*, int, double*, int, int, const double*,
-                                 const double*, double, GatherMode, double, double*, double*);
+                                 const double*, double, GatherMode, double, double*, double*, Prof*);
 template void local_hpsi<float>(hipStream_t, const PwGrid&, const double*, int, double*, int, int, const float*,
-                                const double*, double, GatherMode, double, float*, float*);
+                                const double*, double, GatherMode, double, float*, float*, Prof*);
 
 void sphere_to_box(hipStream_t s, const PwGrid& g, const double* psi, int ld, double* box, int nb, bool add,
                    double factor, double* S, double* X) {
     using C = cplx<double>;
-    pass_backward_half<double>(s, g, reinterpret_cast<const double2*>(psi), ld, S, X, nb);
+    pass_backward_half<double>(s, g, reinterpret_cast<const double2*>(psi), ld, S, X, nb, nullptr);
     const int Lx = lines_for(g.nx);
     hipLaunchKernelGGL((k_xpass<double, 1>), dim3(g.ny, (g.nz + Lx - 1) / Lx, nb), dim3(NT),
                        lds_bytes<double>(g.nx, Lx), s, reinterpret_cast<C*>(X), g.nxr, g.ny, g.nz, g.slot_ix.get(),
@@ -599,7 +621,7 @@ void box_to_sphere(hipStream_t s, const PwGrid& g, const double* box, double* hp
                        g.px, tw_ptr<double>(g, g.twoff_x), Lx);
     APW_LAUNCH_CHECK();
     pass_forward_half<double>(s, g, S, X, nb, reinterpret_cast<double2*>(hpsi), ldh, nullptr, 0, nullptr, 0.0, mode,
-                              scale);
+                              scale, nullptr);
 }
 
 }  // namespace apw

@@ -48,7 +48,8 @@ enum GatherMode { GATHER_SET = 0, GATHER_ADD = 1, GATHER_SET_KIN = 2 };
 // above (GATHER_SET_KIN also adds gk2*tpiba2*psi).  S, X: workspace for nb bands.
 template <class R>
 void local_hpsi(hipStream_t s, const PwGrid& g, const double* psi, int ld, double* hpsi, int ldh, int nb,
-                const R* veff, const double* gk2, double tpiba2, GatherMode mode, double scale, R* S, R* X);
+                const R* veff, const double* gk2, double tpiba2, GatherMode mode, double scale, R* S, R* X,
+                Prof* prof = nullptr);
 
 // Sphere -> full box (backward, unnormalised): box = FFT^+(scatter(psi)), nb bands of
 // nx*ny*nz each.  add: box += factor * result.

@@ -238,13 +238,13 @@ void Ham::local(const double* psi, int ld, double* hpsi, int ldh, int nb, Gather
             S.reserve(2 * grid.bytes_S(1) * n);
             X.reserve(2 * grid.bytes_X(1) * n);
             local_hpsi<double>(ctx->stream, grid, psi + (size_t)2 * b0 * ld, ld, hpsi + (size_t)2 * b0 * ldh, ldh, n,
-                               veff.get(), gk2.get(), tpiba2, mode, scale, S.get(), X.get());
+                               veff.get(), gk2.get(), tpiba2, mode, scale, S.get(), X.get(), &ctx->prof);
         } else {
             S.reserve(grid.bytes_S(1) * n);
             X.reserve(grid.bytes_X(1) * n);
             local_hpsi<float>(ctx->stream, grid, psi + (size_t)2 * b0 * ld, ld, hpsi + (size_t)2 * b0 * ldh, ldh, n,
                               veff32.get(), gk2.get(), tpiba2, mode, scale, reinterpret_cast<float*>(S.get()),
-                              reinterpret_cast<float*>(X.get()));
+                              reinterpret_cast<float*>(X.get()), &ctx->prof);
         }
     }
 }
@@ -257,13 +257,13 @@ void Ham::nonlocal(const double* psi, int ld, double* hpsi, int ldh, int nb, int
         becp.reserve((size_t)2 * nkb * nb);
         ps.reserve((size_t)2 * nkb * nb);
         gemm<double>(s, gws, ncu, 'C', 'N', nkb, nb, npw, 1.0, 0.0, vkb.get(), npwx, psi, ld, 0.0, 0.0, becp.get(),
-                     nkb);
+                     nkb, &ctx->prof);
         hipLaunchKernelGGL(k_dscale<double2>, dim3(grid_for((size_t)nkb * nb)), dim3(256), 0, s, nkb, nb, nhm,
                            kb_iat.get(), kb_off.get(), kb_nh.get(), deeq.get(),
                            reinterpret_cast<const double2*>(becp.get()), reinterpret_cast<double2*>(ps.get()));
         APW_LAUNCH_CHECK();
         gemm<double>(s, gws, ncu, 'N', 'N', npw, nb, nkb, 1.0, 0.0, vkb.get(), npwx, ps.get(), nkb, 1.0, 0.0, hpsi,
-                     ldh);
+                     ldh, &ctx->prof);
     } else {
         // FP32 arithmetic, FP32 accumulation; psi converted on the way in, the product
         // added to the FP64 hpsi on the way out
@@ -278,12 +278,12 @@ void Ham::nonlocal(const double* psi, int ld, double* hpsi, int ldh, int nb, int
         hipLaunchKernelGGL(k_cols_d2f, dim3(grid_for((size_t)npw * nb)), dim3(256), 0, s, npw, nb,
                            reinterpret_cast<const double2*>(psi), ld, reinterpret_cast<float2*>(p32), npw);
         APW_LAUNCH_CHECK();
-        gemm<float>(s, gws, ncu, 'C', 'N', nkb, nb, npw, 1.0, 0.0, vkb32.get(), npwx, p32, npw, 0.0, 0.0, b32, nkb);
+        gemm<float>(s, gws, ncu, 'C', 'N', nkb, nb, npw, 1.0, 0.0, vkb32.get(), npwx, p32, npw, 0.0, 0.0, b32, nkb, &ctx->prof);
         hipLaunchKernelGGL(k_dscale<float2>, dim3(grid_for((size_t)nkb * nb)), dim3(256), 0, s, nkb, nb, nhm,
                            kb_iat.get(), kb_off.get(), kb_nh.get(), deeq.get(), reinterpret_cast<const float2*>(b32),
                            reinterpret_cast<float2*>(s32));
         APW_LAUNCH_CHECK();
-        gemm<float>(s, gws, ncu, 'N', 'N', npw, nb, nkb, 1.0, 0.0, vkb32.get(), npwx, s32, nkb, 0.0, 0.0, hh, npw);
+        gemm<float>(s, gws, ncu, 'N', 'N', npw, nb, nkb, 1.0, 0.0, vkb32.get(), npwx, s32, nkb, 0.0, 0.0, hh, npw, &ctx->prof);
         hipLaunchKernelGGL(k_cols_addf, dim3(grid_for((size_t)npw * nb)), dim3(256), 0, s, npw, nb,
                            reinterpret_cast<const float2*>(hh), npw, reinterpret_cast<double2*>(hpsi), ldh);
         APW_LAUNCH_CHECK();

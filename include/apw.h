@@ -53,6 +53,16 @@ int apw_ctx_destroy(apw_ctx* ctx);
 int apw_ctx_synchronize(apw_ctx* ctx);
 void* apw_ctx_stream(apw_ctx* ctx);
 
+/* Kernel profiler (no counterpart in the reference, whose ModuleBase::timer is host-side,
+ * timer.cpp:71-130): when on, the library's kernels are bracketed by HIP events on the
+ * context stream and accumulated per kernel name with their algorithmic HBM bytes and
+ * flops per launch.  set_profiling clears the table.  profile_count resolves the events
+ * (waits for them) and returns the number of entries. */
+int apw_ctx_set_profiling(apw_ctx* ctx, int on);
+int apw_ctx_profile_count(apw_ctx* ctx);
+int apw_ctx_profile_entry(apw_ctx* ctx, int i, char* name, int name_cap, long long* launches, double* ms,
+                          double* bytes, double* flops);
+
 /* ------------------------------------------------------------------------------
  * Hamiltonian of one k-point (HamiltPW + PW_Basis_K + pseudopot_cell_vnl data).
  * Host arrays, uploaded once:

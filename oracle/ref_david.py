@@ -12,17 +12,22 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "_ref", "libref_david.so")
-_lib = None
+LIB_DETSIGN = os.path.join(HERE, "_ref", "libref_david_detsign.so")
+_libs = {}
 
 
-def available() -> bool:
-    return os.path.exists(LIB)
+def available(detsign: bool = False) -> bool:
+    return os.path.exists(LIB_DETSIGN if detsign else LIB)
 
 
-def load():
-    global _lib
+def load(detsign: bool = False):
+    """The reference build (``detsign``: with zheevx's all-eigenpair path re-phased to the
+    inverse-iteration convention, see ref_lapack_signs.cpp)."""
+    path = LIB_DETSIGN if detsign else LIB
+    _lib = _libs.get(path)
     if _lib is None:
-        _lib = C.CDLL(LIB)
+        _lib = C.CDLL(path)
+        _libs[path] = _lib
         P = C.c_void_p
         _lib.ref_david_diag.restype = C.c_int
         _lib.ref_david_diag.argtypes = [C.c_int] * 5 + [P, P, C.c_double, P, C.c_int, P, C.c_int, P, P, C.c_int, P,
@@ -64,10 +69,11 @@ def hpsi(s, psi, nb, ld):
     return out
 
 
-def david(s, thr=1e-9, maxiter=500, ntry_max=5, notconv_max=0, psi0=None, nband=None, dump_call=-1):
+def david(s, thr=1e-9, maxiter=500, ntry_max=5, notconv_max=0, psi0=None, nband=None, dump_call=-1,
+          detsign=False):
     """Reference DiagoDavid::diag from the system's start vectors.  Returns
     (eigenvalues, psi, iterations, hpsi_calls, hpsi_bands)."""
-    lib = load()
+    lib = load(detsign)
     a = _arrays(s)
     nband = nband or s.nband
     psi = np.array(s.psi0[:nband] if psi0 is None else psi0, dtype=np.complex128, copy=True, order="C")
