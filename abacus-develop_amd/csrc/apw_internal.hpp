@@ -6,6 +6,7 @@
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
+#include <deque>
 #include <string>
 #include <vector>
 
@@ -98,7 +99,7 @@ struct ProfEntry {
 
 struct Prof {
     bool on = false;
-    std::vector<ProfEntry> entries;
+    std::deque<ProfEntry> entries;   // deque: references stay valid while entries are added
     std::vector<hipEvent_t> pool;
     ProfEntry& entry(const char* name);
     hipEvent_t event();
