@@ -162,8 +162,6 @@ struct EigWorkspace {
     DevBuf<double> p;      // n c128
     DevBuf<double> z;      // n x nband real eigenvectors of T
     DevBuf<unsigned> bar;  // grid barrier counter + abort flag
-    DevBuf<double> lu;     // inverse-iteration LU factors, 4 n per eigenvalue
-    DevBuf<int> piv;
 };
 // Lowest nband eigenpairs of Hermitian hcc (n x n, lower triangle, ld ldh).
 // eig: device double[nband]; vcc: device c128 (ld ldv), columns 0..nband-1 rows 0..n-1.

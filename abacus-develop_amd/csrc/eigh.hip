@@ -94,7 +94,42 @@ __device__ void grid_barrier(unsigned* bar, unsigned& target, unsigned nwg, unsi
     __syncthreads();
 }
 
-// Householder tridiagonalisation, cooperative.  LDS: v[n], y[n] (c128) + reduction scratch.
+// Reflector (zlarfg convention) of x[0..mx): P^H x = beta e_1, P = I - tau v v^H, v[0] = 1.
+// x is read from xs (any memory), v written to vo (LDS, may alias xs shifted); every thread
+// of the block calls it and gets tau and beta.
+__device__ void make_reflector(const double2* xs, int mx, double2* vo, double2& tau, double& beta, double* red) {
+    double s2 = 0;
+    for (int i = 1 + threadIdx.x; i < mx; i += NT) {
+        const double2 x = xs[i];
+        s2 += x.x * x.x + x.y * x.y;
+    }
+    const double xn2 = block_sum2(s2, 0.0, red).x;
+    const double2 alpha = xs[0];
+    double2 sc;
+    if (xn2 == 0.0 && alpha.y == 0.0) {
+        tau = make_double2(0.0, 0.0);
+        beta = alpha.x;
+        sc = make_double2(0.0, 0.0);
+    } else {
+        const double nrm = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xn2);
+        beta = alpha.x >= 0 ? -nrm : nrm;
+        tau = make_double2((beta - alpha.x) / beta, -alpha.y / beta);
+        const double2 den = make_double2(alpha.x - beta, alpha.y);   // sc = 1 / (alpha - beta)
+        const double dd = den.x * den.x + den.y * den.y;
+        sc = make_double2(den.x / dd, -den.y / dd);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < mx; i += NT) vo[i] = (i == 0) ? make_double2(1.0, 0.0) : cmul(xs[i], sc);
+    __syncthreads();
+}
+
+// Householder tridiagonalisation (lower), cooperative, one grid barrier per column.
+// Column c of the trailing matrix is owned by global wave c % nwaves for the whole
+// reduction, so its reads (p = A v) and writes (rank-2 update) never cross waves.  What
+// crosses workgroups is p (double-buffered by column parity, so a workgroup one column
+// ahead cannot overwrite what a slower one still reads) and the column that yields the
+// next reflector: its owner leaves it untouched and every workgroup recomputes its
+// rank-2-updated values itself.  LDS: v, y, c (n c128 each) + reduction scratch.
 // bar[0]: barrier counter, bar[1]: abort flag (set on a barrier timeout).
 __global__ __launch_bounds__(NT) void k_tridiag(double2* __restrict__ A, int n, double2* __restrict__ V,
                                                 double2* __restrict__ tau, double* __restrict__ d,
@@ -102,76 +137,49 @@ __global__ __launch_bounds__(NT) void k_tridiag(double2* __restrict__ A, int n, 
     extern __shared__ __align__(16) unsigned char smem[];
     double2* v = reinterpret_cast<double2*>(smem);
     double2* y = v + n;
-    double* red = reinterpret_cast<double*>(y + n);
+    double2* cn = y + n;
+    double* red = reinterpret_cast<double*>(cn + n);
     const unsigned nwg = gridDim.x;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int gwave = blockIdx.x * (NT / 64) + wave;
     const int nwaves = nwg * (NT / 64);
     unsigned target = 0;
+    // reflector of column 0
+    double2 t;
+    double beta;
+    make_reflector(A + 1, n - 1, v, t, beta, red);
+    if (blockIdx.x == 0) {
+        for (int i = threadIdx.x; i < n - 1; i += NT) V[1 + i] = v[i];
+        if (threadIdx.x == 0) {
+            tau[0] = t;
+            d[0] = A[0].x;
+            e[0] = beta;
+        }
+    }
     for (int k = 0; k < n - 1; ++k) {
-        const int m = n - k - 1;
-        const double2* col = A + (size_t)k * n + (k + 1);   // x = A[k+1.., k]
-        // ---- phase 1: reflector (every workgroup, redundantly) ----
-        double s2 = 0;
-        for (int i = 1 + threadIdx.x; i < m; i += NT) {
-            const double2 x = col[i];
-            s2 += x.x * x.x + x.y * x.y;
-        }
-        const double2 r = block_sum2(s2, 0.0, red);
-        const double xn2 = r.x;
-        const double2 alpha = col[0];
-        double2 t;
-        double beta;
-        double2 sc;
-        bool ident = false;
-        if (xn2 == 0.0 && alpha.y == 0.0) {
-            t = make_double2(0.0, 0.0);
-            beta = alpha.x;
-            ident = true;
-            sc = make_double2(0.0, 0.0);
-        } else {
-            const double nrm = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xn2);
-            beta = alpha.x >= 0 ? -nrm : nrm;
-            t = make_double2((beta - alpha.x) / beta, -alpha.y / beta);
-            // sc = 1 / (alpha - beta)
-            const double2 den = make_double2(alpha.x - beta, alpha.y);
-            const double dd = den.x * den.x + den.y * den.y;
-            sc = make_double2(den.x / dd, -den.y / dd);
-        }
-        for (int i = threadIdx.x; i < m; i += NT) {
-            double2 vi;
-            if (i == 0) vi = make_double2(1.0, 0.0);
-            else vi = ident ? make_double2(0.0, 0.0) : cmul(col[i], sc);
-            v[i] = vi;
-        }
-        if (blockIdx.x == 0) {
-            for (int i = threadIdx.x; i < m; i += NT) V[(size_t)k * n + k + 1 + i] = v[i];
-            if (threadIdx.x == 0) {
-                tau[k] = t;
-                d[k] = A[(size_t)k * n + k].x;
-                e[k] = beta;
+        const int m = n - k - 1;           // trailing matrix rows/cols k+1 .. n-1
+        double2* Pk = P + (size_t)(k & 1) * n;
+        // ---- p = A_trail v on owned columns c = k+1+jj ----
+        {
+            int c0 = k + 1 + ((gwave - (k + 1)) % nwaves + nwaves) % nwaves;
+            for (int c = c0; c < n; c += nwaves) {
+                const double2* col = A + (size_t)c * n + (k + 1);
+                double pr = 0, pi = 0;
+                for (int i = lane; i < m; i += 64) {
+                    const double2 q = cconjmul(col[i], v[i]);   // conj(A[i][c]) v_i = A[c][i] v_i
+                    pr += q.x;
+                    pi += q.y;
+                }
+                pr = wave_sum(pr);
+                pi = wave_sum(pi);
+                if (lane == 0) Pk[c - (k + 1)] = make_double2(pr, pi);
             }
-        }
-        __syncthreads();
-        if (ident) continue;   // uniform over the grid: every workgroup computed the same tau
-        // ---- phase 2: p = A_trail v, one wave per owned column ----
-        for (int jj = gwave; jj < m; jj += nwaves) {
-            const double2* c = A + (size_t)(k + 1 + jj) * n + (k + 1);
-            double pr = 0, pi = 0;
-            for (int i = lane; i < m; i += 64) {
-                const double2 q = cconjmul(c[i], v[i]);   // conj(A[i][col]) v_i = A[col][i] v_i
-                pr += q.x;
-                pi += q.y;
-            }
-            pr = wave_sum(pr);
-            pi = wave_sum(pi);
-            if (lane == 0) P[jj] = make_double2(pr, pi);
         }
         grid_barrier(bar, target, nwg, bar + 1);
-        // ---- phase 3: y = tau p - (|tau|^2 mu / 2) v,  mu = v^H p ----
+        // ---- y = tau p - (|tau|^2 mu / 2) v,  mu = v^H p ----
         double mr = 0;
         for (int i = threadIdx.x; i < m; i += NT) {
-            const double2 p = P[i];
+            const double2 p = Pk[i];
             y[i] = p;
             mr += cconjmul(v[i], p).x;
         }
@@ -182,23 +190,55 @@ __global__ __launch_bounds__(NT) void k_tridiag(double2* __restrict__ A, int n, 
             y[i] = make_double2(tp.x - c2 * v[i].x, tp.y - c2 * v[i].y);
         }
         __syncthreads();
-        // ---- phase 4: A_trail -= y v^H + v y^H on owned columns ----
-        for (int jj = gwave; jj < m; jj += nwaves) {
-            double2* c = A + (size_t)(k + 1 + jj) * n + (k + 1);
-            const double2 vj = v[jj], yj = y[jj];
-            for (int i = lane; i < m; i += 64) {
-                // A[i][j] -= y_i conj(v_j) + v_i conj(y_j)
-                const double2 a1 = cconjmul(vj, y[i]);
-                const double2 a2 = cconjmul(yj, v[i]);
-                double2 a = c[i];
+        // ---- updated column k+1 (every workgroup): cn[i] = A[k+1+i][k+1] - y_i conj(v_0) - v_i conj(y_0)
+        {
+            const double2* col = A + (size_t)(k + 1) * n + (k + 1);
+            const double2 v0 = v[0], y0 = y[0];
+            for (int i = threadIdx.x; i < m; i += NT) {
+                const double2 a1 = cconjmul(v0, y[i]);
+                const double2 a2 = cconjmul(y0, v[i]);
+                double2 a = col[i];
                 a.x -= a1.x + a2.x;
                 a.y -= a1.y + a2.y;
-                c[i] = a;
+                cn[i] = a;
             }
         }
-        grid_barrier(bar, target, nwg, bar + 1);
+        __syncthreads();
+        // ---- rank-2 update of owned columns k+2 .. n-1, rows k+2 .. n-1 ----
+        {
+            int c0 = k + 2 + ((gwave - (k + 2)) % nwaves + nwaves) % nwaves;
+            for (int c = c0; c < n; c += nwaves) {
+                double2* col = A + (size_t)c * n + (k + 1);
+                const int jj = c - (k + 1);
+                const double2 vj = v[jj], yj = y[jj];
+                for (int i = 1 + lane; i < m; i += 64) {
+                    const double2 a1 = cconjmul(vj, y[i]);
+                    const double2 a2 = cconjmul(yj, v[i]);
+                    double2 a = col[i];
+                    a.x -= a1.x + a2.x;
+                    a.y -= a1.y + a2.y;
+                    col[i] = a;
+                }
+            }
+        }
+        __threadfence_block();
+        __syncthreads();
+        // ---- next reflector from the updated column (every workgroup, redundantly) ----
+        if (m >= 2) {
+            make_reflector(cn + 1, m - 1, v, t, beta, red);
+            if (blockIdx.x == 0) {
+                for (int i = threadIdx.x; i < m - 1; i += NT) V[(size_t)(k + 1) * n + (k + 2) + i] = v[i];
+                if (threadIdx.x == 0) {
+                    tau[k + 1] = t;
+                    d[k + 1] = cn[0].x;
+                    e[k + 1] = beta;
+                }
+            }
+        } else if (blockIdx.x == 0 && threadIdx.x == 0) {
+            d[k + 1] = cn[0].x;
+        }
+        __syncthreads();
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) d[n - 1] = A[(size_t)(n - 1) * n + (n - 1)].x;
 }
 
 // Sturm count: number of eigenvalues of T strictly below x
@@ -215,29 +255,48 @@ __device__ int sturm(const double* d, const double* e, int n, double x, double p
     return c;
 }
 
-__global__ void k_bisect(const double* __restrict__ d, const double* __restrict__ e, int n, int nband,
-                         double* __restrict__ w) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= nband) return;
+// Multisection: one wave per eigenvalue index j; each step the 64 lanes evaluate the Sturm
+// count at 64 interior points of [lo, hi] and the wave keeps the sub-interval holding the
+// j-th eigenvalue (a 65x shrink per step, ~9 steps to full precision).
+__global__ __launch_bounds__(256) void k_bisect(const double* __restrict__ d, const double* __restrict__ e, int n,
+                                                int nband, double* __restrict__ w) {
+    const int lane = threadIdx.x & 63;
+    const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (j >= nband) return;   // whole wave
     double lo = 1e300, hi = -1e300, emax2 = 0;
-    for (int i = 0; i < n; ++i) {
+    for (int i = lane; i < n; i += 64) {
         const double r = (i > 0 ? fabs(e[i - 1]) : 0.0) + (i < n - 1 ? fabs(e[i]) : 0.0);
         lo = fmin(lo, d[i] - r);
         hi = fmax(hi, d[i] + r);
         if (i < n - 1) emax2 = fmax(emax2, e[i] * e[i]);
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        lo = fmin(lo, __shfl_xor(lo, o, 64));
+        hi = fmax(hi, __shfl_xor(hi, o, 64));
+        emax2 = fmax(emax2, __shfl_xor(emax2, o, 64));
     }
     const double pivmin = DBL_MIN * fmax(1.0, emax2);
     const double tnorm = fmax(fabs(lo), fabs(hi));
     lo -= 2.0 * DBL_EPSILON * tnorm * n + 2 * pivmin;
     hi += 2.0 * DBL_EPSILON * tnorm * n + 2 * pivmin;
     // invariant: sturm(lo) <= j < sturm(hi)
-    for (int it = 0; it < 200; ++it) {
-        const double mid = 0.5 * (lo + hi);
-        if (hi - lo <= 2.0 * DBL_EPSILON * fmax(fabs(lo), fabs(hi)) + pivmin || mid == lo || mid == hi) break;
-        if (sturm(d, e, n, mid, pivmin) <= j) lo = mid;
-        else hi = mid;
+    for (int it = 0; it < 64; ++it) {
+        if (hi - lo <= 2.0 * DBL_EPSILON * fmax(fabs(lo), fabs(hi)) + pivmin) break;
+        const double x = lo + (hi - lo) * (double)(lane + 1) / 65.0;
+        const int c = sturm(d, e, n, x, pivmin);
+        const unsigned long long above = __ballot(c > j);
+        const double x63 = __shfl(x, 63, 64);
+        if (above == 0ull) {
+            lo = x63;
+        } else {
+            const int l = __ffsll((long long)above) - 1;
+            const double xl = __shfl(x, l, 64);
+            const double xlm = __shfl(x, l > 0 ? l - 1 : 0, 64);
+            if (l > 0) lo = xlm;
+            hi = xl;
+        }
     }
-    w[j] = 0.5 * (lo + hi);
+    if (lane == 0) w[j] = 0.5 * (lo + hi);
 }
 
 __device__ inline unsigned hash_u32(unsigned x) {
@@ -251,20 +310,20 @@ __device__ inline unsigned hash_u32(unsigned x) {
 
 // Inverse iteration, one workgroup per cluster of close eigenvalues (the workgroup of the
 // cluster's first index does the whole cluster; the others exit).
-// LU factors a, b, c, dd (n doubles each) and pivots live in a per-workgroup slice of lu /
-// lpiv (global, touched by thread 0 only); the iterate xv (n doubles) is in LDS.
+// LDS: the iterate xv, the LU factors a, b, c, dd (n doubles each, touched by thread 0) and
+// the pivots (n ints).
 __global__ __launch_bounds__(NT) void k_invit(const double* __restrict__ d, const double* __restrict__ e, int n,
                                               int nband, const double* __restrict__ w, double* __restrict__ Z,
-                                              double ctol_rel, double* __restrict__ lu, int* __restrict__ lpiv) {
+                                              double ctol_rel) {
     extern __shared__ __align__(16) unsigned char smem[];
     double* xv = reinterpret_cast<double*>(smem);
     double* red = xv + n;
-    const int j0 = blockIdx.x;
-    double* a = lu + (size_t)j0 * 4 * n;
+    double* a = red + 64;
     double* b = a + n;
     double* c = b + n;
     double* dd = c + n;
-    int* piv = lpiv + (size_t)j0 * n;
+    int* piv = reinterpret_cast<int*>(dd + n);
+    const int j0 = blockIdx.x;
     // ||T||_1 bound (Gershgorin)
     double tn = 0;
     for (int i = 0; i < n; ++i) {
@@ -514,13 +573,13 @@ __global__ __launch_bounds__(1024) void k_chol_inv(int n, const double2* __restr
 void heevx_lowest(hipStream_t s, EigWorkspace& ws, int num_cu, int n, int ldh, const double* hcc, int nband,
                   double* eig, double* vcc, int ldv, Prof* prof) {
     APW_CHECK(n >= 1 && nband >= 1 && nband <= n && ldh >= n && ldv >= n, "heevx: bad sizes n=%d nband=%d", n, nband);
-    APW_CHECK(n <= 2000, "heevx: n=%d too large (LDS of the reduction kernel)", n);
+    APW_CHECK(n <= 1300, "heevx: n=%d too large (LDS of the reduction kernel)", n);
     ws.a.reserve((size_t)2 * n * n);
     ws.v.reserve((size_t)2 * n * n);
     ws.tau.reserve((size_t)2 * n);
     ws.d.reserve(n);
     ws.e.reserve(n);
-    ws.p.reserve((size_t)2 * n);
+    ws.p.reserve((size_t)4 * n);
     ws.z.reserve((size_t)n * nband);
     ws.bar.reserve(2);
     double2* A = reinterpret_cast<double2*>(ws.a.get());
@@ -530,10 +589,10 @@ void heevx_lowest(hipStream_t s, EigWorkspace& ws, int num_cu, int n, int ldh, c
     if (n > 1) {
         hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, ws.bar.get(), 2);
         APW_LAUNCH_CHECK();
-        const size_t lds = (size_t)2 * n * sizeof(double2) + 64 * sizeof(double);
+        const size_t lds = (size_t)3 * n * sizeof(double2) + 64 * sizeof(double);
         int occ = 0;
         APW_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(k_tridiag), NT, lds));
-        int nwg = std::max(1, std::min(n / 8, num_cu));
+        int nwg = std::max(1, std::min(n / 16, std::min(num_cu, 64)));
         nwg = std::min(nwg, std::max(1, occ) * num_cu);
         double2* Vp = reinterpret_cast<double2*>(ws.v.get());
         double2* taup = reinterpret_cast<double2*>(ws.tau.get());
@@ -552,13 +611,11 @@ void heevx_lowest(hipStream_t s, EigWorkspace& ws, int num_cu, int n, int ldh, c
     }
     {
     ProfScope ps(prof, s, "eig_bisect_invit", 0.0, 0.0);
-    hipLaunchKernelGGL(k_bisect, dim3((nband + 63) / 64), dim3(64), 0, s, ws.d.get(), ws.e.get(), n, nband, eig);
+    hipLaunchKernelGGL(k_bisect, dim3((nband + 3) / 4), dim3(256), 0, s, ws.d.get(), ws.e.get(), n, nband, eig);
     APW_LAUNCH_CHECK();
-    const size_t lds_inv = (size_t)n * sizeof(double) + 64 * sizeof(double);
-    ws.lu.reserve((size_t)4 * n * nband);
-    ws.piv.reserve((size_t)n * nband);
+    const size_t lds_inv = (size_t)5 * n * sizeof(double) + 64 * sizeof(double) + (size_t)n * sizeof(int);
     hipLaunchKernelGGL(k_invit, dim3(nband), dim3(NT), lds_inv, s, ws.d.get(), ws.e.get(), n, nband, eig, ws.z.get(),
-                       1e-7, ws.lu.get(), ws.piv.get());
+                       1e-7);
     APW_LAUNCH_CHECK();
     }
     const size_t lds_bt = (size_t)n * sizeof(double2) + 64 * sizeof(double);

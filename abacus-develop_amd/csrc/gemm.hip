@@ -291,7 +291,9 @@ void gemm(hipStream_t s, GemmWorkspace& ws, int num_cu, char ta, char tb, int m,
     dim3 grid(tiles_m, tiles_n, splits);
     const double esz = 2.0 * sizeof(R);
     const bool beta_nz = (beta_re != 0.0 || beta_im != 0.0);
-    ProfScope ps(prof, s, std::is_same<R, double>::value ? "zgemm_f64" : "cgemm_f32",
+    const char* pname = std::is_same<R, double>::value ? (splits > 1 ? "zgemm_f64_splitk" : "zgemm_f64_tiled")
+                                                        : (splits > 1 ? "cgemm_f32_splitk" : "cgemm_f32_tiled");
+    ProfScope ps(prof, s, pname,
                  esz * ((double)m * k + (double)k * n + (double)m * n * (splits > 1 ? splits : (beta_nz ? 2 : 1))),
                  8.0 * m * n * (double)k);
 #define APW_GEMM_CASE(X, Y)                                                                            \

@@ -167,15 +167,97 @@ __device__ inline cplx<R> to_c(double2 v) {
 
 constexpr int NT = 256;
 
+// ---- compile-time plans ----------------------------------------------------------------
+// Lengths with a dedicated instantiation (every FFT length of the named workloads); other
+// 2-3-5 lengths run the same kernels with the plan read at run time.
+#define APW_FFT_SIZES(X) X(12) X(16) X(18) X(20) X(24) X(25) X(27) X(30) X(32) X(36) X(40) X(45) \
+    X(48) X(50) X(54) X(60) X(64) X(72) X(80) X(90) X(96) X(100) X(120) X(128) X(144) X(160)
+
+constexpr int ct_nstages(int n) {
+    int c = 0;
+    while (n % 4 == 0) { n /= 4; ++c; }
+    while (n % 2 == 0) { n /= 2; ++c; }
+    while (n % 3 == 0) { n /= 3; ++c; }
+    while (n % 5 == 0) { n /= 5; ++c; }
+    return c;
+}
+
+constexpr int ct_radix(int n, int st) {
+    int c = 0;
+    while (n % 4 == 0) { n /= 4; if (c++ == st) return 4; }
+    while (n % 2 == 0) { n /= 2; if (c++ == st) return 2; }
+    while (n % 3 == 0) { n /= 3; if (c++ == st) return 3; }
+    while (n % 5 == 0) { n /= 5; if (c++ == st) return 5; }
+    return 1;
+}
+
+// lines per workgroup: two LDS buffers of n * (L+1) complex doubles within ~64 KB
+constexpr int lines_for(int n) {
+    int L = 16;
+    while (L > 1 && 2 * n * (L + 1) * 16 > 66000) L /= 2;
+    return L;
+}
+
+// Stockham stages with the length, radices and direction known at compile time: every
+// index division and twiddle stride is a constant.  Returns the buffer holding the result.
+template <class R, int N, int ST, int NS, int SG>
+__device__ inline cplx<R>* ct_stages(cplx<R>* x, cplx<R>* y, const cplx<R>* tw, int ld) {
+    using C = cplx<R>;
+    if constexpr (ST == ct_nstages(N)) {
+        return x;
+    } else {
+        constexpr int RAD = ct_radix(N, ST);
+        constexpr int NB = N / RAD;
+        constexpr int STRIDE = N / (NS * RAD);
+        constexpr int L = lines_for(N);
+        for (int b = threadIdx.x; b < NB * L; b += NT) {
+            const int l = b % L;
+            const int j = b / L;
+            const int jm = j % NS;
+            C v[RAD];
+#pragma unroll
+            for (int r = 0; r < RAD; ++r) v[r] = x[(j + r * NB) * ld + l];
+            if constexpr (NS > 1) {
+#pragma unroll
+                for (int r = 1; r < RAD; ++r) {
+                    C t = tw[r * jm * STRIDE];
+                    if (SG > 0) t.y = -t.y;
+                    v[r] = cmul(v[r], t);
+                }
+            }
+            dft_radix<C, R>(v, RAD, SG);
+            const int base = (j / NS) * NS * RAD + jm;
+#pragma unroll
+            for (int r = 0; r < RAD; ++r) y[(base + r * NS) * ld + l] = v[r];
+        }
+        __syncthreads();
+        return ct_stages<R, N, ST + 1, NS * RAD, SG>(y, x, tw, ld);
+    }
+}
+
+// FFT of the L lines in LDS: compile-time path when NF != 0, run-time plan otherwise
+template <class R, int NF, int SG>
+__device__ inline cplx<R>* fft_lines(cplx<R>* x, cplx<R>* y, const cplx<R>* tw, const FftPlan& p, int L, int ld) {
+    if constexpr (NF != 0) {
+        return ct_stages<R, NF, 0, 1, SG>(x, y, tw, ld);
+    } else {
+        return lds_fft<R>(x, y, tw, p, L, ld, SG);
+    }
+}
+
+#define APW_LEN(NF, p) ((NF) != 0 ? (NF) : (p).n)
+#define APW_LINES(NF, Lrt) ((NF) != 0 ? lines_for(NF) : (Lrt))
+
 // ---- pass 1: z-backward on sticks, gathered from the sphere --------------------------
 // grid (ceil(ns/L), nb); LDS: 2 * nz * (L+1) + nz complex
-template <class R>
+template <class R, int NF>
 __global__ __launch_bounds__(NT) void k_zbwd(const double2* __restrict__ psi, int ld, const int* __restrict__ stick_ig,
                                              int ns, cplx<R>* __restrict__ S, FftPlan pz,
-                                             const cplx<R>* __restrict__ tw, int L) {
+                                             const cplx<R>* __restrict__ tw, int Lrt) {
     using C = cplx<R>;
     extern __shared__ __align__(16) unsigned char smem[];
-    const int nz = pz.n;
+    const int nz = APW_LEN(NF, pz);
+    const int L = APW_LINES(NF, Lrt);
     const int ldl = L + 1;
     C* x = reinterpret_cast<C*>(smem);
     C* y = x + nz * ldl;
@@ -184,7 +266,7 @@ __global__ __launch_bounds__(NT) void k_zbwd(const double2* __restrict__ psi, in
     const int s0 = blockIdx.x * L;
     const double2* pb = psi + (size_t)band * ld;
     load_tw<R>(tws, tw, nz);
-    for (int t = threadIdx.x; t < L * nz; t += blockDim.x) {
+    for (int t = threadIdx.x; t < L * nz; t += NT) {
         const int l = t / nz, iz = t - (t / nz) * nz;
         const int s = s0 + l;
         C v;
@@ -197,9 +279,9 @@ __global__ __launch_bounds__(NT) void k_zbwd(const double2* __restrict__ psi, in
         x[iz * ldl + l] = v;
     }
     __syncthreads();
-    C* res = lds_fft<R>(x, y, tws, pz, L, ldl, +1);
+    C* res = fft_lines<R, NF, +1>(x, y, tws, pz, L, ldl);
     C* Sb = S + (size_t)band * ns * nz;
-    for (int t = threadIdx.x; t < L * nz; t += blockDim.x) {
+    for (int t = threadIdx.x; t < L * nz; t += NT) {
         const int l = t / nz, iz = t - (t / nz) * nz;
         const int s = s0 + l;
         if (s < ns) Sb[(size_t)s * nz + iz] = res[iz * ldl + l];
@@ -208,14 +290,15 @@ __global__ __launch_bounds__(NT) void k_zbwd(const double2* __restrict__ psi, in
 
 // ---- pass 2: y-backward on the x slots -----------------------------------------------
 // grid (nxr, ceil(nz/L), nb); LDS 2 * ny * (L+1) + ny
-template <class R>
+template <class R, int NF>
 __global__ __launch_bounds__(NT) void k_ybwd(const cplx<R>* __restrict__ S, int ns, int nz, int nxr,
                                              const int* __restrict__ slot_start, const int* __restrict__ slot_cnt,
                                              const int* __restrict__ stick_iy, cplx<R>* __restrict__ X, FftPlan py,
-                                             const cplx<R>* __restrict__ tw, int L) {
+                                             const cplx<R>* __restrict__ tw, int Lrt) {
     using C = cplx<R>;
     extern __shared__ __align__(16) unsigned char smem[];
-    const int ny = py.n;
+    const int ny = APW_LEN(NF, py);
+    const int L = APW_LINES(NF, Lrt);
     const int ldl = L + 1;
     C* x = reinterpret_cast<C*>(smem);
     C* y = x + ny * ldl;
@@ -224,7 +307,7 @@ __global__ __launch_bounds__(NT) void k_ybwd(const cplx<R>* __restrict__ S, int 
     const int iz0 = blockIdx.y * L;
     const int band = blockIdx.z;
     load_tw<R>(tws, tw, ny);
-    for (int t = threadIdx.x; t < ny * ldl; t += blockDim.x) {
+    for (int t = threadIdx.x; t < ny * ldl; t += NT) {
         C z;
         z.x = 0;
         z.y = 0;
@@ -233,16 +316,16 @@ __global__ __launch_bounds__(NT) void k_ybwd(const cplx<R>* __restrict__ S, int 
     __syncthreads();
     const int st0 = slot_start[slot], cnt = slot_cnt[slot];
     const C* Sb = S + (size_t)band * ns * nz;
-    for (int t = threadIdx.x; t < cnt * L; t += blockDim.x) {
+    for (int t = threadIdx.x; t < cnt * L; t += NT) {
         const int q = t / L, dz = t - (t / L) * L;
         const int iz = iz0 + dz;
         const int s = st0 + q;
         if (iz < nz) x[stick_iy[s] * ldl + dz] = Sb[(size_t)s * nz + iz];
     }
     __syncthreads();
-    C* res = lds_fft<R>(x, y, tws, py, L, ldl, +1);
+    C* res = fft_lines<R, NF, +1>(x, y, tws, py, L, ldl);
     C* Xb = X + ((size_t)band * nxr + slot) * ny * nz;
-    for (int t = threadIdx.x; t < ny * L; t += blockDim.x) {
+    for (int t = threadIdx.x; t < ny * L; t += NT) {
         const int iy = t / L, dz = t - (t / L) * L;
         const int iz = iz0 + dz;
         if (iz < nz) Xb[(size_t)iy * nz + iz] = res[iy * ldl + dz];
@@ -251,14 +334,15 @@ __global__ __launch_bounds__(NT) void k_ybwd(const cplx<R>* __restrict__ S, int 
 
 // ---- pass 3: x pass.  MODE 0: bwd, *veff, fwd (X -> X); 1: bwd, write full box; 2: read full box, fwd
 // grid (ny, ceil(nz/L), nb); LDS 2 * nx * (L+1) + nx
-template <class R, int MODE>
+template <class R, int MODE, int NF>
 __global__ __launch_bounds__(NT) void k_xpass(cplx<R>* __restrict__ X, int nxr, int ny, int nz,
                                               const int* __restrict__ slot_ix, const R* __restrict__ veff,
                                               double2* __restrict__ box, int add, double factor, FftPlan px,
-                                              const cplx<R>* __restrict__ tw, int L) {
+                                              const cplx<R>* __restrict__ tw, int Lrt) {
     using C = cplx<R>;
     extern __shared__ __align__(16) unsigned char smem[];
-    const int nx = px.n;
+    const int nx = APW_LEN(NF, px);
+    const int L = APW_LINES(NF, Lrt);
     const int ldl = L + 1;
     C* x = reinterpret_cast<C*>(smem);
     C* y = x + nx * ldl;
@@ -271,7 +355,7 @@ __global__ __launch_bounds__(NT) void k_xpass(cplx<R>* __restrict__ X, int nxr, 
     const size_t nxyz = (size_t)nx * ny * nz;
     double2* boxb = box ? box + (size_t)band * nxyz : nullptr;
     if (MODE == 2) {
-        for (int t = threadIdx.x; t < nx * L; t += blockDim.x) {
+        for (int t = threadIdx.x; t < nx * L; t += NT) {
             const int ix = t / L, dz = t - (t / L) * L;
             const int iz = iz0 + dz;
             C v;
@@ -281,14 +365,14 @@ __global__ __launch_bounds__(NT) void k_xpass(cplx<R>* __restrict__ X, int nxr, 
             x[ix * ldl + dz] = v;
         }
     } else {
-        for (int t = threadIdx.x; t < nx * ldl; t += blockDim.x) {
+        for (int t = threadIdx.x; t < nx * ldl; t += NT) {
             C z;
             z.x = 0;
             z.y = 0;
             x[t] = z;
         }
         __syncthreads();
-        for (int t = threadIdx.x; t < nxr * L; t += blockDim.x) {
+        for (int t = threadIdx.x; t < nxr * L; t += NT) {
             const int q = t / L, dz = t - (t / L) * L;
             const int iz = iz0 + dz;
             if (iz < nz) x[slot_ix[q] * ldl + dz] = Xb[((size_t)q * ny + iy) * nz + iz];
@@ -296,9 +380,9 @@ __global__ __launch_bounds__(NT) void k_xpass(cplx<R>* __restrict__ X, int nxr, 
     }
     __syncthreads();
     C* res = x;
-    if (MODE != 2) res = lds_fft<R>(x, y, tws, px, L, ldl, +1);
+    if (MODE != 2) res = fft_lines<R, NF, +1>(x, y, tws, px, L, ldl);
     if (MODE == 1) {
-        for (int t = threadIdx.x; t < nx * L; t += blockDim.x) {
+        for (int t = threadIdx.x; t < nx * L; t += NT) {
             const int ix = t / L, dz = t - (t / L) * L;
             const int iz = iz0 + dz;
             if (iz < nz) {
@@ -319,7 +403,7 @@ __global__ __launch_bounds__(NT) void k_xpass(cplx<R>* __restrict__ X, int nxr, 
         return;
     }
     if (MODE == 0) {
-        for (int t = threadIdx.x; t < nx * L; t += blockDim.x) {
+        for (int t = threadIdx.x; t < nx * L; t += NT) {
             const int ix = t / L, dz = t - (t / L) * L;
             const int iz = iz0 + dz;
             if (iz < nz) {
@@ -333,8 +417,8 @@ __global__ __launch_bounds__(NT) void k_xpass(cplx<R>* __restrict__ X, int nxr, 
         __syncthreads();
     }
     C* other = (res == x) ? y : x;
-    C* out = lds_fft<R>(res, other, tws, px, L, ldl, -1);
-    for (int t = threadIdx.x; t < nxr * L; t += blockDim.x) {
+    C* out = fft_lines<R, NF, -1>(res, other, tws, px, L, ldl);
+    for (int t = threadIdx.x; t < nxr * L; t += NT) {
         const int q = t / L, dz = t - (t / L) * L;
         const int iz = iz0 + dz;
         if (iz < nz) Xb[((size_t)q * ny + iy) * nz + iz] = out[slot_ix[q] * ldl + dz];
@@ -342,14 +426,15 @@ __global__ __launch_bounds__(NT) void k_xpass(cplx<R>* __restrict__ X, int nxr, 
 }
 
 // ---- pass 4: y-forward, keeping the stick rows -----------------------------------------
-template <class R>
+template <class R, int NF>
 __global__ __launch_bounds__(NT) void k_yfwd(const cplx<R>* __restrict__ X, int ns, int nz, int nxr,
                                              const int* __restrict__ slot_start, const int* __restrict__ slot_cnt,
                                              const int* __restrict__ stick_iy, cplx<R>* __restrict__ S, FftPlan py,
-                                             const cplx<R>* __restrict__ tw, int L) {
+                                             const cplx<R>* __restrict__ tw, int Lrt) {
     using C = cplx<R>;
     extern __shared__ __align__(16) unsigned char smem[];
-    const int ny = py.n;
+    const int ny = APW_LEN(NF, py);
+    const int L = APW_LINES(NF, Lrt);
     const int ldl = L + 1;
     C* x = reinterpret_cast<C*>(smem);
     C* y = x + ny * ldl;
@@ -359,7 +444,7 @@ __global__ __launch_bounds__(NT) void k_yfwd(const cplx<R>* __restrict__ X, int 
     const int band = blockIdx.z;
     load_tw<R>(tws, tw, ny);
     const C* Xb = X + ((size_t)band * nxr + slot) * ny * nz;
-    for (int t = threadIdx.x; t < ny * L; t += blockDim.x) {
+    for (int t = threadIdx.x; t < ny * L; t += NT) {
         const int iy = t / L, dz = t - (t / L) * L;
         const int iz = iz0 + dz;
         C v;
@@ -369,10 +454,10 @@ __global__ __launch_bounds__(NT) void k_yfwd(const cplx<R>* __restrict__ X, int 
         x[iy * ldl + dz] = v;
     }
     __syncthreads();
-    C* res = lds_fft<R>(x, y, tws, py, L, ldl, -1);
+    C* res = fft_lines<R, NF, -1>(x, y, tws, py, L, ldl);
     const int st0 = slot_start[slot], cnt = slot_cnt[slot];
     C* Sb = S + (size_t)band * ns * nz;
-    for (int t = threadIdx.x; t < cnt * L; t += blockDim.x) {
+    for (int t = threadIdx.x; t < cnt * L; t += NT) {
         const int q = t / L, dz = t - (t / L) * L;
         const int iz = iz0 + dz;
         const int s = st0 + q;
@@ -381,14 +466,15 @@ __global__ __launch_bounds__(NT) void k_yfwd(const cplx<R>* __restrict__ X, int 
 }
 
 // ---- pass 5: z-forward on sticks, gathered into hpsi ----------------------------------
-template <class R>
+template <class R, int NF>
 __global__ __launch_bounds__(NT) void k_zfwd(const cplx<R>* __restrict__ S, const int* __restrict__ stick_ig, int ns,
                                              double2* __restrict__ hpsi, int ldh, const double2* __restrict__ psi,
                                              int ld, const double* __restrict__ gk2, double tpiba2, int mode,
-                                             double scale, FftPlan pz, const cplx<R>* __restrict__ tw, int L) {
+                                             double scale, FftPlan pz, const cplx<R>* __restrict__ tw, int Lrt) {
     using C = cplx<R>;
     extern __shared__ __align__(16) unsigned char smem[];
-    const int nz = pz.n;
+    const int nz = APW_LEN(NF, pz);
+    const int L = APW_LINES(NF, Lrt);
     const int ldl = L + 1;
     C* x = reinterpret_cast<C*>(smem);
     C* y = x + nz * ldl;
@@ -397,7 +483,7 @@ __global__ __launch_bounds__(NT) void k_zfwd(const cplx<R>* __restrict__ S, cons
     const int s0 = blockIdx.x * L;
     load_tw<R>(tws, tw, nz);
     const C* Sb = S + (size_t)band * ns * nz;
-    for (int t = threadIdx.x; t < L * nz; t += blockDim.x) {
+    for (int t = threadIdx.x; t < L * nz; t += NT) {
         const int l = t / nz, iz = t - (t / nz) * nz;
         const int s = s0 + l;
         C v;
@@ -407,10 +493,10 @@ __global__ __launch_bounds__(NT) void k_zfwd(const cplx<R>* __restrict__ S, cons
         x[iz * ldl + l] = v;
     }
     __syncthreads();
-    C* res = lds_fft<R>(x, y, tws, pz, L, ldl, -1);
+    C* res = fft_lines<R, NF, -1>(x, y, tws, pz, L, ldl);
     double2* hb = hpsi + (size_t)band * ldh;
     const double2* pb = psi ? psi + (size_t)band * ld : nullptr;
-    for (int t = threadIdx.x; t < L * nz; t += blockDim.x) {
+    for (int t = threadIdx.x; t < L * nz; t += NT) {
         const int l = t / nz, iz = t - (t / nz) * nz;
         const int s = s0 + l;
         if (s >= ns) continue;
@@ -436,13 +522,6 @@ __global__ __launch_bounds__(NT) void k_zfwd(const cplx<R>* __restrict__ S, cons
 
 double fft_flops(int n) { return 5.0 * n * std::log2((double)n); }
 
-int lines_for(int n) {
-    // 2 buffers of n * (L+1) complex doubles within ~64 KB of LDS
-    int L = 16;
-    while (L > 1 && 2 * (size_t)n * (L + 1) * 16 > 66000) L /= 2;
-    return L;
-}
-
 template <class R>
 size_t lds_bytes(int n, int L) {
     return (2 * (size_t)n * (L + 1) + n) * sizeof(cplx<R>);
@@ -454,6 +533,15 @@ const cplx<R>* tw_ptr(const PwGrid& g, int off) {
     return reinterpret_cast<const cplx<R>*>(g.tw32.get() + 2 * off);
 }
 
+// launch KERNEL<..., NF> with NF = n when n has a dedicated instantiation, else NF = 0
+#define APW_CASE_LAUNCH(N) \
+    case N: APW_DO_LAUNCH(N); break;
+#define APW_DISPATCH(n)                      \
+    switch (n) {                             \
+        APW_FFT_SIZES(APW_CASE_LAUNCH)       \
+        default: APW_DO_LAUNCH(0); break;    \
+    }
+
 template <class R>
 void pass_forward_half(hipStream_t s, const PwGrid& g, R* S, R* X, int nb, double2* hpsi, int ldh,
                        const double2* psi, int ld, const double* gk2, double tpiba2, GatherMode mode, double scale,
@@ -463,17 +551,23 @@ void pass_forward_half(hipStream_t s, const PwGrid& g, R* S, R* X, int nb, doubl
     const double cs = sizeof(C);
     const double nSz = (double)g.ns * g.nz, nX = (double)g.nxr * g.ny * g.nz;
     {
-    ProfScope ps(prof, s, "fft_y_fwd", nb * cs * (nX + nSz), nb * fft_flops(g.ny) * g.nxr * g.nz);
-    hipLaunchKernelGGL((k_yfwd<R>), dim3(g.nxr, (g.nz + Ly - 1) / Ly, nb), dim3(NT), lds_bytes<R>(g.ny, Ly), s,
-                       reinterpret_cast<const C*>(X), g.ns, g.nz, g.nxr, g.slot_start.get(), g.slot_cnt.get(),
-                       g.stick_iy.get(), reinterpret_cast<C*>(S), g.py, tw_ptr<R>(g, g.twoff_y), Ly);
-    APW_LAUNCH_CHECK();
+        ProfScope ps(prof, s, "fft_y_fwd", nb * cs * (nX + nSz), nb * fft_flops(g.ny) * g.nxr * g.nz);
+#define APW_DO_LAUNCH(NF)                                                                                         \
+    hipLaunchKernelGGL((k_yfwd<R, NF>), dim3(g.nxr, (g.nz + Ly - 1) / Ly, nb), dim3(NT), lds_bytes<R>(g.ny, Ly), s, \
+                       reinterpret_cast<const C*>(X), g.ns, g.nz, g.nxr, g.slot_start.get(), g.slot_cnt.get(),     \
+                       g.stick_iy.get(), reinterpret_cast<C*>(S), g.py, tw_ptr<R>(g, g.twoff_y), Ly)
+        APW_DISPATCH(g.ny)
+#undef APW_DO_LAUNCH
+        APW_LAUNCH_CHECK();
     }
     const double pwio = 16.0 * g.npw * ((mode == GATHER_SET) ? 1 : 2);
     ProfScope ps(prof, s, "fft_z_fwd_gather", nb * (cs * nSz + pwio), nb * (fft_flops(g.nz) * g.ns + 4.0 * g.npw));
-    hipLaunchKernelGGL((k_zfwd<R>), dim3((g.ns + Lz - 1) / Lz, nb), dim3(NT), lds_bytes<R>(g.nz, Lz), s,
-                       reinterpret_cast<const C*>(S), g.stick_ig.get(), g.ns, hpsi, ldh, psi, ld, gk2, tpiba2,
-                       (int)mode, scale, g.pz, tw_ptr<R>(g, g.twoff_z), Lz);
+#define APW_DO_LAUNCH(NF)                                                                                           \
+    hipLaunchKernelGGL((k_zfwd<R, NF>), dim3((g.ns + Lz - 1) / Lz, nb), dim3(NT), lds_bytes<R>(g.nz, Lz), s,          \
+                       reinterpret_cast<const C*>(S), g.stick_ig.get(), g.ns, hpsi, ldh, psi, ld, gk2, tpiba2,       \
+                       (int)mode, scale, g.pz, tw_ptr<R>(g, g.twoff_z), Lz)
+    APW_DISPATCH(g.nz)
+#undef APW_DO_LAUNCH
     APW_LAUNCH_CHECK();
 }
 
@@ -484,15 +578,21 @@ void pass_backward_half(hipStream_t s, const PwGrid& g, const double2* psi, int 
     const double cs = sizeof(C);
     const double nSz = (double)g.ns * g.nz, nX = (double)g.nxr * g.ny * g.nz;
     {
-    ProfScope ps(prof, s, "fft_z_bwd_scatter", nb * (16.0 * g.npw + cs * nSz), nb * fft_flops(g.nz) * g.ns);
-    hipLaunchKernelGGL((k_zbwd<R>), dim3((g.ns + Lz - 1) / Lz, nb), dim3(NT), lds_bytes<R>(g.nz, Lz), s, psi, ld,
-                       g.stick_ig.get(), g.ns, reinterpret_cast<C*>(S), g.pz, tw_ptr<R>(g, g.twoff_z), Lz);
-    APW_LAUNCH_CHECK();
+        ProfScope ps(prof, s, "fft_z_bwd_scatter", nb * (16.0 * g.npw + cs * nSz), nb * fft_flops(g.nz) * g.ns);
+#define APW_DO_LAUNCH(NF)                                                                                        \
+    hipLaunchKernelGGL((k_zbwd<R, NF>), dim3((g.ns + Lz - 1) / Lz, nb), dim3(NT), lds_bytes<R>(g.nz, Lz), s, psi, \
+                       ld, g.stick_ig.get(), g.ns, reinterpret_cast<C*>(S), g.pz, tw_ptr<R>(g, g.twoff_z), Lz)
+        APW_DISPATCH(g.nz)
+#undef APW_DO_LAUNCH
+        APW_LAUNCH_CHECK();
     }
     ProfScope ps(prof, s, "fft_y_bwd", nb * cs * (nSz + nX), nb * fft_flops(g.ny) * g.nxr * g.nz);
-    hipLaunchKernelGGL((k_ybwd<R>), dim3(g.nxr, (g.nz + Ly - 1) / Ly, nb), dim3(NT), lds_bytes<R>(g.ny, Ly), s,
-                       reinterpret_cast<const C*>(S), g.ns, g.nz, g.nxr, g.slot_start.get(), g.slot_cnt.get(),
-                       g.stick_iy.get(), reinterpret_cast<C*>(X), g.py, tw_ptr<R>(g, g.twoff_y), Ly);
+#define APW_DO_LAUNCH(NF)                                                                                         \
+    hipLaunchKernelGGL((k_ybwd<R, NF>), dim3(g.nxr, (g.nz + Ly - 1) / Ly, nb), dim3(NT), lds_bytes<R>(g.ny, Ly), s, \
+                       reinterpret_cast<const C*>(S), g.ns, g.nz, g.nxr, g.slot_start.get(), g.slot_cnt.get(),     \
+                       g.stick_iy.get(), reinterpret_cast<C*>(X), g.py, tw_ptr<R>(g, g.twoff_y), Ly)
+    APW_DISPATCH(g.ny)
+#undef APW_DO_LAUNCH
     APW_LAUNCH_CHECK();
 }
 
@@ -583,13 +683,16 @@ void local_hpsi(hipStream_t s, const PwGrid& g, const double* psi, int ld, doubl
     pass_backward_half<R>(s, g, p2, ld, S, X, nb, prof);
     const int Lx = lines_for(g.nx);
     {
-    const double nX = (double)g.nxr * g.ny * g.nz, nxyz = (double)g.nx * g.ny * g.nz;
-    ProfScope ps(prof, s, "fft_x_veff_fused", nb * (2.0 * sizeof(C) * nX + sizeof(R) * nxyz),
-                 nb * (2.0 * fft_flops(g.nx) * g.ny * g.nz + 2.0 * nxyz));
-    hipLaunchKernelGGL((k_xpass<R, 0>), dim3(g.ny, (g.nz + Lx - 1) / Lx, nb), dim3(NT), lds_bytes<R>(g.nx, Lx), s,
-                       reinterpret_cast<C*>(X), g.nxr, g.ny, g.nz, g.slot_ix.get(), veff, (double2*)nullptr, 0, 0.0,
-                       g.px, tw_ptr<R>(g, g.twoff_x), Lx);
-    APW_LAUNCH_CHECK();
+        const double nX = (double)g.nxr * g.ny * g.nz, nxyz = (double)g.nx * g.ny * g.nz;
+        ProfScope ps(prof, s, "fft_x_veff_fused", nb * (2.0 * sizeof(C) * nX + sizeof(R) * nxyz),
+                     nb * (2.0 * fft_flops(g.nx) * g.ny * g.nz + 2.0 * nxyz));
+#define APW_DO_LAUNCH(NF)                                                                                        \
+    hipLaunchKernelGGL((k_xpass<R, 0, NF>), dim3(g.ny, (g.nz + Lx - 1) / Lx, nb), dim3(NT), lds_bytes<R>(g.nx, Lx), \
+                       s, reinterpret_cast<C*>(X), g.nxr, g.ny, g.nz, g.slot_ix.get(), veff, (double2*)nullptr, 0,  \
+                       0.0, g.px, tw_ptr<R>(g, g.twoff_x), Lx)
+        APW_DISPATCH(g.nx)
+#undef APW_DO_LAUNCH
+        APW_LAUNCH_CHECK();
     }
     pass_forward_half<R>(s, g, S, X, nb, reinterpret_cast<double2*>(hpsi), ldh, p2, ld, gk2, tpiba2, mode, scale, prof);
 }
@@ -604,7 +707,7 @@ void sphere_to_box(hipStream_t s, const PwGrid& g, const double* psi, int ld, do
     using C = cplx<double>;
     pass_backward_half<double>(s, g, reinterpret_cast<const double2*>(psi), ld, S, X, nb, nullptr);
     const int Lx = lines_for(g.nx);
-    hipLaunchKernelGGL((k_xpass<double, 1>), dim3(g.ny, (g.nz + Lx - 1) / Lx, nb), dim3(NT),
+    hipLaunchKernelGGL((k_xpass<double, 1, 0>), dim3(g.ny, (g.nz + Lx - 1) / Lx, nb), dim3(NT),
                        lds_bytes<double>(g.nx, Lx), s, reinterpret_cast<C*>(X), g.nxr, g.ny, g.nz, g.slot_ix.get(),
                        (const double*)nullptr, reinterpret_cast<double2*>(box), add ? 1 : 0, factor, g.px,
                        tw_ptr<double>(g, g.twoff_x), Lx);
@@ -615,7 +718,7 @@ void box_to_sphere(hipStream_t s, const PwGrid& g, const double* box, double* hp
                    GatherMode mode, double scale, double* S, double* X) {
     using C = cplx<double>;
     const int Lx = lines_for(g.nx);
-    hipLaunchKernelGGL((k_xpass<double, 2>), dim3(g.ny, (g.nz + Lx - 1) / Lx, nb), dim3(NT),
+    hipLaunchKernelGGL((k_xpass<double, 2, 0>), dim3(g.ny, (g.nz + Lx - 1) / Lx, nb), dim3(NT),
                        lds_bytes<double>(g.nx, Lx), s, reinterpret_cast<C*>(X), g.nxr, g.ny, g.nz, g.slot_ix.get(),
                        (const double*)nullptr, const_cast<double2*>(reinterpret_cast<const double2*>(box)), 0, 0.0,
                        g.px, tw_ptr<double>(g, g.twoff_x), Lx);

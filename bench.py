@@ -210,7 +210,9 @@ def main():
         "warmup_s": [round(t, 4) for t in warm],
         "roofline": roof,
         "kernels": {k: {"launches": v["launches"], "ms": round(v["ms"], 3),
-                        "avg_us": round(1e3 * v["ms"] / max(1, v["launches"]), 2)} for k, v in top},
+                        "avg_us": round(1e3 * v["ms"] / max(1, v["launches"]), 2),
+                        "GB_s": round(v["bytes"] / max(v["ms"], 1e-9) / 1e6, 1),
+                        "TFLOP_s": round(v["flops"] / max(v["ms"], 1e-9) / 1e9, 2)} for k, v in top},
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.workload, s, st64["hpsi_bands"], it64)
