@@ -47,10 +47,12 @@ struct Mfma<float> {
     __device__ static inline int drow(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 
-constexpr int BM = 64, BN = 64, BK = 16, NT = 256;
+constexpr int BK = 16, NT = 256;
 
-// op codes: 0 = N, 1 = T, 2 = C
-template <class R, int TA, int TB>
+// op codes: 0 = N, 1 = T, 2 = C.  Each of the 4 waves (2 x 2) owns (16 TM) x (16 TN) of the
+// (32 TM) x (32 TN) tile.  The next k-tile is fetched into registers while the MFMAs
+// consume the current one from LDS.
+template <class R, int TA, int TB, int TM, int TN>
 __global__ __launch_bounds__(NT) void zgemm_kernel(int M, int N, int K, int kchunk,
                                                    const typename Mfma<R>::c_t* __restrict__ A, int lda,
                                                    const typename Mfma<R>::c_t* __restrict__ B, int ldb,
@@ -59,6 +61,8 @@ __global__ __launch_bounds__(NT) void zgemm_kernel(int M, int N, int K, int kchu
                                                    typename Mfma<R>::c_t* __restrict__ part) {
     using c_t = typename Mfma<R>::c_t;
     using acc_t = typename Mfma<R>::acc_t;
+    constexpr int BM = 32 * TM, BN = 32 * TN;
+    constexpr int NA = BM * BK / NT, NB = BN * BK / NT;
     __shared__ R As_re[BK][BM], As_im[BK][BM], Bs_re[BK][BN], Bs_im[BK][BN];
 
     const int tid = threadIdx.x;
@@ -69,22 +73,22 @@ __global__ __launch_bounds__(NT) void zgemm_kernel(int M, int N, int K, int kchu
     const int kbeg = blockIdx.z * kchunk;
     const int kend = min(K, kbeg + kchunk);
 
-    acc_t acc_re[2][2], acc_im[2][2];
+    acc_t acc_re[TN][TM], acc_im[TN][TM];
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < TN; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b) {
+        for (int b = 0; b < TM; ++b) {
             acc_re[a][b] = acc_t{0, 0, 0, 0};
             acc_im[a][b] = acc_t{0, 0, 0, 0};
         }
 
-    for (int k0 = kbeg; k0 < kend; k0 += BK) {
-        c_t ra[4], rb[4];
+    c_t ra[NA], rb[NB];
+    auto fetch = [&](int k0) {
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
+        for (int i = 0; i < NA; ++i) {
             const int e = tid + NT * i;
             int mm, kk;
-            if (TA == 0) { mm = e & 63; kk = e >> 6; } else { kk = e & 15; mm = e >> 4; }
+            if (TA == 0) { mm = e % BM; kk = e / BM; } else { kk = e % BK; mm = e / BK; }
             const int gm = m0 + mm, gk = k0 + kk;
             c_t v = c_t{0, 0};
             if (gm < M && gk < kend) {
@@ -92,48 +96,63 @@ __global__ __launch_bounds__(NT) void zgemm_kernel(int M, int N, int K, int kchu
                 if (TA == 2) v.y = -v.y;
             }
             ra[i] = v;
-            int nn;
-            if (TB == 0) { kk = e & 15; nn = e >> 4; } else { nn = e & 63; kk = e >> 6; }
-            const int gn = n0 + nn;
-            const int gk2 = k0 + kk;
+        }
+#pragma unroll
+        for (int i = 0; i < NB; ++i) {
+            const int e = tid + NT * i;
+            int nn, kk;
+            if (TB == 0) { kk = e % BK; nn = e / BK; } else { nn = e % BN; kk = e / BN; }
+            const int gn = n0 + nn, gk = k0 + kk;
             c_t w = c_t{0, 0};
-            if (gn < N && gk2 < kend) {
-                w = (TB == 0) ? B[(size_t)gk2 + (size_t)gn * ldb] : B[(size_t)gn + (size_t)gk2 * ldb];
+            if (gn < N && gk < kend) {
+                w = (TB == 0) ? B[(size_t)gk + (size_t)gn * ldb] : B[(size_t)gn + (size_t)gk * ldb];
                 if (TB == 2) w.y = -w.y;
             }
             rb[i] = w;
         }
+    };
+
+    if (kbeg < kend) fetch(kbeg);
+    for (int k0 = kbeg; k0 < kend; k0 += BK) {
         __syncthreads();
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
+        for (int i = 0; i < NA; ++i) {
             const int e = tid + NT * i;
             int mm, kk;
-            if (TA == 0) { mm = e & 63; kk = e >> 6; } else { kk = e & 15; mm = e >> 4; }
+            if (TA == 0) { mm = e % BM; kk = e / BM; } else { kk = e % BK; mm = e / BK; }
             As_re[kk][mm] = ra[i].x;
             As_im[kk][mm] = ra[i].y;
-            int nn;
-            if (TB == 0) { kk = e & 15; nn = e >> 4; } else { nn = e & 63; kk = e >> 6; }
+        }
+#pragma unroll
+        for (int i = 0; i < NB; ++i) {
+            const int e = tid + NT * i;
+            int nn, kk;
+            if (TB == 0) { kk = e % BK; nn = e / BK; } else { nn = e % BN; kk = e / BN; }
             Bs_re[kk][nn] = rb[i].x;
             Bs_im[kk][nn] = rb[i].y;
         }
         __syncthreads();
+        if (k0 + BK < kend) fetch(k0 + BK);
 #pragma unroll
         for (int ks = 0; ks < BK / 4; ++ks) {
             const int kk = ks * 4 + (lane >> 4);
-            R xr[2], xi[2], yr[2], yi[2];
+            R xr[TN], xi[TN], yr[TM], yi[TM];
 #pragma unroll
-            for (int t = 0; t < 2; ++t) {
-                const int nn = wn * 32 + t * 16 + (lane & 15);
+            for (int t = 0; t < TN; ++t) {
+                const int nn = wn * 16 * TN + t * 16 + (lane & 15);
                 xr[t] = Bs_re[kk][nn];
                 xi[t] = Bs_im[kk][nn];
-                const int mm = wm * 32 + t * 16 + (lane & 15);
+            }
+#pragma unroll
+            for (int t = 0; t < TM; ++t) {
+                const int mm = wm * 16 * TM + t * 16 + (lane & 15);
                 yr[t] = As_re[kk][mm];
                 yi[t] = As_im[kk][mm];
             }
 #pragma unroll
-            for (int tn = 0; tn < 2; ++tn)
+            for (int tn = 0; tn < TN; ++tn)
 #pragma unroll
-                for (int tm = 0; tm < 2; ++tm) {
+                for (int tm = 0; tm < TM; ++tm) {
                     acc_re[tn][tm] = Mfma<R>::mma(xr[tn], yr[tm], acc_re[tn][tm]);
                     acc_re[tn][tm] = Mfma<R>::mma(-xi[tn], yi[tm], acc_re[tn][tm]);
                     acc_im[tn][tm] = Mfma<R>::mma(xr[tn], yi[tm], acc_im[tn][tm]);
@@ -144,13 +163,13 @@ __global__ __launch_bounds__(NT) void zgemm_kernel(int M, int N, int K, int kchu
 
     // epilogue: lane holds D[n = drow(lane, r)][m = lane%16] of each sub-tile
 #pragma unroll
-    for (int tn = 0; tn < 2; ++tn)
+    for (int tn = 0; tn < TN; ++tn)
 #pragma unroll
-        for (int tm = 0; tm < 2; ++tm)
+        for (int tm = 0; tm < TM; ++tm)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int gm = m0 + wm * 32 + tm * 16 + (lane & 15);
-                const int gn = n0 + wn * 32 + tn * 16 + Mfma<R>::drow(lane, r);
+                const int gm = m0 + wm * 16 * TM + tm * 16 + (lane & 15);
+                const int gn = n0 + wn * 16 * TN + tn * 16 + Mfma<R>::drow(lane, r);
                 if (gm >= M || gn >= N) continue;
                 const double xr = (double)acc_re[tn][tm][r];
                 const double xi = (double)acc_im[tn][tm][r];
@@ -238,11 +257,11 @@ int opcode(char t) {
     }
 }
 
-template <class R, int TA, int TB>
-void launch(hipStream_t s, dim3 grid, int M, int N, int K, int kchunk, const R* A, int lda, const R* B,
-            int ldb, R* C, int ldc, double ar, double ai, double br, double bi, R* part) {
+template <class R, int TA, int TB, int TM, int TN>
+void launch(hipStream_t s, dim3 grid, int M, int N, int K, int kchunk, const R* A, int lda, const R* B, int ldb,
+            R* C, int ldc, double ar, double ai, double br, double bi, R* part) {
     using c_t = typename Mfma<R>::c_t;
-    hipLaunchKernelGGL((zgemm_kernel<R, TA, TB>), grid, dim3(NT), 0, s, M, N, K, kchunk,
+    hipLaunchKernelGGL((zgemm_kernel<R, TA, TB, TM, TN>), grid, dim3(NT), 0, s, M, N, K, kchunk,
                        reinterpret_cast<const c_t*>(A), lda, reinterpret_cast<const c_t*>(B), ldb,
                        reinterpret_cast<c_t*>(C), ldc, ar, ai, br, bi, reinterpret_cast<c_t*>(part));
 }
@@ -269,6 +288,9 @@ void gemm(hipStream_t s, GemmWorkspace& ws, int num_cu, char ta, char tb, int m,
         APW_LAUNCH_CHECK();
         return;
     }
+    // tile shape: 128 x 64 for tall C (m >> n, the npw x nb updates), 64 x 128 for wide, else 64 x 64
+    const int shape = (m >= 4 * n && m >= 256) ? 1 : ((n >= 4 * m && n >= 256) ? 2 : 0);
+    const int BM = shape == 1 ? 128 : 64, BN = shape == 2 ? 128 : 64;
     const int tiles_m = (m + BM - 1) / BM, tiles_n = (n + BN - 1) / BN;
     const long tiles = (long)tiles_m * tiles_n;
     const int target = 2 * std::max(num_cu, 64);
@@ -296,10 +318,17 @@ void gemm(hipStream_t s, GemmWorkspace& ws, int num_cu, char ta, char tb, int m,
     ProfScope ps(prof, s, pname,
                  esz * ((double)m * k + (double)k * n + (double)m * n * (splits > 1 ? splits : (beta_nz ? 2 : 1))),
                  8.0 * m * n * (double)k);
-#define APW_GEMM_CASE(X, Y)                                                                            \
-    if (oa == X && ob == Y) {                                                                         \
-        launch<R, X, Y>(s, grid, m, n, k, kchunk, A, lda, B, ldb, C, ldc, alpha_re, alpha_im, beta_re, \
-                        beta_im, part);                                                               \
+#define APW_GEMM_CASE(X, Y)                                                                                  \
+    if (oa == X && ob == Y) {                                                                               \
+        if (shape == 1)                                                                                     \
+            launch<R, X, Y, 4, 2>(s, grid, m, n, k, kchunk, A, lda, B, ldb, C, ldc, alpha_re, alpha_im, beta_re, \
+                                  beta_im, part);                                                           \
+        else if (shape == 2)                                                                                \
+            launch<R, X, Y, 2, 4>(s, grid, m, n, k, kchunk, A, lda, B, ldb, C, ldc, alpha_re, alpha_im, beta_re, \
+                                  beta_im, part);                                                           \
+        else                                                                                                \
+            launch<R, X, Y, 2, 2>(s, grid, m, n, k, kchunk, A, lda, B, ldb, C, ldc, alpha_re, alpha_im, beta_re, \
+                                  beta_im, part);                                                           \
     }
     APW_GEMM_CASE(0, 0) APW_GEMM_CASE(0, 1) APW_GEMM_CASE(0, 2)
     APW_GEMM_CASE(1, 0) APW_GEMM_CASE(1, 1) APW_GEMM_CASE(1, 2)
