@@ -313,11 +313,13 @@ def fft3d(ctx, nx, ny, nz, box, direction):
 
 
 def gemm_op(ctx, transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc):
-    al = (C.c_double * 2)(complex(alpha).real, complex(alpha).imag)
-    be = (C.c_double * 2)(complex(beta).real, complex(beta).imag)
+    """gemm_op<complex<double>|complex<float>, DEVICE_GPU>: column-major, dtype from ``c``."""
+    single = str(getattr(c, "dtype", "")) == "torch.complex64"
+    ct, fn = (C.c_float, ctx.lib.apw_cgemm) if single else (C.c_double, ctx.lib.apw_zgemm)
+    al = (ct * 2)(complex(alpha).real, complex(alpha).imag)
+    be = (ct * 2)(complex(beta).real, complex(beta).imag)
     ctx.pre()
-    check(ctx.lib.apw_zgemm(ctx.handle, transa.encode(), transb.encode(), m, n, k, al, _ptr(a), lda, _ptr(b), ldb,
-                            be, _ptr(c), ldc))
+    check(fn(ctx.handle, transa.encode(), transb.encode(), m, n, k, al, _ptr(a), lda, _ptr(b), ldb, be, _ptr(c), ldc))
 
 
 def dnevx_op(ctx, nstart, ldh, hcc, nbands, eigenvalue, vcc):

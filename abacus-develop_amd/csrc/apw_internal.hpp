@@ -159,19 +159,24 @@ struct EigWorkspace {
     DevBuf<double> v;      // n x n c128 Householder vectors
     DevBuf<double> tau;    // n c128
     DevBuf<double> d, e;   // tridiagonal
-    DevBuf<double> p;      // n c128
+    DevBuf<double> p;      // 4n c128 exchange buffers of the global-memory reduction
+    DevBuf<double> ring;   // 2n(n-1) c128 exchange ring of the LDS-resident reduction
     DevBuf<double> z;      // n x nband real eigenvectors of T
     DevBuf<unsigned> bar;  // grid barrier counter + abort flag
 };
-// Lowest nband eigenpairs of Hermitian hcc (n x n, lower triangle, ld ldh).
+// Lowest nband eigenpairs of Hermitian hcc (n x n, lower triangle, ld ldh), n <= HEEVX_MAX.
+constexpr int HEEVX_MAX = 2000;
 // eig: device double[nband]; vcc: device c128 (ld ldv), columns 0..nband-1 rows 0..n-1.
 void heevx_lowest(hipStream_t s, EigWorkspace& ws, int num_cu, int n, int ldh, const double* hcc,
                   int nband, double* eig, double* vcc, int ldv, Prof* prof = nullptr);
 
 // ---------------------------------------------------------------- small dense helpers
-// Cholesky of Hermitian positive definite G (n x n, ld) and inverse of the upper factor:
-// G = R^H R, Rinv written to rinv (ld n).  status[0] != 0 on breakdown (pivot <= tiny).
-void chol_inv_upper(hipStream_t s, int n, const double* g, int ldg, double* rinv, int* status,
-                    double min_norm);
+// Cholesky of Hermitian positive definite G (n x n, ld ldg, upper triangle read) and inverse
+// of the upper factor: G = R^H R, R^-1 written to rinv (n x n, ld ldr, zeros below the
+// diagonal).  status[0] = j + 1 if pivot j is not positive or its square root is below
+// min_norm (then rinv is not written).  n <= CHOL_MAX (one register-resident workgroup).
+constexpr int CHOL_MAX = 128;
+void chol_inv_upper(hipStream_t s, int n, const double* g, int ldg, double* rinv, int ldr, int* status,
+                    double min_norm, Prof* prof = nullptr);
 
 }  // namespace apw

@@ -126,7 +126,7 @@ __global__ void k_refresh_diag(int nband, double2* __restrict__ hcc, double2* __
 struct David {
     apw_ctx* ctx = nullptr;
     int nband = 0, dim = 0, ndim = 0, nbx = 0;
-    DevBuf<double> pre, basis, hp, sp, hcc, vcc, eig, tmp, vc, lmat, gmat, rinv, resid;
+    DevBuf<double> pre, basis, hp, sp, hcc, vcc, eig, tmp, vc, lmat, resid;
     DevBuf<int> unconv_d, status;
     EigWorkspace ews;
     GemmWorkspace gws;
@@ -156,7 +156,8 @@ struct David {
         APW_CHECK(ndim > 1, "DiagoDavid: david_ndim must be > 1");                 // diago_david.cpp:60
         APW_CHECK((long)ndim * nband < (long)dim, "DiagoDavid: david_ndim*nband must be < dim");  // :61
         nbx = ndim * nband;
-        APW_CHECK(nbx <= 2000, "DiagoDavid: david_ndim*nband = %d exceeds the subspace solver limit", nbx);
+        APW_CHECK(nbx <= HEEVX_MAX, "DiagoDavid: david_ndim*nband = %d exceeds the subspace solver limit %d", nbx,
+                  HEEVX_MAX);
         pre.reserve(dim);
         APW_HIP(hipMemcpy(pre.get(), precondition, dim * sizeof(double), hipMemcpyHostToDevice));
         basis.reserve((size_t)2 * dim * nbx);
@@ -167,8 +168,6 @@ struct David {
         tmp.reserve((size_t)2 * dim * nband);
         vc.reserve((size_t)2 * nbx * nband);
         lmat.reserve((size_t)2 * nbx * nband);
-        gmat.reserve((size_t)2 * nband * nband);
-        rinv.reserve((size_t)4 * nband * nband);
         resid.reserve(nbx);
         unconv_d.reserve(nband);
         status.reserve(4);
@@ -220,23 +219,33 @@ struct David {
         }
     }
 
-    // S-orthonormalise basis[:, c0 .. c0+nc) against basis[:, 0 .. c0) and among themselves
+    // S-orthonormalise basis[:, c0 .. c0+nc) against basis[:, 0 .. c0) and among themselves:
+    // block classical Gram-Schmidt with Pythagorean norms (SchmidtOrth, diago_david.cpp:950-1075),
+    // in blocks of at most CHOL_MAX columns, each against everything before it.
     void orthonormalize(int c0, int nc) {
+        hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, st(), (size_t)2, reinterpret_cast<double*>(status.get()));
+        APW_LAUNCH_CHECK();
+        const int nblk = (nc + CHOL_MAX - 1) / CHOL_MAX;
+        const int bs = (nc + nblk - 1) / nblk;
+        for (int b0 = 0; b0 < nc; b0 += bs) orthonormalize_block(c0 + b0, std::min(bs, nc - b0));
+    }
+
+    // With Q = basis[:, 0..c0), V = basis[:, c0..c0+nc), W = [Q V]^H S V = [L; G0]:
+    // G = G0 - L^H L = R^H R,  V <- (V - Q L) R^-1 = [Q V] [-L R^-1; R^-1].
+    void orthonormalize_block(int c0, int nc) {
         double* V = Bd(basis.get(), c0);
         apply_s(c0, nc);
         double* SV = Bd(S(), c0);
-        double* G = gmat.get();
-        gemm('C', 'N', nc, nc, dim, 1.0, V, dim, SV, dim, 0.0, G, nc);
-        if (c0 > 0) {
-            double* L = lmat.get();
-            gemm('C', 'N', c0, nc, dim, 1.0, basis.get(), dim, SV, dim, 0.0, L, c0);   // L = Q^H S V
-            gemm('C', 'N', nc, nc, c0, -1.0, L, c0, L, c0, 1.0, G, nc);                // G -= L^H L
-            gemm('N', 'N', dim, nc, c0, -1.0, basis.get(), dim, L, c0, 1.0, V, dim);   // V -= Q L
-        }
-        hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, st(), (size_t)2, reinterpret_cast<double*>(status.get()));
-        APW_LAUNCH_CHECK();
-        chol_inv_upper(st(), nc, G, nc, rinv.get(), status.get(), 1.0e-12);
-        gemm('N', 'N', dim, nc, nc, 1.0, V, dim, rinv.get(), nc, 0.0, tmp.get(), dim);   // tmp = V R^-1
+        const int ld = c0 + nc;
+        double* W = lmat.get();                                        // ld x nc: [L; G]
+        double* G = reinterpret_cast<double*>(reinterpret_cast<double2*>(W) + c0);
+        double* X = vc.get();                                          // ld x nc: [-L R^-1; R^-1]
+        double* Rinv = reinterpret_cast<double*>(reinterpret_cast<double2*>(X) + c0);
+        gemm('C', 'N', ld, nc, dim, 1.0, basis.get(), dim, SV, dim, 0.0, W, ld);
+        if (c0 > 0) gemm('C', 'N', nc, nc, c0, -1.0, W, ld, W, ld, 1.0, G, ld);
+        chol_inv_upper(st(), nc, G, ld, Rinv, ld, status.get(), 1.0e-12, &ctx->prof);
+        if (c0 > 0) gemm('N', 'N', c0, nc, nc, -1.0, W, ld, Rinv, ld, 0.0, X, ld);
+        gemm('N', 'N', dim, nc, ld, 1.0, basis.get(), dim, X, ld, 0.0, tmp.get(), dim);
         hipLaunchKernelGGL(k_copy_cols, dim3(gsz((size_t)dim * nc)), dim3(256), 0, st(), dim, nc,
                            reinterpret_cast<const double2*>(tmp.get()), dim, reinterpret_cast<double2*>(V), dim);
         APW_LAUNCH_CHECK();
@@ -245,11 +254,17 @@ struct David {
 
     void check_status() {
         int h[2] = {0, 0};
+        unsigned b[2] = {0, 0};
         APW_HIP(hipMemcpyAsync(h, status.get(), sizeof(h), hipMemcpyDeviceToHost, st()));
+        if (ews.bar.get()) APW_HIP(hipMemcpyAsync(b, ews.bar.get(), sizeof(b), hipMemcpyDeviceToHost, st()));
         APW_HIP(hipStreamSynchronize(st()));
         if (h[0] != 0) {
             set_error("DiagoDavid::SchmidtOrth: psi_norm < 1.0e-12 (vector %d of the block)", h[0] - 1);
             throw Error{APW_EBREAKDOWN};
+        }
+        if (b[1] != 0) {
+            set_error("DiagoDavid: subspace eigensolver grid barrier timed out");
+            throw Error{APW_EEIG};
         }
     }
 

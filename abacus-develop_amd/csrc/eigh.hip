@@ -5,7 +5,8 @@
 // (rocm/dngvd_op.hip.cu:146-170).  Same algorithm family as zheevx, all on the device:
 //   1. Householder reduction of the lower triangle to a real symmetric tridiagonal T
 //      (zhetrd 'L' form, unblocked) in one cooperative kernel: the trailing matrix is
-//      partitioned by columns over all waves of the grid, two grid barriers per column;
+//      partitioned by columns over the workgroups and held in their LDS (global memory
+//      for n too large for that), one grid barrier per column;
 //   2. bisection with Sturm counts for eigenvalues 0..nband-1 of T (dstebz form);
 //   3. inverse iteration with partial-pivoting tridiagonal LU and Gram-Schmidt inside
 //      clusters of close eigenvalues (dstein form);
@@ -21,6 +22,7 @@ namespace {
 
 constexpr int NT = 256;
 constexpr unsigned SPIN_LIMIT = 1u << 24;
+constexpr size_t TRIDIAG_LDS_BYTES = 160 * 1024 - 512;   // LDS per workgroup on gfx950, less static use
 
 __device__ inline double2 cmul(double2 a, double2 b) { return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
 __device__ inline double2 cconjmul(double2 a, double2 b) {  // conj(a) * b
@@ -241,14 +243,185 @@ __global__ __launch_bounds__(NT) void k_tridiag(double2* __restrict__ A, int n, 
     }
 }
 
+// Cross-workgroup exchange of the LDS-resident reduction: a ring with one slot per
+// column step, pre-filled with a NaN bit pattern no arithmetic produces.  Producers store
+// each value with device-scope atomic stores (they bypass the non-coherent caches);
+// consumers poll the slots they need with device-scope atomic loads until no word holds
+// the pattern.  Every value validates itself, so there is no flag, no fence and no grid
+// barrier: one memory round trip per column.
+constexpr unsigned long long RING_EMPTY = 0x7ff4dead5eed0001ull;
+
+__global__ void k_fill_ring(double2* ring, size_t count) {
+    const double e = __longlong_as_double((long long)RING_EMPTY);
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x)
+        ring[i] = make_double2(e, e);
+}
+
+__device__ inline void ring_put(double2* slot, double2 v) {
+    __hip_atomic_store(&slot->x, v.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&slot->y, v.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ inline bool ring_get(const double2* slot, double2& v) {
+    v.x = __hip_atomic_load(&slot->x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    v.y = __hip_atomic_load(&slot->y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return (unsigned long long)__double_as_longlong(v.x) != RING_EMPTY &&
+           (unsigned long long)__double_as_longlong(v.y) != RING_EMPTY;
+}
+
+// Householder tridiagonalisation (lower) with the trailing matrix resident in LDS.
+// Column c belongs to workgroup c % nwg (slot c / nwg) for the whole reduction and lives
+// in that workgroup's LDS, so the hemv reads and rank-2 writes never touch global memory.
+// Per column, what crosses workgroups is p = A v on each workgroup's columns and the one
+// column that yields the next reflector (its owner publishes it un-updated; every
+// workgroup applies the rank-2 correction itself and forms the reflector redundantly).
+// Both go through the ring: slot k * 2n + i holds p_i of step k, k * 2n + n + i the
+// published column.  abort_flag: raised when a poll exceeds its spin bound.
+// LDS: ncol x n columns, v, y, c (n each), ncol spare, reduction scratch.
+__global__ __launch_bounds__(NT) void k_tridiag_lds(const double2* __restrict__ A, int n, int ncol,
+                                                    double2* __restrict__ V, double2* __restrict__ tau,
+                                                    double* __restrict__ d, double* __restrict__ e,
+                                                    double2* __restrict__ ring, unsigned* abort_flag) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    double2* Al = reinterpret_cast<double2*>(smem);   // Al[lc * n + i] = A[i][c], c = blockIdx.x + lc * nwg
+    double2* v = Al + (size_t)ncol * n;
+    double2* y = v + n;
+    double2* cn = y + n;
+    double* red = reinterpret_cast<double*>(cn + n + ncol);
+    __shared__ int stop;
+    const unsigned nwg = gridDim.x;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int lc = 0; lc < ncol; ++lc) {
+        const int c = blockIdx.x + lc * nwg;
+        if (c >= n) break;
+        for (int i = threadIdx.x; i < n; i += NT) Al[(size_t)lc * n + i] = A[(size_t)c * n + i];
+    }
+    double2 t;
+    double beta;
+    make_reflector(A + 1, n - 1, v, t, beta, red);
+    if (blockIdx.x == 0) {
+        for (int i = threadIdx.x; i < n - 1; i += NT) V[1 + i] = v[i];
+        if (threadIdx.x == 0) {
+            tau[0] = t;
+            d[0] = A[0].x;
+            e[0] = beta;
+        }
+    }
+    for (int k = 0; k < n - 1; ++k) {
+        const int m = n - k - 1;   // trailing rows/cols k+1 .. n-1
+        double2* slot = ring + (size_t)k * 2 * n;
+        // ---- p = A_trail^H v on own columns c >= k+1 (one wave per column) ----
+        const int lc0 = (k + 1 - (int)blockIdx.x + (int)nwg - 1) / (int)nwg;   // first own slot with c >= k+1
+        for (int lc = lc0 + wave; lc < ncol; lc += NT / 64) {
+            const int c = blockIdx.x + lc * nwg;
+            if (c >= n) break;
+            const double2* col = Al + (size_t)lc * n + (k + 1);
+            double pr = 0, pi = 0;
+            for (int i = lane; i < m; i += 64) {
+                const double2 q = cconjmul(col[i], v[i]);
+                pr += q.x;
+                pi += q.y;
+            }
+            pr = wave_sum(pr);
+            pi = wave_sum(pi);
+            if (lane == 0) ring_put(slot + (c - (k + 1)), make_double2(pr, pi));
+        }
+        if ((k + 1) % (int)nwg == (int)blockIdx.x) {   // owner of column k+1 publishes it
+            const double2* col = Al + (size_t)((k + 1) / nwg) * n + (k + 1);
+            for (int i = threadIdx.x; i < m; i += NT) ring_put(slot + n + i, col[i]);
+        }
+        // ---- gather p -> y and the published column -> cn ----
+        {
+            unsigned spins = 0;
+            while (true) {
+                bool mine = true;
+                for (int i = threadIdx.x; i < 2 * m; i += NT) {
+                    const bool isp = i < m;
+                    const int ii = isp ? i : i - m;
+                    double2 val;
+                    if (ring_get(slot + (isp ? ii : n + ii), val)) (isp ? y : cn)[ii] = val;
+                    else mine = false;
+                }
+                if (__syncthreads_and(mine)) break;
+                if (threadIdx.x == 0) {
+                    int st = __hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u;
+                    if (!st && ++spins > SPIN_LIMIT) {
+                        atomicExch(abort_flag, 1u);
+                        st = 1;
+                    }
+                    stop = st;
+                }
+                __syncthreads();
+                if (stop) break;
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+        // ---- y = tau p - (|tau|^2 mu / 2) v,  mu = v^H p;  cn -= rank-2 correction ----
+        double mr = 0;
+        for (int i = threadIdx.x; i < m; i += NT) mr += cconjmul(v[i], y[i]).x;
+        const double mu = block_sum2(mr, 0.0, red).x;
+        const double c2 = 0.5 * (t.x * t.x + t.y * t.y) * mu;
+        for (int i = threadIdx.x; i < m; i += NT) {
+            const double2 tp = cmul(t, y[i]);
+            y[i] = make_double2(tp.x - c2 * v[i].x, tp.y - c2 * v[i].y);
+        }
+        __syncthreads();
+        {
+            const double2 v0 = v[0], y0 = y[0];
+            for (int i = threadIdx.x; i < m; i += NT) {
+                const double2 a1 = cconjmul(v0, y[i]);
+                const double2 a2 = cconjmul(y0, v[i]);
+                cn[i].x -= a1.x + a2.x;
+                cn[i].y -= a1.y + a2.y;
+            }
+        }
+        // ---- rank-2 update of own columns c >= k+2, rows k+2 .. n-1 ----
+        {
+            const int lc1 = (k + 2 - (int)blockIdx.x + (int)nwg - 1) / (int)nwg;
+            for (int lc = lc1 + wave; lc < ncol; lc += NT / 64) {
+                const int c = blockIdx.x + lc * nwg;
+                if (c >= n) break;
+                double2* col = Al + (size_t)lc * n + (k + 1);
+                const int jj = c - (k + 1);
+                const double2 vj = v[jj], yj = y[jj];
+                for (int i = 1 + lane; i < m; i += 64) {
+                    const double2 a1 = cconjmul(vj, y[i]);
+                    const double2 a2 = cconjmul(yj, v[i]);
+                    double2 a = col[i];
+                    a.x -= a1.x + a2.x;
+                    a.y -= a1.y + a2.y;
+                    col[i] = a;
+                }
+            }
+        }
+        __syncthreads();
+        // ---- next reflector from the updated column (every workgroup, redundantly) ----
+        if (m >= 2) {
+            make_reflector(cn + 1, m - 1, v, t, beta, red);
+            if (blockIdx.x == 0) {
+                for (int i = threadIdx.x; i < m - 1; i += NT) V[(size_t)(k + 1) * n + (k + 2) + i] = v[i];
+                if (threadIdx.x == 0) {
+                    tau[k + 1] = t;
+                    d[k + 1] = cn[0].x;
+                    e[k + 1] = beta;
+                }
+            }
+        } else if (blockIdx.x == 0 && threadIdx.x == 0) {
+            d[k + 1] = cn[0].x;
+        }
+        __syncthreads();
+    }
+}
+
 // Sturm count: number of eigenvalues of T strictly below x
-__device__ int sturm(const double* d, const double* e, int n, double x, double pivmin) {
+// d and e2 (squared off-diagonals) in LDS; every lane of a wave reads the same address.
+__device__ int sturm(const double* d, const double* e2, int n, double x, double pivmin) {
     int c = 0;
     double q = d[0] - x;
     if (fabs(q) <= pivmin) q = -pivmin;
     if (q < 0) ++c;
     for (int i = 1; i < n; ++i) {
-        q = d[i] - x - e[i - 1] * e[i - 1] / q;
+        q = d[i] - x - e2[i - 1] / q;
         if (fabs(q) <= pivmin) q = -pivmin;
         if (q < 0) ++c;
     }
@@ -260,15 +433,23 @@ __device__ int sturm(const double* d, const double* e, int n, double x, double p
 // j-th eigenvalue (a 65x shrink per step, ~9 steps to full precision).
 __global__ __launch_bounds__(256) void k_bisect(const double* __restrict__ d, const double* __restrict__ e, int n,
                                                 int nband, double* __restrict__ w) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    double* ds = reinterpret_cast<double*>(smem);   // d, then e^2
+    double* e2 = ds + n;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        ds[i] = d[i];
+        if (i < n - 1) e2[i] = e[i] * e[i];
+    }
+    __syncthreads();
     const int lane = threadIdx.x & 63;
     const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (j >= nband) return;   // whole wave
+    if (j >= nband) return;   // whole wave; no barrier follows
     double lo = 1e300, hi = -1e300, emax2 = 0;
     for (int i = lane; i < n; i += 64) {
         const double r = (i > 0 ? fabs(e[i - 1]) : 0.0) + (i < n - 1 ? fabs(e[i]) : 0.0);
-        lo = fmin(lo, d[i] - r);
-        hi = fmax(hi, d[i] + r);
-        if (i < n - 1) emax2 = fmax(emax2, e[i] * e[i]);
+        lo = fmin(lo, ds[i] - r);
+        hi = fmax(hi, ds[i] + r);
+        if (i < n - 1) emax2 = fmax(emax2, e2[i]);
     }
     for (int o = 32; o > 0; o >>= 1) {
         lo = fmin(lo, __shfl_xor(lo, o, 64));
@@ -283,7 +464,7 @@ __global__ __launch_bounds__(256) void k_bisect(const double* __restrict__ d, co
     for (int it = 0; it < 64; ++it) {
         if (hi - lo <= 2.0 * DBL_EPSILON * fmax(fabs(lo), fabs(hi)) + pivmin) break;
         const double x = lo + (hi - lo) * (double)(lane + 1) / 65.0;
-        const int c = sturm(d, e, n, x, pivmin);
+        const int c = sturm(ds, e2, n, x, pivmin);
         const unsigned long long above = __ballot(c > j);
         const double x63 = __shfl(x, 63, 64);
         if (above == 0ull) {
@@ -309,30 +490,44 @@ __device__ inline unsigned hash_u32(unsigned x) {
 }
 
 // Inverse iteration, one workgroup per cluster of close eigenvalues (the workgroup of the
-// cluster's first index does the whole cluster; the others exit).
-// LDS: the iterate xv, the LU factors a, b, c, dd (n doubles each, touched by thread 0) and
-// the pivots (n ints).
+// cluster's first index does the whole cluster; the others exit).  The LU factorisation
+// and the two triangular solves are sequential (thread 0); their recurrences carry the
+// running values in registers and read the next operands from LDS one step ahead, so the
+// chain waits on arithmetic, not on LDS round trips.
+// LDS: d, e, the iterate xv, the LU factors A (diagonal), IA = 1/A, B, C, DD (n doubles
+// each), reduction scratch, the pivots (n ints).
 __global__ __launch_bounds__(NT) void k_invit(const double* __restrict__ d, const double* __restrict__ e, int n,
                                               int nband, const double* __restrict__ w, double* __restrict__ Z,
                                               double ctol_rel) {
     extern __shared__ __align__(16) unsigned char smem[];
-    double* xv = reinterpret_cast<double*>(smem);
-    double* red = xv + n;
-    double* a = red + 64;
-    double* b = a + n;
-    double* c = b + n;
-    double* dd = c + n;
-    int* piv = reinterpret_cast<int*>(dd + n);
+    double* ds = reinterpret_cast<double*>(smem);
+    double* es = ds + n;
+    double* xv = es + n;
+    double* A = xv + n;
+    double* IA = A + n;
+    double* B = IA + n;
+    double* C = B + n;
+    double* DD = C + n;
+    double* red = DD + n;
+    int* piv = reinterpret_cast<int*>(red + 64);
     const int j0 = blockIdx.x;
-    // ||T||_1 bound (Gershgorin)
+    // ||T||_1 bound (Gershgorin), block-parallel
     double tn = 0;
-    for (int i = 0; i < n; ++i) {
-        const double r = fabs(d[i]) + (i > 0 ? fabs(e[i - 1]) : 0.0) + (i < n - 1 ? fabs(e[i]) : 0.0);
-        tn = fmax(tn, r);
+    for (int i = threadIdx.x; i < n; i += NT) {
+        const double di = d[i];
+        const double ei = i < n - 1 ? e[i] : 0.0;
+        ds[i] = di;
+        es[i] = ei;
+        tn = fmax(tn, fabs(di) + (i > 0 ? fabs(e[i - 1]) : 0.0) + fabs(ei));
     }
-    tn = fmax(tn, DBL_MIN);
+    for (int o = 32; o > 0; o >>= 1) tn = fmax(tn, __shfl_xor(tn, o, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = tn;
+    __syncthreads();
+    tn = DBL_MIN;
+    for (int q = 0; q < NT / 64; ++q) tn = fmax(tn, red[q]);
+    __syncthreads();
     const double ctol = ctol_rel * tn;
-    if (j0 > 0 && w[j0] - w[j0 - 1] <= ctol) return;   // not a cluster head
+    if (j0 > 0 && w[j0] - w[j0 - 1] <= ctol) return;   // not a cluster head (uniform)
     int j1 = j0;
     while (j1 + 1 < nband && w[j1 + 1] - w[j1] <= ctol) ++j1;
     const double eps = DBL_EPSILON;
@@ -343,38 +538,47 @@ __global__ __launch_bounds__(NT) void k_invit(const double* __restrict__ d, cons
         double lam = w[j];
         if (j > j0 && lam - lam_prev < pertol) lam = lam_prev + pertol;
         lam_prev = lam;
-        // LU of T - lam I with partial pivoting (thread 0, sequential)
+        // LU of T - lam I with partial pivoting (thread 0, sequential); ak, bk carry the
+        // running diagonal and superdiagonal of the active row
         if (threadIdx.x == 0) {
-            for (int i = 0; i < n; ++i) a[i] = d[i] - lam;
-            for (int i = 0; i < n - 1; ++i) {
-                b[i] = e[i];
-                c[i] = e[i];
-            }
+            double ak = ds[0] - lam, bk = n > 1 ? es[0] : 0.0;
+            double an = n > 1 ? ds[1] - lam : 0.0, ck = es[0], bn = n > 2 ? es[1] : 0.0;
             for (int k = 0; k < n - 1; ++k) {
-                if (fabs(c[k]) <= fabs(a[k])) {
+                // operands of step k+1, independent of this step
+                const double an2 = k + 2 < n ? ds[k + 2] - lam : 0.0;
+                const double ck2 = k + 1 < n - 1 ? es[k + 1] : 0.0;
+                const double bn2 = k + 3 < n ? es[k + 2] : 0.0;
+                if (fabs(ck) <= fabs(ak)) {
                     piv[k] = 0;
-                    double ak = a[k];
                     if (fabs(ak) < tiny) ak = (ak >= 0 ? tiny : -tiny);
-                    a[k] = ak;
-                    const double mult = c[k] / ak;
-                    c[k] = mult;
-                    a[k + 1] -= mult * b[k];
-                    if (k < n - 2) dd[k] = 0.0;
+                    const double ia = 1.0 / ak;
+                    const double mult = ck * ia;
+                    A[k] = ak;
+                    IA[k] = ia;
+                    B[k] = bk;
+                    C[k] = mult;
+                    DD[k] = 0.0;
+                    ak = an - mult * bk;
+                    bk = bn;
                 } else {
                     piv[k] = 1;
-                    const double mult = a[k] / c[k];
-                    a[k] = c[k];
-                    const double temp = a[k + 1];
-                    a[k + 1] = b[k] - mult * temp;
-                    if (k < n - 2) {
-                        dd[k] = b[k + 1];
-                        b[k + 1] = -mult * dd[k];
-                    }
-                    b[k] = temp;
-                    c[k] = mult;
+                    const double ia = 1.0 / ck;
+                    const double mult = ak * ia;
+                    A[k] = ck;
+                    IA[k] = ia;
+                    B[k] = an;
+                    C[k] = mult;
+                    DD[k] = bn;
+                    ak = bk - mult * an;
+                    bk = -mult * bn;
                 }
+                an = an2;
+                ck = ck2;
+                bn = bn2;
             }
-            if (fabs(a[n - 1]) < tiny) a[n - 1] = (a[n - 1] >= 0 ? tiny : -tiny);
+            if (fabs(ak) < tiny) ak = (ak >= 0 ? tiny : -tiny);
+            A[n - 1] = ak;
+            IA[n - 1] = 1.0 / ak;
         }
         // start vector
         for (int i = threadIdx.x; i < n; i += NT) {
@@ -384,20 +588,31 @@ __global__ __launch_bounds__(NT) void k_invit(const double* __restrict__ d, cons
         __syncthreads();
         for (int it = 0; it < 3; ++it) {
             if (threadIdx.x == 0) {
-                // forward: L
+                // forward: L (row interchanges per piv), carry the running entry
+                double xk = xv[0];
+                double nx = n > 1 ? xv[1] : 0.0;
                 for (int k = 0; k < n - 1; ++k) {
+                    const double nx2 = k + 2 < n ? xv[k + 2] : 0.0;
+                    const double ck = C[k];
                     if (piv[k] == 0) {
-                        xv[k + 1] -= c[k] * xv[k];
+                        xv[k] = xk;
+                        xk = nx - ck * xk;
                     } else {
-                        const double t = xv[k];
-                        xv[k] = xv[k + 1];
-                        xv[k + 1] = t - c[k] * xv[k];
+                        xv[k] = nx;
+                        xk = xk - ck * nx;
                     }
+                    nx = nx2;
                 }
-                // backward: U (diag a, super b, second super dd)
-                xv[n - 1] /= a[n - 1];
-                if (n > 1) xv[n - 2] = (xv[n - 2] - b[n - 2] * xv[n - 1]) / a[n - 2];
-                for (int k = n - 3; k >= 0; --k) xv[k] = (xv[k] - b[k] * xv[k + 1] - dd[k] * xv[k + 2]) / a[k];
+                xv[n - 1] = xk;
+                // backward: U (diagonal A, superdiagonals B, DD), carry x[k+1], x[k+2]
+                double x1 = xk * IA[n - 1], x2 = 0.0;
+                xv[n - 1] = x1;
+                for (int k = n - 2; k >= 0; --k) {
+                    const double xk0 = (xv[k] - B[k] * x1 - (k < n - 2 ? DD[k] * x2 : 0.0)) * IA[k];
+                    xv[k] = xk0;
+                    x2 = x1;
+                    x1 = xk0;
+                }
             }
             __syncthreads();
             // scale to unit max-norm first (avoid overflow), then Gram-Schmidt against the
@@ -503,69 +718,202 @@ __global__ __launch_bounds__(NT) void k_backtransform(int n, const double2* __re
     for (int i = threadIdx.x; i < n; i += NT) vcc[(size_t)j * ldv + i] = z[i];
 }
 
-// ---- Cholesky G = R^H R and R^-1 (single workgroup) ----
-// W (n x n, ld n) work copy of G; on exit rinv holds R^-1 (upper, zeros below).
-__global__ __launch_bounds__(1024) void k_chol_inv(int n, const double2* __restrict__ g, int ldg,
-                                                   double2* __restrict__ W, double2* __restrict__ rinv,
-                                                   int* __restrict__ status, double min_norm) {
-    for (size_t t = threadIdx.x; t < (size_t)n * n; t += blockDim.x) {
-        const int r = (int)(t % n), c = (int)(t / n);
-        W[t] = g[r + (size_t)c * ldg];
+// vcc[:, j] = Q z_j with one wave per column, no workgroup barriers: lane l holds rows
+// l + 64 r (r < RPL) of z in registers; reflector k is read straight from V into
+// registers one reflector ahead of its use, so its load overlaps the previous update.
+// Rows below k + 1 are skipped a register slot at a time (uniform across the wave).
+template <int RPL>
+__global__ __launch_bounds__(NT) void k_backtransform_w(int n, int nband, const double2* __restrict__ V,
+                                                        const double2* __restrict__ tau,
+                                                        const double* __restrict__ Z, double2* __restrict__ vcc,
+                                                        int ldv) {
+    const int lane = threadIdx.x & 63;
+    const int j = blockIdx.x * (NT / 64) + (threadIdx.x >> 6);
+    if (j >= nband) return;   // whole wave; the kernel has no workgroup barrier
+    double2 z[RPL], vc[RPL], vn[RPL];
+#pragma unroll
+    for (int r = 0; r < RPL; ++r) {
+        const int i = lane + 64 * r;
+        z[r] = make_double2(i < n ? Z[(size_t)j * n + i] : 0.0, 0.0);
     }
-    __shared__ int bad;
-    if (threadIdx.x == 0) bad = 0;
-    __syncthreads();
+    // reflector k occupies rows k+1 .. n-1 of column k of V
+    auto load = [&](int k, double2* dst) {
+#pragma unroll
+        for (int r = 0; r < RPL; ++r) {
+            const int i = lane + 64 * r;
+            dst[r] = V[(size_t)k * n + (i < n ? i : 0)];
+        }
+    };
+    int k = n - 2;
+    load(k, vc);
+    double2 tc = tau[k];
+    for (; k >= 0; --k) {
+        double2 tn = make_double2(0.0, 0.0);
+        if (k > 0) {
+            load(k - 1, vn);
+            tn = tau[k - 1];
+        }
+        const int rmin = (k + 1) >> 6;
+        double sr = 0, si = 0;
+#pragma unroll
+        for (int r = 0; r < RPL; ++r) {
+            if (r < rmin) continue;
+            const int i = lane + 64 * r;
+            if (i > k && i < n) {
+                const double2 q = cconjmul(vc[r], z[r]);
+                sr += q.x;
+                si += q.y;
+            }
+        }
+        sr = wave_sum(sr);
+        si = wave_sum(si);
+        const double2 f = cmul(tc, make_double2(sr, si));
+#pragma unroll
+        for (int r = 0; r < RPL; ++r) {
+            if (r < rmin) continue;
+            const int i = lane + 64 * r;
+            if (i > k && i < n) {
+                const double2 q = cmul(vc[r], f);
+                z[r].x -= q.x;
+                z[r].y -= q.y;
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < RPL; ++r) vc[r] = vn[r];
+        tc = tn;
+    }
+#pragma unroll
+    for (int r = 0; r < RPL; ++r) {
+        const int i = lane + 64 * r;
+        if (i < n) vcc[(size_t)j * ldv + i] = z[r];
+    }
+}
+
+// ---- Cholesky G = R^H R and R^-1, register resident (single workgroup) ----
+// 1024 threads as a 32 x 32 grid; thread (tr, tc) holds entries (tr + 32a, tc + 32b) of the
+// upper triangle, a, b < P, so n <= 32 P.  Cholesky is right-looking with row j of the
+// trailing matrix broadcast through LDS; R^-1 is formed in place by Gauss-Jordan
+// elimination from the bottom row up (the eliminated slots of R hold R^-1).  One barrier
+// per step, the broadcast buffers alternate between steps.
+template <int P>
+__global__ __launch_bounds__(1024) void k_chol_inv(int n, const double2* __restrict__ g, int ldg,
+                                                   double2* __restrict__ rinv, int ldr, int* __restrict__ status,
+                                                   double min_norm) {
+    __shared__ double2 rowbuf[2][32 * P];
+    __shared__ double2 colbuf[2][32 * P];
+    __shared__ double diagr[32 * P];
+    const int tr = threadIdx.x & 31, tc = threadIdx.x >> 5;
+    double2 M[P][P];
+#pragma unroll
+    for (int a = 0; a < P; ++a)
+#pragma unroll
+        for (int b = 0; b < P; ++b) {
+            if (a > b) continue;
+            const int i = tr + 32 * a, c = tc + 32 * b;
+            M[a][b] = (i <= c && c < n) ? g[i + (size_t)c * ldg] : make_double2(0.0, 0.0);
+        }
+    int bad = 0;
     for (int j = 0; j < n; ++j) {
-        const double piv2 = W[j + (size_t)j * n].x;
+        const int buf = j & 1;
+        if (tr == (j & 31)) {
+#pragma unroll
+            for (int a = 0; a < P; ++a)
+                if (a == (j >> 5)) {
+#pragma unroll
+                    for (int b = 0; b < P; ++b) {
+                        const int c = tc + 32 * b;
+                        if (a <= b && c >= j && c < n) rowbuf[buf][c] = M[a][b];
+                    }
+                }
+        }
+        __syncthreads();
+        const double piv2 = rowbuf[buf][j].x;
         const double piv = piv2 > 0 ? sqrt(piv2) : 0.0;
-        if (!(piv2 > 0) || piv < min_norm) {
-            if (threadIdx.x == 0) bad = j + 1;
-            __syncthreads();
+        if (!(piv2 > 0) || piv < min_norm) {   // uniform: every thread read the same pivot
+            bad = j + 1;
             break;
         }
-        const double ip = 1.0 / piv;
-        // row j of R: R[j][c] = W[j][c] / piv for c > j
-        for (int c = j + 1 + threadIdx.x; c < n; c += blockDim.x) {
-            double2 x = W[j + (size_t)c * n];
-            W[j + (size_t)c * n] = make_double2(x.x * ip, x.y * ip);
+        const double ip = 1.0 / piv, ip2 = 1.0 / piv2;
+        if (threadIdx.x == 0) diagr[j] = piv;
+#pragma unroll
+        for (int a = 0; a < P; ++a) {
+            const int i = tr + 32 * a;
+            if (i < j || i >= n) continue;
+            const double2 ri = rowbuf[buf][i];
+#pragma unroll
+            for (int b = 0; b < P; ++b) {
+                const int c = tc + 32 * b;
+                if (a > b || c < i || c >= n) continue;
+                const double2 rc = rowbuf[buf][c];
+                if (i == j) {
+                    M[a][b] = (c == j) ? make_double2(piv, 0.0) : make_double2(rc.x * ip, rc.y * ip);
+                } else {
+                    const double2 q = cconjmul(ri, rc);
+                    M[a][b].x -= q.x * ip2;
+                    M[a][b].y -= q.y * ip2;
+                }
+            }
         }
-        if (threadIdx.x == 0) W[j + (size_t)j * n] = make_double2(piv, 0.0);
-        __syncthreads();
-        // trailing update of the upper triangle: W[r][c] -= conj(R[j][r]) R[j][c], j < r <= c
-        const int m = n - j - 1;
-        for (size_t t = threadIdx.x; t < (size_t)m * m; t += blockDim.x) {
-            const int r = j + 1 + (int)(t % m), c = j + 1 + (int)(t / m);
-            if (r > c) continue;
-            const double2 a = W[j + (size_t)r * n], b = W[j + (size_t)c * n];
-            const double2 q = cconjmul(a, b);
-            double2 x = W[r + (size_t)c * n];
-            x.x -= q.x;
-            x.y -= q.y;
-            W[r + (size_t)c * n] = x;
-        }
-        __syncthreads();
     }
     if (bad) {
         if (threadIdx.x == 0) status[0] = bad;
         return;
     }
-    // R^-1 column by column (thread per column): X[c][c] = 1/R[c][c];
-    // X[r][c] = -(sum_{k=r+1..c} R[r][k] X[k][c]) / R[r][r]
-    for (int c = threadIdx.x; c < n; c += blockDim.x) {
-        double2* X = rinv + (size_t)c * n;
-        for (int r = c + 1; r < n; ++r) X[r] = make_double2(0.0, 0.0);
-        X[c] = make_double2(1.0 / W[c + (size_t)c * n].x, 0.0);
-        for (int r = c - 1; r >= 0; --r) {
-            double sr = 0, si = 0;
-            for (int k = r + 1; k <= c; ++k) {
-                const double2 q = cmul(W[r + (size_t)k * n], X[k]);
-                sr += q.x;
-                si += q.y;
+    __syncthreads();
+    for (int r = n - 1; r >= 0; --r) {
+        const int buf = r & 1;
+        const double id = 1.0 / diagr[r];
+        if (tr == (r & 31)) {   // row r of R^-1: (delta + accumulated) / R[r][r]
+#pragma unroll
+            for (int a = 0; a < P; ++a)
+                if (a == (r >> 5)) {
+#pragma unroll
+                    for (int b = 0; b < P; ++b) {
+                        const int c = tc + 32 * b;
+                        if (a > b || c < r || c >= n) continue;
+                        const double2 x = (c == r) ? make_double2(id, 0.0) : make_double2(M[a][b].x * id, M[a][b].y * id);
+                        M[a][b] = x;
+                        rowbuf[buf][c] = x;
+                    }
+                }
+        }
+        if (tc == (r & 31)) {   // multipliers R[k][r], k < r
+#pragma unroll
+            for (int b = 0; b < P; ++b)
+                if (b == (r >> 5)) {
+#pragma unroll
+                    for (int a = 0; a < P; ++a) {
+                        const int k = tr + 32 * a;
+                        if (a <= b && k < r) colbuf[buf][k] = M[a][b];
+                    }
+                }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int a = 0; a < P; ++a) {
+            const int k = tr + 32 * a;
+            if (k >= r) continue;
+            const double2 f = colbuf[buf][k];
+#pragma unroll
+            for (int b = 0; b < P; ++b) {
+                const int c = tc + 32 * b;
+                if (a > b || c < r || c >= n) continue;
+                const double2 q = cmul(f, rowbuf[buf][c]);
+                if (c == r) M[a][b] = make_double2(-q.x, -q.y);
+                else {
+                    M[a][b].x -= q.x;
+                    M[a][b].y -= q.y;
+                }
             }
-            const double id = 1.0 / W[r + (size_t)r * n].x;
-            X[r] = make_double2(-sr * id, -si * id);
         }
     }
+#pragma unroll
+    for (int a = 0; a < P; ++a)
+#pragma unroll
+        for (int b = 0; b < P; ++b) {
+            const int i = tr + 32 * a, c = tc + 32 * b;
+            if (i < n && c < n) rinv[i + (size_t)c * ldr] = (a <= b && i <= c) ? M[a][b] : make_double2(0.0, 0.0);
+        }
 }
 
 }  // namespace
@@ -573,65 +921,128 @@ __global__ __launch_bounds__(1024) void k_chol_inv(int n, const double2* __restr
 void heevx_lowest(hipStream_t s, EigWorkspace& ws, int num_cu, int n, int ldh, const double* hcc, int nband,
                   double* eig, double* vcc, int ldv, Prof* prof) {
     APW_CHECK(n >= 1 && nband >= 1 && nband <= n && ldh >= n && ldv >= n, "heevx: bad sizes n=%d nband=%d", n, nband);
-    APW_CHECK(n <= 1300, "heevx: n=%d too large (LDS of the reduction kernel)", n);
+    APW_CHECK(n <= HEEVX_MAX, "heevx: n=%d above %d (LDS of the inverse-iteration kernel)", n, HEEVX_MAX);
     ws.a.reserve((size_t)2 * n * n);
     ws.v.reserve((size_t)2 * n * n);
     ws.tau.reserve((size_t)2 * n);
     ws.d.reserve(n);
     ws.e.reserve(n);
-    ws.p.reserve((size_t)4 * n);
+    ws.p.reserve((size_t)8 * n);   // 4n c128: p and the published column, each double-buffered
     ws.z.reserve((size_t)n * nband);
-    ws.bar.reserve(2);
+    ws.bar.reserve(64 + 256);
     double2* A = reinterpret_cast<double2*>(ws.a.get());
     hipLaunchKernelGGL(k_copy_herm, dim3(std::min(4096, (n * n + 255) / 256)), dim3(256), 0, s, n,
                        reinterpret_cast<const double2*>(hcc), ldh, A);
     APW_LAUNCH_CHECK();
     if (n > 1) {
-        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, ws.bar.get(), 2);
+        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(256), 0, s, ws.bar.get(), 64 + 256);
         APW_LAUNCH_CHECK();
-        const size_t lds = (size_t)3 * n * sizeof(double2) + 64 * sizeof(double);
-        int occ = 0;
-        APW_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(k_tridiag), NT, lds));
-        int nwg = std::max(1, std::min(n / 16, std::min(num_cu, 64)));
-        nwg = std::min(nwg, std::max(1, occ) * num_cu);
+        // exchange buffers: 4n c128 (p and the published column, double-buffered); flags
+        // for up to 256 workgroups after the counter / abort words
+        APW_CHECK(ws.p.n >= (size_t)8 * n && ws.bar.n >= 64 + 256 && ws.v.n >= (size_t)2 * n * n,
+                  "heevx: workspace too small for n=%d", n);
         double2* Vp = reinterpret_cast<double2*>(ws.v.get());
         double2* taup = reinterpret_cast<double2*>(ws.tau.get());
         double* dp = ws.d.get();
         double* ep = ws.e.get();
         double2* Pp = reinterpret_cast<double2*>(ws.p.get());
         unsigned* barp = ws.bar.get();
-        void* args[] = {&A, &n, &Vp, &taup, &dp, &ep, &Pp, &barp};
         // algorithmic: per column a hemv read and a rank-2 read+write of the trailing matrix
         ProfScope ps(prof, s, "eig_tridiag", 16.0 * 3.0 * n * (double)n * n / 3.0, 16.0 / 3.0 * n * (double)n * n);
-        APW_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_tridiag), dim3(nwg), dim3(NT), args,
-                                           (unsigned)lds, s));
+        // LDS-resident form: the fewest workgroups whose column share fits the LDS budget
+        const size_t fixed = (size_t)3 * n * sizeof(double2) + 64 * sizeof(double);
+        int nwg = -1, ncol = 0;
+        for (int w = std::max(1, std::min(n / 8, 16)); w <= std::min(num_cu, 256); ++w) {
+            const int nc = (n + w - 1) / w;
+            if (fixed + (size_t)nc * (n + 1) * sizeof(double2) <= TRIDIAG_LDS_BYTES) {
+                nwg = w;
+                ncol = nc;
+                break;
+            }
+        }
+        if (nwg > 0) {
+            const size_t lds = fixed + (size_t)ncol * (n + 1) * sizeof(double2);
+            const size_t nring = (size_t)2 * n * (n - 1);
+            ws.ring.reserve(2 * nring);
+            double2* ring = reinterpret_cast<double2*>(ws.ring.get());
+            hipLaunchKernelGGL(k_fill_ring, dim3((unsigned)std::min<size_t>(1024, (nring + 255) / 256)), dim3(256), 0, s,
+                               ring, nring);
+            APW_LAUNCH_CHECK();
+            APW_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_tridiag_lds),
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+            const double2* Ac = A;
+            unsigned* abortp = barp + 1;
+            void* args[] = {&Ac, &n, &ncol, &Vp, &taup, &dp, &ep, &ring, &abortp};
+            APW_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_tridiag_lds), dim3(nwg), dim3(NT), args,
+                                               (unsigned)lds, s));
+        } else {
+            const size_t lds = fixed;
+            APW_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_tridiag), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        (int)lds));
+            int occ = 0;
+            APW_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(k_tridiag), NT, lds));
+            nwg = std::max(1, std::min(n / 16, std::min(num_cu, 64)));
+            nwg = std::min(nwg, std::max(1, occ) * num_cu);
+            void* args[] = {&A, &n, &Vp, &taup, &dp, &ep, &Pp, &barp};
+            APW_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_tridiag), dim3(nwg), dim3(NT), args,
+                                               (unsigned)lds, s));
+        }
     } else {
         // n == 1: T = [Re a00]; the real part is the first double of A
         APW_HIP(hipMemcpyAsync(ws.d.get(), A, sizeof(double), hipMemcpyDeviceToDevice, s));
     }
     {
     ProfScope ps(prof, s, "eig_bisect_invit", 0.0, 0.0);
-    hipLaunchKernelGGL(k_bisect, dim3((nband + 3) / 4), dim3(256), 0, s, ws.d.get(), ws.e.get(), n, nband, eig);
+    hipLaunchKernelGGL(k_bisect, dim3((nband + 3) / 4), dim3(256), (size_t)2 * n * sizeof(double), s, ws.d.get(),
+                       ws.e.get(), n, nband, eig);
     APW_LAUNCH_CHECK();
-    const size_t lds_inv = (size_t)5 * n * sizeof(double) + 64 * sizeof(double) + (size_t)n * sizeof(int);
+    const size_t lds_inv = (size_t)8 * n * sizeof(double) + 64 * sizeof(double) + (size_t)n * sizeof(int);
+    APW_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_invit), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)lds_inv));
     hipLaunchKernelGGL(k_invit, dim3(nband), dim3(NT), lds_inv, s, ws.d.get(), ws.e.get(), n, nband, eig, ws.z.get(),
                        1e-7);
     APW_LAUNCH_CHECK();
     }
-    const size_t lds_bt = (size_t)n * sizeof(double2) + 64 * sizeof(double);
     ProfScope pb(prof, s, "eig_backtransform", 16.0 * n * (double)n / 2.0 + 16.0 * n * nband,
                  8.0 * n * (double)n * nband);
-    hipLaunchKernelGGL(k_backtransform, dim3(nband), dim3(NT), lds_bt, s, n,
-                       reinterpret_cast<const double2*>(ws.v.get()), reinterpret_cast<const double2*>(ws.tau.get()),
-                       ws.z.get(), reinterpret_cast<double2*>(vcc), ldv);
+    const double2* Vc = reinterpret_cast<const double2*>(ws.v.get());
+    const double2* tc = reinterpret_cast<const double2*>(ws.tau.get());
+    double2* out = reinterpret_cast<double2*>(vcc);
+    const dim3 gw((nband + NT / 64 - 1) / (NT / 64));
+    if (n < 2) {
+        hipLaunchKernelGGL(k_backtransform_w<1>, gw, dim3(NT), 0, s, n, nband, Vc, tc, ws.z.get(), out, ldv);
+    } else if (n <= 128) {
+        hipLaunchKernelGGL(k_backtransform_w<2>, gw, dim3(NT), 0, s, n, nband, Vc, tc, ws.z.get(), out, ldv);
+    } else if (n <= 256) {
+        hipLaunchKernelGGL(k_backtransform_w<4>, gw, dim3(NT), 0, s, n, nband, Vc, tc, ws.z.get(), out, ldv);
+    } else if (n <= 512) {
+        hipLaunchKernelGGL(k_backtransform_w<8>, gw, dim3(NT), 0, s, n, nband, Vc, tc, ws.z.get(), out, ldv);
+    } else if (n <= 768) {
+        hipLaunchKernelGGL(k_backtransform_w<12>, gw, dim3(NT), 0, s, n, nband, Vc, tc, ws.z.get(), out, ldv);
+    } else if (n <= 1024) {
+        hipLaunchKernelGGL(k_backtransform_w<16>, gw, dim3(NT), 0, s, n, nband, Vc, tc, ws.z.get(), out, ldv);
+    } else {
+        const size_t lds_bt = (size_t)n * sizeof(double2) + 64 * sizeof(double);
+        APW_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_backtransform),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bt));
+        hipLaunchKernelGGL(k_backtransform, dim3(nband), dim3(NT), lds_bt, s, n, Vc, tc, ws.z.get(), out, ldv);
+    }
     APW_LAUNCH_CHECK();
 }
 
-void chol_inv_upper(hipStream_t s, int n, const double* g, int ldg, double* rinv, int* status, double min_norm) {
-    // work copy lives in rinv's tail: caller passes rinv with room for 2 * n * n c128
-    double2* W = reinterpret_cast<double2*>(rinv) + (size_t)n * n;
-    hipLaunchKernelGGL(k_chol_inv, dim3(1), dim3(1024), 0, s, n, reinterpret_cast<const double2*>(g), ldg, W,
-                       reinterpret_cast<double2*>(rinv), status, min_norm);
+void chol_inv_upper(hipStream_t s, int n, const double* g, int ldg, double* rinv, int ldr, int* status,
+                    double min_norm, Prof* prof) {
+    APW_CHECK(n >= 1 && n <= CHOL_MAX, "chol_inv_upper: n=%d outside 1..%d", n, CHOL_MAX);
+    // algorithmic: n^3/3 complex multiply-adds for the factor, n^3/3 for the inverse
+    ProfScope ps(prof, s, "chol_inv", 32.0 * n * (double)n, 16.0 / 3.0 * n * (double)n * n);
+    const double2* gp = reinterpret_cast<const double2*>(g);
+    double2* rp = reinterpret_cast<double2*>(rinv);
+    switch ((n + 31) / 32) {
+        case 1: hipLaunchKernelGGL(k_chol_inv<1>, dim3(1), dim3(1024), 0, s, n, gp, ldg, rp, ldr, status, min_norm); break;
+        case 2: hipLaunchKernelGGL(k_chol_inv<2>, dim3(1), dim3(1024), 0, s, n, gp, ldg, rp, ldr, status, min_norm); break;
+        case 3: hipLaunchKernelGGL(k_chol_inv<3>, dim3(1), dim3(1024), 0, s, n, gp, ldg, rp, ldr, status, min_norm); break;
+        default: hipLaunchKernelGGL(k_chol_inv<4>, dim3(1), dim3(1024), 0, s, n, gp, ldg, rp, ldr, status, min_norm); break;
+    }
     APW_LAUNCH_CHECK();
 }
 

@@ -82,20 +82,24 @@ __global__ __launch_bounds__(NT) void zgemm_kernel(int M, int N, int K, int kchu
             acc_im[a][b] = acc_t{0, 0, 0, 0};
         }
 
+    // Prefetch registers.  The loads are unconditional (an out-of-range element reads
+    // element 0 instead, which is valid for k > 0) so the compiler never has to wait on
+    // one inside a branch; range masking and the conjugation happen at the LDS store.
     c_t ra[NA], rb[NB];
+    unsigned oka = 0, okb = 0;
     auto fetch = [&](int k0) {
+        oka = 0;
+        okb = 0;
 #pragma unroll
         for (int i = 0; i < NA; ++i) {
             const int e = tid + NT * i;
             int mm, kk;
             if (TA == 0) { mm = e % BM; kk = e / BM; } else { kk = e % BK; mm = e / BK; }
             const int gm = m0 + mm, gk = k0 + kk;
-            c_t v = c_t{0, 0};
-            if (gm < M && gk < kend) {
-                v = (TA == 0) ? A[(size_t)gm + (size_t)gk * lda] : A[(size_t)gk + (size_t)gm * lda];
-                if (TA == 2) v.y = -v.y;
-            }
-            ra[i] = v;
+            const bool ok = gm < M && gk < kend;
+            const size_t off = (TA == 0) ? (size_t)gm + (size_t)gk * lda : (size_t)gk + (size_t)gm * lda;
+            ra[i] = A[ok ? off : 0];
+            oka |= (unsigned)ok << i;
         }
 #pragma unroll
         for (int i = 0; i < NB; ++i) {
@@ -103,12 +107,10 @@ __global__ __launch_bounds__(NT) void zgemm_kernel(int M, int N, int K, int kchu
             int nn, kk;
             if (TB == 0) { kk = e % BK; nn = e / BK; } else { nn = e % BN; kk = e / BN; }
             const int gn = n0 + nn, gk = k0 + kk;
-            c_t w = c_t{0, 0};
-            if (gn < N && gk < kend) {
-                w = (TB == 0) ? B[(size_t)gk + (size_t)gn * ldb] : B[(size_t)gn + (size_t)gk * ldb];
-                if (TB == 2) w.y = -w.y;
-            }
-            rb[i] = w;
+            const bool ok = gn < N && gk < kend;
+            const size_t off = (TB == 0) ? (size_t)gk + (size_t)gn * ldb : (size_t)gn + (size_t)gk * ldb;
+            rb[i] = B[ok ? off : 0];
+            okb |= (unsigned)ok << i;
         }
     };
 
@@ -120,16 +122,18 @@ __global__ __launch_bounds__(NT) void zgemm_kernel(int M, int N, int K, int kchu
             const int e = tid + NT * i;
             int mm, kk;
             if (TA == 0) { mm = e % BM; kk = e / BM; } else { kk = e % BK; mm = e / BK; }
-            As_re[kk][mm] = ra[i].x;
-            As_im[kk][mm] = ra[i].y;
+            const bool ok = (oka >> i) & 1u;
+            As_re[kk][mm] = ok ? ra[i].x : R(0);
+            As_im[kk][mm] = ok ? (TA == 2 ? -ra[i].y : ra[i].y) : R(0);
         }
 #pragma unroll
         for (int i = 0; i < NB; ++i) {
             const int e = tid + NT * i;
             int nn, kk;
             if (TB == 0) { kk = e % BK; nn = e / BK; } else { nn = e % BN; kk = e / BN; }
-            Bs_re[kk][nn] = rb[i].x;
-            Bs_im[kk][nn] = rb[i].y;
+            const bool ok = (okb >> i) & 1u;
+            Bs_re[kk][nn] = ok ? rb[i].x : R(0);
+            Bs_im[kk][nn] = ok ? (TB == 2 ? -rb[i].y : rb[i].y) : R(0);
         }
         __syncthreads();
         if (k0 + BK < kend) fetch(k0 + BK);

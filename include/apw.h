@@ -182,6 +182,11 @@ int apw_fft3d(apw_ctx* ctx, int nx, int ny, int nz, double* box, int dir);
 int apw_zgemm(apw_ctx* ctx, char transa, char transb, int m, int n, int k,
               const double* alpha, const double* a, int lda, const double* b, int ldb,
               const double* beta, double* c, int ldc);
+/* gemm_op<complex<float>, DEVICE_GPU> (math_kernel_op.h:86-120): as apw_zgemm on c64 data,
+ * FP32 MFMA with FP32 accumulation.  alpha/beta HOST c64 (2 floats each). */
+int apw_cgemm(apw_ctx* ctx, char transa, char transb, int m, int n, int k,
+              const float* alpha, const float* a, int lda, const float* b, int ldb,
+              const float* beta, float* c, int ldc);
 /* dnevx_op<complex<double>, DEVICE_GPU> (dngvd_op.h:85-107): lowest nbands eigenpairs
  * of the Hermitian nstart x nstart matrix hcc (lower triangle, ld ldh).  eigenvalue:
  * device double[nbands]; vcc: device c128, ld ldh, eigenvector j in column j. */

@@ -126,3 +126,44 @@ def test_david_oracle_si8_matches_dense():
     w = np.linalg.eigvalsh(O.HamiltOracle(s).dense())
     eig, psi, it, dav = O.david_solve(s, thr=1e-9)
     assert np.abs(eig - w[: s.nband]).max() < 1e-9
+
+
+@pytest.mark.parametrize("n", [2, 7, 48])
+def test_householder_matches_lapack_zhetrd(n):
+    """The oracle's reduction reproduces LAPACK zhetrd('L') exactly in convention: the same
+    reflectors v_k (v_k[k+1] = 1, rest in the strict lower part), the same tau_k, and
+    Q^H A Q = tridiag(d, e) with zhetrd's real e (beta = -sign(Re alpha) ||x||)."""
+    import scipy.linalg.lapack as L
+    rng = np.random.default_rng(n)
+    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    A = A + A.conj().T
+    V, tau = O.householder_tridiag_lower(A)
+    c, d, e, tau_l, info = L.zhetrd(A, lower=1)
+    assert info == 0
+    assert np.abs(tau[: n - 1] - tau_l[: n - 1]).max() < 1e-13
+    for k in range(n - 2):
+        assert np.abs(V[k + 2:, k] - c[k + 2:, k]).max() < 1e-12
+    Q = np.eye(n, dtype=complex)
+    for k in range(n - 1):
+        v = V[:, k]
+        Q = Q @ (np.eye(n) - tau[k] * np.outer(v, v.conj()))
+    T = Q.conj().T @ A @ Q
+    assert np.abs(np.diag(T).real - d).max() < 1e-12 * n
+    assert np.abs(np.diag(T, -1) - e).max() < 1e-12 * n
+
+
+def test_dnevx_phase_convention():
+    """dnevx's eigenvectors are Q z with z = Q^H v real and its first largest-magnitude
+    component positive (LAPACK zheevx's inverse-iteration path)."""
+    rng = np.random.default_rng(3)
+    n, nb = 30, 12
+    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    A = A + A.conj().T
+    w, v = O.dnevx(n, np.tril(A), nb)
+    assert np.abs(w - np.linalg.eigvalsh(A)[:nb]).max() < 1e-12
+    V, tau = O.householder_tridiag_lower(A)
+    z = O._apply_qh(V, tau, v)
+    assert np.abs(z.imag).max() < 1e-12
+    for j in range(nb):
+        i = int(np.argmax(np.abs(z[:, j])))
+        assert z[i, j].real > 0
