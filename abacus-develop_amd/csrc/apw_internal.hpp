@@ -117,6 +117,17 @@ struct ProfScope {
     ~ProfScope();
 };
 
+struct EigWorkspace {
+    DevBuf<double> a;      // n x n c128 working matrix
+    DevBuf<double> v;      // n x n c128 Householder vectors
+    DevBuf<double> tau;    // n c128
+    DevBuf<double> d, e;   // tridiagonal
+    DevBuf<double> p;      // 4n c128 exchange buffers of the global-memory reduction
+    DevBuf<double> ring;   // 2n(n-1) c128 exchange ring of the LDS-resident reduction
+    DevBuf<double> z;      // n x nband real eigenvectors of T
+    DevBuf<unsigned> bar;  // grid barrier counter + abort flag
+};
+
 }  // namespace apw
 
 struct apw_ctx {
@@ -127,6 +138,7 @@ struct apw_ctx {
     // scratch shared by the functor-level entry points
     apw::DevBuf<double> scratch;
     apw::GemmWorkspace gws;
+    apw::EigWorkspace ews;   // subspace eigensolver workspace of apw_dnevx_op
     apw::Prof prof;
 };
 
@@ -154,16 +166,6 @@ void gemm(hipStream_t s, GemmWorkspace& ws, int num_cu, char ta, char tb, int m,
           double beta_re, double beta_im, R* C, int ldc, Prof* prof = nullptr);
 
 // ---------------------------------------------------------------- eigensolver
-struct EigWorkspace {
-    DevBuf<double> a;      // n x n c128 working matrix
-    DevBuf<double> v;      // n x n c128 Householder vectors
-    DevBuf<double> tau;    // n c128
-    DevBuf<double> d, e;   // tridiagonal
-    DevBuf<double> p;      // 4n c128 exchange buffers of the global-memory reduction
-    DevBuf<double> ring;   // 2n(n-1) c128 exchange ring of the LDS-resident reduction
-    DevBuf<double> z;      // n x nband real eigenvectors of T
-    DevBuf<unsigned> bar;  // grid barrier counter + abort flag
-};
 // Lowest nband eigenpairs of Hermitian hcc (n x n, lower triangle, ld ldh), n <= HEEVX_MAX.
 constexpr int HEEVX_MAX = 2000;
 // eig: device double[nband]; vcc: device c128 (ld ldv), columns 0..nband-1 rows 0..n-1.

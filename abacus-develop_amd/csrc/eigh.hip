@@ -1053,8 +1053,8 @@ extern "C" int apw_dnevx_op(apw_ctx* ctx, int nstart, int ldh, const double* hcc
     return apw::guarded([&] {
         APW_CHECK(ctx && hcc && eigenvalue && vcc, "apw_dnevx_op: null argument");
         apw::DeviceGuard g(ctx->device);
-        apw::EigWorkspace ws;
-        apw::heevx_lowest(ctx->stream, ws, ctx->num_cu, nstart, ldh, hcc, nbands, eigenvalue, vcc, ldh);
+        apw::EigWorkspace& ws = ctx->ews;
+        apw::heevx_lowest(ctx->stream, ws, ctx->num_cu, nstart, ldh, hcc, nbands, eigenvalue, vcc, ldh, &ctx->prof);
         APW_HIP(hipStreamSynchronize(ctx->stream));
         unsigned flag[2] = {0, 0};
         if (nstart > 1) APW_HIP(hipMemcpy(flag, ws.bar.get(), sizeof(flag), hipMemcpyDeviceToHost));

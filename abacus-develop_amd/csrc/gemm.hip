@@ -7,9 +7,9 @@
 //   * update shapes         C(npw x n) += A(npw x k) B(k x n) with k <= ~2e3
 //     (back-projection, cal_grad, refresh, final psi) -> plain 2D tiling.
 //
-// Tile: 64 (m) x 64 (n) per workgroup, k step 16, 4 waves in a 2 x 2 arrangement, each
-// wave owning a 32 x 32 block = 2 x 2 MFMA 16x16 sub-tiles with separate real and
-// imaginary accumulators (a complex product = 4 real MFMAs).  Operand roles are swapped
+// Tile: 64 x 64, 128 x 64 or 64 x 128 (m x n) per workgroup, k step 16 through two LDS
+// stages, 4 waves in a 2 x 2 arrangement, each wave owning TM x TN MFMA 16x16 sub-tiles
+// with separate real and imaginary accumulators (a complex product = 4 real MFMAs).  Operand roles are swapped
 // (MFMA "A" <- op(B) indexed by n, MFMA "B" <- op(A) indexed by m) so that the 16 lanes of
 // a group hold 16 consecutive rows m of one column n: the column-major stores of C are then
 // 256-byte coalesced runs.  Accumulator register r of lane l holds D row drow(l, r) and
@@ -50,8 +50,10 @@ struct Mfma<float> {
 constexpr int BK = 16, NT = 256;
 
 // op codes: 0 = N, 1 = T, 2 = C.  Each of the 4 waves (2 x 2) owns (16 TM) x (16 TN) of the
-// (32 TM) x (32 TN) tile.  The next k-tile is fetched into registers while the MFMAs
-// consume the current one from LDS.
+// (32 TM) x (32 TN) tile.  Two LDS stages: while the MFMAs consume stage s, the k-tile
+// after the next is loaded into registers and the next one is written to stage s^1, so a
+// k step costs one workgroup barrier.  LDS rows are padded by one element so that the
+// k-fastest operand stores (16 lanes down a column) fall in distinct banks.
 template <class R, int TA, int TB, int TM, int TN>
 __global__ __launch_bounds__(NT) void zgemm_kernel(int M, int N, int K, int kchunk,
                                                    const typename Mfma<R>::c_t* __restrict__ A, int lda,
@@ -63,7 +65,9 @@ __global__ __launch_bounds__(NT) void zgemm_kernel(int M, int N, int K, int kchu
     using acc_t = typename Mfma<R>::acc_t;
     constexpr int BM = 32 * TM, BN = 32 * TN;
     constexpr int NA = BM * BK / NT, NB = BN * BK / NT;
-    __shared__ R As_re[BK][BM], As_im[BK][BM], Bs_re[BK][BN], Bs_im[BK][BN];
+    constexpr int LA = BM + 1, LB = BN + 1;
+    __shared__ R As[2][2][BK][LA];   // [stage][re, im][k][m]
+    __shared__ R Bs[2][2][BK][LB];   // [stage][re, im][k][n]
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -113,18 +117,15 @@ __global__ __launch_bounds__(NT) void zgemm_kernel(int M, int N, int K, int kchu
             okb |= (unsigned)ok << i;
         }
     };
-
-    if (kbeg < kend) fetch(kbeg);
-    for (int k0 = kbeg; k0 < kend; k0 += BK) {
-        __syncthreads();
+    auto stage = [&](int st) {
 #pragma unroll
         for (int i = 0; i < NA; ++i) {
             const int e = tid + NT * i;
             int mm, kk;
             if (TA == 0) { mm = e % BM; kk = e / BM; } else { kk = e % BK; mm = e / BK; }
             const bool ok = (oka >> i) & 1u;
-            As_re[kk][mm] = ok ? ra[i].x : R(0);
-            As_im[kk][mm] = ok ? (TA == 2 ? -ra[i].y : ra[i].y) : R(0);
+            As[st][0][kk][mm] = ok ? ra[i].x : R(0);
+            As[st][1][kk][mm] = ok ? (TA == 2 ? -ra[i].y : ra[i].y) : R(0);
         }
 #pragma unroll
         for (int i = 0; i < NB; ++i) {
@@ -132,11 +133,19 @@ __global__ __launch_bounds__(NT) void zgemm_kernel(int M, int N, int K, int kchu
             int nn, kk;
             if (TB == 0) { kk = e % BK; nn = e / BK; } else { nn = e % BN; kk = e / BN; }
             const bool ok = (okb >> i) & 1u;
-            Bs_re[kk][nn] = ok ? rb[i].x : R(0);
-            Bs_im[kk][nn] = ok ? (TB == 2 ? -rb[i].y : rb[i].y) : R(0);
+            Bs[st][0][kk][nn] = ok ? rb[i].x : R(0);
+            Bs[st][1][kk][nn] = ok ? (TB == 2 ? -rb[i].y : rb[i].y) : R(0);
         }
-        __syncthreads();
-        if (k0 + BK < kend) fetch(k0 + BK);
+    };
+
+    if (kbeg < kend) {
+        fetch(kbeg);
+        stage(0);
+        if (kbeg + BK < kend) fetch(kbeg + BK);
+    }
+    __syncthreads();
+    int st = 0;
+    for (int k0 = kbeg; k0 < kend; k0 += BK) {
 #pragma unroll
         for (int ks = 0; ks < BK / 4; ++ks) {
             const int kk = ks * 4 + (lane >> 4);
@@ -144,14 +153,14 @@ __global__ __launch_bounds__(NT) void zgemm_kernel(int M, int N, int K, int kchu
 #pragma unroll
             for (int t = 0; t < TN; ++t) {
                 const int nn = wn * 16 * TN + t * 16 + (lane & 15);
-                xr[t] = Bs_re[kk][nn];
-                xi[t] = Bs_im[kk][nn];
+                xr[t] = Bs[st][0][kk][nn];
+                xi[t] = Bs[st][1][kk][nn];
             }
 #pragma unroll
             for (int t = 0; t < TM; ++t) {
                 const int mm = wm * 16 * TM + t * 16 + (lane & 15);
-                yr[t] = As_re[kk][mm];
-                yi[t] = As_im[kk][mm];
+                yr[t] = As[st][0][kk][mm];
+                yi[t] = As[st][1][kk][mm];
             }
 #pragma unroll
             for (int tn = 0; tn < TN; ++tn)
@@ -163,6 +172,12 @@ __global__ __launch_bounds__(NT) void zgemm_kernel(int M, int N, int K, int kchu
                     acc_im[tn][tm] = Mfma<R>::mma(xi[tn], yr[tm], acc_im[tn][tm]);
                 }
         }
+        if (k0 + BK < kend) {
+            stage(st ^ 1);
+            if (k0 + 2 * BK < kend) fetch(k0 + 2 * BK);
+        }
+        __syncthreads();
+        st ^= 1;
     }
 
     // epilogue: lane holds D[n = drow(lane, r)][m = lane%16] of each sub-tile
