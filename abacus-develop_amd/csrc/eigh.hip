@@ -413,6 +413,228 @@ __global__ __launch_bounds__(NT) void k_tridiag_lds(const double2* __restrict__ 
     }
 }
 
+// Value of a per-lane register array at absolute row i (lane i % 64, slot i / 64),
+// broadcast to the whole wave.
+template <int R>
+__device__ inline double2 row_bcast(const double2 (&x)[R], int i) {
+    double2 t = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+        if (r == (i >> 6)) t = x[r];
+    return make_double2(__shfl(t.x, i & 63, 64), __shfl(t.y, i & 63, 64));
+}
+
+// Householder tridiagonalisation (lower), LDS-resident columns, vectors in registers.
+// Same ownership and exchange as k_tridiag_lds; what changes is where the per-column work
+// happens.  Every wave holds the full vectors v_k, y_k and the next reflector v_{k+1} in
+// registers (lane l: rows l + 64 r, r < R), so y, the rank-2-corrected column and the next
+// reflector are formed with wave reductions only, and one pass over each own column both
+// applies the rank-2 update of step k and accumulates p_{k+1} = A_{k+1}^H v_{k+1} (the
+// column is read and written once per step).  Workgroup barriers: the gather's only.
+template <int R>
+__global__ __launch_bounds__(NT) void k_tridiag_reg(const double2* __restrict__ A, int n, int ncol,
+                                                    double2* __restrict__ V, double2* __restrict__ tau,
+                                                    double* __restrict__ d, double* __restrict__ e,
+                                                    double2* __restrict__ ring, unsigned* abort_flag) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    double2* Al = reinterpret_cast<double2*>(smem);   // Al[lc * n + i] = A[i][c], c = blockIdx.x + lc * nwg
+    double2* gs = Al + (size_t)ncol * n;                // gathered p and column, two buffers of 2n
+    __shared__ int stop;
+    const int nwg = (int)gridDim.x, bid = (int)blockIdx.x;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    constexpr int NW = NT / 64;
+    for (int lc = 0; lc < ncol; ++lc) {
+        const int c = bid + lc * nwg;
+        if (c >= n) break;
+        for (int i = threadIdx.x; i < n; i += NT) Al[(size_t)lc * n + i] = A[(size_t)c * n + i];
+    }
+    __syncthreads();
+    double2 v[R], y[R], w[R];   // v_k, y_k, and the corrected column / v_{k+1}
+    // reflector of w on rows > k0 (alpha at row k0 + 1, the norm over rows > k0 + 1):
+    // overwrites w with it (1 at row k0 + 1, 0 at rows <= k0), returns tau and beta
+    auto reflector = [&](int k0, double2& tv, double& beta) {
+        double s2 = 0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int i = lane + 64 * r;
+            if (i > k0 + 1 && i < n) s2 += w[r].x * w[r].x + w[r].y * w[r].y;
+        }
+        const double xn2 = wave_sum(s2);
+        const double2 alpha = row_bcast<R>(w, k0 + 1);
+        double2 sc;
+        if (xn2 == 0.0 && alpha.y == 0.0) {
+            tv = make_double2(0.0, 0.0);
+            beta = alpha.x;
+            sc = make_double2(0.0, 0.0);
+        } else {
+            const double nrm = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xn2);
+            beta = alpha.x >= 0 ? -nrm : nrm;
+            tv = make_double2((beta - alpha.x) / beta, -alpha.y / beta);
+            const double2 den = make_double2(alpha.x - beta, alpha.y);
+            const double dd = den.x * den.x + den.y * den.y;
+            sc = make_double2(den.x / dd, -den.y / dd);
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int i = lane + 64 * r;
+            w[r] = (i == k0 + 1) ? make_double2(1.0, 0.0) : ((i > k0 + 1 && i < n) ? cmul(w[r], sc) : make_double2(0.0, 0.0));
+        }
+    };
+    // p_{k+1}[c] = sum_{i > k+1} conj(A[i][c]) w_i over own columns c > k + 1 (k = -1 for
+    // the first reflector), optionally after the rank-2 update with (v, y) of step k;
+    // published into the ring slot of step k + 1 with the owner's copy of column k + 2.
+    auto update_and_project = [&](int k, bool update) {
+        double2* slot = ring + (size_t)(k + 1) * 2 * n;
+        const int lc1 = (k + 2 - bid + nwg - 1) / nwg;
+        for (int lc = lc1 + wave; lc < ncol; lc += NW) {
+            const int c = bid + lc * nwg;
+            if (c >= n) break;
+            double2* col = Al + (size_t)lc * n;
+            double2 yc = make_double2(0.0, 0.0), vc = make_double2(0.0, 0.0);
+            if (update) {
+                yc = row_bcast<R>(y, c);
+                vc = row_bcast<R>(v, c);
+            }
+            double pr = 0, pi = 0;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int i = lane + 64 * r;
+                if (i > k + 1 && i < n) {
+                    double2 a = col[i];
+                    if (update) {
+                        const double2 a1 = cconjmul(yc, v[r]);   // v_i conj(y_c)
+                        const double2 a2 = cconjmul(vc, y[r]);   // y_i conj(v_c)
+                        a.x -= a1.x + a2.x;
+                        a.y -= a1.y + a2.y;
+                        col[i] = a;
+                    }
+                    const double2 q = cconjmul(a, w[r]);
+                    pr += q.x;
+                    pi += q.y;
+                }
+            }
+            pr = wave_sum(pr);
+            pi = wave_sum(pi);
+            if (k + 2 < n && lane == 0) ring_put(slot + (c - (k + 2)), make_double2(pr, pi));
+            if (c == k + 2) {
+                // owner: publish the updated column k + 2 (rows k+2 .. n-1)
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const int i = lane + 64 * r;
+                    if (i >= k + 2 && i < n) ring_put(slot + n + (i - (k + 2)), col[i]);
+                }
+            }
+        }
+    };
+    // first reflector from column 0 (rows 1 .. n-1)
+    double2 t;
+    double beta;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int i = lane + 64 * r;
+        w[r] = (i >= 1 && i < n) ? A[i] : make_double2(0.0, 0.0);
+    }
+    reflector(0, t, beta);
+    if (bid == 0 && wave == 0) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int i = lane + 64 * r;
+            if (i >= 1 && i < n) V[i] = w[r];
+        }
+        if (lane == 0) {
+            tau[0] = t;
+            d[0] = A[0].x;
+            e[0] = beta;
+        }
+    }
+    // p_0 and column 1 (its owner publishes it as loaded)
+    update_and_project(-1, false);
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = w[r];
+    for (int k = 0; k < n - 1; ++k) {
+        const int m = n - k - 1;
+        const double2* slot = ring + (size_t)k * 2 * n;
+        double2* g = gs + (size_t)(k & 1) * 2 * n;
+        // ---- gather p_k and column k+1 ----
+        {
+            unsigned spins = 0;
+            while (true) {
+                bool mine = true;
+                for (int i = threadIdx.x; i < 2 * m; i += NT) {
+                    const int ii = i < m ? i : n + (i - m);
+                    double2 val;
+                    if (ring_get(slot + ii, val)) g[ii] = val;
+                    else mine = false;
+                }
+                if (__syncthreads_and(mine)) break;
+                if (threadIdx.x == 0) {
+                    int st = __hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u;
+                    if (!st && ++spins > SPIN_LIMIT) {
+                        atomicExch(abort_flag, 1u);
+                        st = 1;
+                    }
+                    stop = st;
+                }
+                __syncthreads();
+                if (stop) break;
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+        // ---- y = tau p - (|tau|^2 mu / 2) v ----
+        double mr = 0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int i = lane + 64 * r;
+            const bool in = i > k && i < n;
+            y[r] = in ? g[i - (k + 1)] : make_double2(0.0, 0.0);
+            w[r] = in ? g[n + (i - (k + 1))] : make_double2(0.0, 0.0);
+            mr += cconjmul(v[r], y[r]).x;
+        }
+        const double mu = wave_sum(mr);
+        const double c2 = 0.5 * (t.x * t.x + t.y * t.y) * mu;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const double2 tp = cmul(t, y[r]);
+            y[r] = make_double2(tp.x - c2 * v[r].x, tp.y - c2 * v[r].y);
+        }
+        // ---- column k+1 after the rank-2 update, then the next reflector ----
+        {
+            const double2 v0 = row_bcast<R>(v, k + 1), y0 = row_bcast<R>(y, k + 1);
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const double2 a1 = cconjmul(v0, y[r]);
+                const double2 a2 = cconjmul(y0, v[r]);
+                w[r].x -= a1.x + a2.x;
+                w[r].y -= a1.y + a2.y;
+            }
+        }
+        const double dk1 = row_bcast<R>(w, k + 1).x;
+        if (m >= 2) {
+            double2 tn;
+            double bn;
+            reflector(k + 1, tn, bn);
+            if (bid == 0 && wave == 0) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const int i = lane + 64 * r;
+                    if (i >= k + 2 && i < n) V[(size_t)(k + 1) * n + i] = w[r];
+                }
+                if (lane == 0) {
+                    tau[k + 1] = tn;
+                    d[k + 1] = dk1;
+                    e[k + 1] = bn;
+                }
+            }
+            update_and_project(k, true);
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[r] = w[r];
+            t = tn;
+        } else if (bid == 0 && threadIdx.x == 0) {
+            d[k + 1] = dk1;
+        }
+    }
+}
+
 // Sturm count: number of eigenvalues of T strictly below x
 // d and e2 (squared off-diagonals) in LDS; every lane of a wave reads the same address.
 __device__ int sturm(const double* d, const double* e2, int n, double x, double pivmin) {
@@ -960,7 +1182,43 @@ void heevx_lowest(hipStream_t s, EigWorkspace& ws, int num_cu, int n, int ldh, c
                 break;
             }
         }
-        if (nwg > 0) {
+        const int R = (n + 63) / 64;
+        if (R <= 16) {
+            // register form: LDS = own columns + two gather buffers of 2n
+            const size_t fixed_r = (size_t)4 * n * sizeof(double2);
+            int w = std::max(1, std::min(n / 8, 16)), nc = 0;
+            for (; w <= std::min(num_cu, 256); ++w) {
+                nc = (n + w - 1) / w;
+                if (fixed_r + (size_t)nc * n * sizeof(double2) <= TRIDIAG_LDS_BYTES) break;
+            }
+            APW_CHECK(w <= std::min(num_cu, 256), "heevx: no workgroup count fits n=%d", n);
+            const size_t lds = fixed_r + (size_t)nc * n * sizeof(double2);
+            const size_t nring = (size_t)2 * n * n;
+            ws.ring.reserve(2 * nring);
+            double2* ring = reinterpret_cast<double2*>(ws.ring.get());
+            hipLaunchKernelGGL(k_fill_ring, dim3((unsigned)std::min<size_t>(1024, (nring + 255) / 256)), dim3(256), 0, s,
+                               ring, nring);
+            APW_LAUNCH_CHECK();
+            const void* fn = nullptr;
+            switch (R <= 6 ? R : (R <= 8 ? 8 : (R <= 10 ? 10 : (R <= 12 ? 12 : 16)))) {
+                case 1: fn = reinterpret_cast<const void*>(k_tridiag_reg<1>); break;
+                case 2: fn = reinterpret_cast<const void*>(k_tridiag_reg<2>); break;
+                case 3: fn = reinterpret_cast<const void*>(k_tridiag_reg<3>); break;
+                case 4: fn = reinterpret_cast<const void*>(k_tridiag_reg<4>); break;
+                case 5: fn = reinterpret_cast<const void*>(k_tridiag_reg<5>); break;
+                case 6: fn = reinterpret_cast<const void*>(k_tridiag_reg<6>); break;
+                case 8: fn = reinterpret_cast<const void*>(k_tridiag_reg<8>); break;
+                case 10: fn = reinterpret_cast<const void*>(k_tridiag_reg<10>); break;
+                case 12: fn = reinterpret_cast<const void*>(k_tridiag_reg<12>); break;
+                default: fn = reinterpret_cast<const void*>(k_tridiag_reg<16>); break;
+            }
+            APW_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+            const double2* Ac = A;
+            unsigned* abortp = barp + 1;
+            int ncol_r = nc;
+            void* args[] = {&Ac, &n, &ncol_r, &Vp, &taup, &dp, &ep, &ring, &abortp};
+            APW_HIP(hipLaunchCooperativeKernel(fn, dim3(w), dim3(NT), args, (unsigned)lds, s));
+        } else if (nwg > 0) {
             const size_t lds = fixed + (size_t)ncol * (n + 1) * sizeof(double2);
             const size_t nring = (size_t)2 * n * (n - 1);
             ws.ring.reserve(2 * nring);
