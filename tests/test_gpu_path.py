@@ -356,6 +356,36 @@ def test_david_si8_mixed(torch, ctx, si8):
     assert np.abs(eig - oe).max() < 1e-9
 
 
+def _oracle_case(key):
+    with open(os.path.join(GOLD, "oracle_eigs.json")) as f:
+        ref = json.load(f)[key]
+    name, ik = key.split(":")
+    s = configs.make(name, ik=int(ik)) if name == "kpts16" else configs.make(name)
+    assert (s.npw, s.nband) == (ref["npw"], ref["nband"])
+    return s, ref
+
+
+@pytest.mark.parametrize("key", ["kpts16:0", "kpts16:37", "cell1:0", "cell3:0", "cell6:0"])
+def test_david_kpoints_and_cells_fp64(torch, ctx, key):
+    """Multi-k workload (k = 0 and a general k of the 4x4x4 mesh: uneven spheres, complex
+    phases in vkb) and cells of the uneven batch (18^3-64^3 grids): FP64 eigenvalues
+    against the oracle's (fixture written by tests/golden/make_oracle_eigs.py)."""
+    s, ref = _oracle_case(key)
+    eig, psi, it, st = _run_david(torch, ctx, s)
+    de = np.abs(eig - np.array(ref["eig"])).max()
+    print(key, "gpu iters", it, "oracle iters", ref["iterations"], "max |de|", de)
+    assert de < 1e-10
+
+
+@pytest.mark.parametrize("key", ["kpts16:37", "cell6:0"])
+def test_david_kpoints_and_cells_mixed(torch, ctx, key):
+    s, ref = _oracle_case(key)
+    eig, psi, it, st = _run_david(torch, ctx, s, precision="fp32_mixed")
+    de = np.abs(eig - np.array(ref["eig"])).max()
+    print(key, "mixed iters", it, "switch at", st["fp64_switch_iter"], "max |de|", de)
+    assert de < 1e-9
+
+
 @pytest.mark.parametrize("n,nband,ndim", [(200, 8, 4), (900, 150, 2)])
 def test_david_callback_dense(torch, ctx, n, nband, ndim):
     """Reference-style test (diago_david_test.cpp): a dense random Hermitian H applied by a

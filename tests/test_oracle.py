@@ -167,3 +167,14 @@ def test_dnevx_phase_convention():
     for j in range(nb):
         i = int(np.argmax(np.abs(z[:, j])))
         assert z[i, j].real > 0
+
+
+def test_oracle_eig_fixture_is_current():
+    """The committed multi-k / uneven-cell eigenvalue fixture matches the oracle as it is
+    now (cheapest case recomputed; the GPU parity tests compare against the fixture)."""
+    ref = _load("oracle_eigs.json")["cell3:0"]
+    s = configs.make("cell3")
+    assert (s.npw, s.nband, [s.nx, s.ny, s.nz]) == (ref["npw"], ref["nband"], ref["grid"])
+    eig, _, iters, _ = O.david_solve(s, thr=1e-9)
+    assert iters == ref["iterations"]
+    assert np.abs(eig - np.array(ref["eig"])).max() < 1e-12
