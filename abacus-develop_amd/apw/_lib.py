@@ -77,6 +77,22 @@ PROTOTYPES = {
     "apw_set_real_to_recip_output_op": (_i, [_vp, _i, _i, _i, _d, _vp, _vp, _vp]),
     "apw_fft3d": (_i, [_vp, _i, _i, _i, _vp, _i]),
     "apw_zgemm": (_i, [_vp, _ch, _ch, _i, _i, _i, C.POINTER(_d), _vp, _i, _vp, _i, C.POINTER(_d), _vp, _i]),
+    "apw_ham_set_vkb": (_i, [_vp, _vp, _i]),
+    "apw_cal_ylm_real_op": (_i, [_vp, _i, _i, _d, _d, _d, _d, _d, _vp, _vp, _vp]),
+    "apw_cal_sk_op": (_i, [_vp] + [_i] * 14 + [_vp, _vp, _vp, _d, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "apw_cal_vnl_op": (_i, [_vp, _i, _i, _i, _i, _i, _i, _vp, _vp, _vp, _d, _d, _d, _d] + [_vp] * 9),
+    "apw_getvnl": (_i, [_vp, _i, _i, _i, _i, _i, _i, _vp, _vp, _vp, _d, _d] + [_vp] * 7),
+    "apw_write_wfc_pw": (_i, [_vp, C.c_char_p, _i, _i, _vp, _d, _i, _d, _d, _d, _vp, _i, _i, _i, _i, _vp, _vp, _i]),
+    "apw_read_wfc_pw": (_i, [_vp, C.c_char_p, _i, _i, _i, _vp, _d, _d, _vp, _i, _i, _i, _i, _vp, _vp, _i]),
+    "apw_dngvx_op": (_i, [_vp, _i, _i, _vp, _vp, _i, _vp, _vp]),
+    "apw_dngvd_op": (_i, [_vp, _i, _i, _vp, _vp, _vp, _vp]),
+    "apw_dav_subspace_create": (_i, [_vp, C.POINTER(_d), _i, _i, _i, _d, _i, _i, C.POINTER(_vp)]),
+    "apw_dav_subspace_destroy": (_i, [_vp]),
+    "apw_dav_subspace_diag": (_i, [_vp, _vp, HPSI_CB, _vp, _i, _vp, C.POINTER(_d), C.POINTER(_i), _i,
+                                   C.POINTER(DiagStats)]),
+    "apw_psi_to_rho": (_i, [_vp, _vp, _i, _i, C.POINTER(_d), _d, _vp, _i]),
+    "apw_elecstate_pw_op": (_i, [_vp, _i, _i, _d, C.POINTER(_vp), _vp]),
+    "apw_elecstate_pw_op_nc": (_i, [_vp, _i, _i, _i, _d, C.POINTER(_vp), _vp, _vp]),
     "apw_cgemm": (_i, [_vp, _ch, _ch, _i, _i, _i, C.POINTER(C.c_float), _vp, _i, _vp, _i, C.POINTER(C.c_float), _vp, _i]),
     "apw_dnevx_op": (_i, [_vp, _i, _i, _vp, _i, _vp, _vp]),
     "apw_probe_mfma_layout": (_i, [_vp, _vp]),

@@ -19,6 +19,7 @@ aborts instead).
 from __future__ import annotations
 
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -160,6 +161,24 @@ class HamiltPW:
         self.ctx.pre()
         check(self.lib.apw_real_to_recip(self.handle, _ptr(inp), _ptr(out), int(add), float(factor)))
 
+    def set_vkb(self, vkb, ldvkb: int):
+        """Use device projectors (nkb rows of ``ldvkb``, c128) from now on."""
+        self.ctx.pre()
+        check(self.lib.apw_ham_set_vkb(self.handle, _ptr(vkb), int(ldvkb)))
+
+    def psi_to_rho(self, psi, nbands, wg, omega, rho=None, add=False, ldpsi=None):
+        """ElecStatePW::psiToRho for this k-point (nspin 1): rho(r) (+)= sum_b wg[b]/omega
+        |psi_b(r)|^2; ``psi`` device c128 (nbands, ld), ``rho`` device float64 (nxyz)."""
+        torch = _torch()
+        ld = ldpsi or psi.shape[-1]
+        if rho is None:
+            rho = torch.zeros(self.nx * self.ny * self.nz, dtype=torch.float64, device=f"cuda:{self.ctx.device}")
+        w = (C.c_double * max(1, nbands))(*[float(x) for x in wg[:nbands]])
+        self.ctx.pre()
+        check(self.lib.apw_psi_to_rho(self.handle, _ptr(_dev_c128(psi)), ld, nbands, w, float(omega), _ptr(rho),
+                                      int(add)))
+        return rho
+
     def close(self):
         if self.handle:
             self.lib.apw_ham_destroy(self.handle)
@@ -271,6 +290,75 @@ def _wrap(ptr, count, device):
 
 
 # ---------------------------------------------------------------------------- functors
+class DiagoDavSubspace:
+    """hsolver::Diago_DavSubspace on the GPU (the dav_subspace method): ``precondition``
+    is a host array (len >= dim; HSolverPW passes |k+G|^2 tpiba2 for this method)."""
+
+    def __init__(self, ctx: Context, precondition, nband: int, dim: int, david_ndim: int, diag_thr: float,
+                 diag_nmax: int, need_subspace: bool = False):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.nband, self.dim = nband, dim
+        pre = np.ascontiguousarray(precondition[:dim], dtype=np.float64)
+        self.handle = C.c_void_p()
+        check(self.lib.apw_dav_subspace_create(ctx.handle, pre.ctypes.data_as(C.POINTER(C.c_double)), nband, dim,
+                                               david_ndim, float(diag_thr), int(diag_nmax), int(need_subspace),
+                                               C.byref(self.handle)))
+        self.stats = None
+
+    def diag(self, hpsi_func, psi, psi_in_dmax, eigenvalue_in, is_occupied=None, scf_type=True):
+        """Diago_DavSubspace::diag(hpsi_func, psi_in, psi_in_dmax, eigenvalue_in,
+        is_occupied, scf_type).  ``hpsi_func``: a :class:`HamiltPW` or a callable
+        ``f(X, HX, nvec, dim, id_start, id_end)`` on torch views; ``psi``: CUDA complex128
+        (nband x psi_in_dmax); ``eigenvalue_in``: host float64 array, filled in place."""
+        torch = _torch()
+        _dev_c128(psi)
+        eig = np.zeros(self.nband, dtype=np.float64)
+        stats = _lib.DiagStats()
+        ham = None
+        hcb = _lib.HPSI_CB()
+        errors = []
+        if isinstance(hpsi_func, HamiltPW):
+            ham = hpsi_func.handle
+        else:
+            def _h(user, X, HX, nvec, d, i0, i1):
+                try:
+                    xs = _wrap(X, nvec * d, psi.device).view(nvec, d)
+                    hs = _wrap(HX, (i1 - i0 + 1) * d, psi.device).view(i1 - i0 + 1, d)
+                    hpsi_func(xs, hs, nvec, d, i0, i1)
+                    torch.cuda.synchronize(psi.device)
+                    return 0
+                except Exception as exc:
+                    errors.append(exc)
+                    return 1
+
+            hcb = _lib.HPSI_CB(_h)
+        occ = None
+        if is_occupied is not None:
+            occ = (C.c_int * self.nband)(*[int(bool(x)) for x in is_occupied[: self.nband]])
+        torch.cuda.synchronize(psi.device)
+        rc = self.lib.apw_dav_subspace_diag(self.handle, ham, hcb, None, int(psi_in_dmax), _ptr(psi),
+                                            eig.ctypes.data_as(C.POINTER(C.c_double)), occ, int(bool(scf_type)),
+                                            C.byref(stats))
+        if errors:
+            raise errors[0]
+        check(rc)
+        eigenvalue_in[: self.nband] = eig
+        self.stats = stats.as_dict()
+        return rc
+
+    def close(self):
+        if self.handle:
+            self.lib.apw_dav_subspace_destroy(self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def ekinetic_pw_op(ctx, nband, npw, max_npw, tpiba2, gk2, hpsi, psi):
     ctx.pre()
     check(ctx.lib.apw_ekinetic_pw_op(ctx.handle, nband, npw, max_npw, float(tpiba2), _ptr(gk2), _ptr(hpsi), _ptr(psi)))
@@ -279,6 +367,20 @@ def ekinetic_pw_op(ctx, nband, npw, max_npw, tpiba2, gk2, hpsi, psi):
 def veff_pw_op(ctx, size, out, inp):
     ctx.pre()
     check(ctx.lib.apw_veff_pw_op(ctx.handle, size, _ptr(out), _ptr(inp)))
+
+
+def elecstate_pw_op(ctx, spin, nrxx, w1, rho, wfcr):
+    """elecstate_pw_op: rho[spin] += w1 |wfcr|^2; ``rho`` a list of device float64 arrays."""
+    ptrs = (C.c_void_p * len(rho))(*[_ptr(r) for r in rho])
+    ctx.pre()
+    check(ctx.lib.apw_elecstate_pw_op(ctx.handle, spin, nrxx, float(w1), ptrs, _ptr(wfcr)))
+
+
+def elecstate_pw_op_nc(ctx, domag, domag_z, nrxx, w1, rho, wfcr, wfcr_another_spin):
+    ptrs = (C.c_void_p * 4)(*[_ptr(r) for r in rho])
+    ctx.pre()
+    check(ctx.lib.apw_elecstate_pw_op_nc(ctx.handle, int(domag), int(domag_z), nrxx, float(w1), ptrs, _ptr(wfcr),
+                                         _ptr(wfcr_another_spin)))
 
 
 def nonlocal_pw_op(ctx, l1, l2, l3, sum_, iat, spin, nkb, deeq_x, deeq_y, deeq_z, deeq, ps, becp):
@@ -325,6 +427,90 @@ def gemm_op(ctx, transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc):
 def dnevx_op(ctx, nstart, ldh, hcc, nbands, eigenvalue, vcc):
     ctx.pre()
     check(ctx.lib.apw_dnevx_op(ctx.handle, nstart, ldh, _ptr(hcc), nbands, _ptr(eigenvalue), _ptr(vcc)))
+
+
+def _hptr(a, dtype):
+    """Host pointer of a numpy copy (kept alive by the returned pair)."""
+    arr = np.ascontiguousarray(a, dtype=dtype)
+    return arr, C.c_void_p(arr.ctypes.data)
+
+
+def cal_ylm_real_op(ctx, ng, lmax, g, p, ylm):
+    """ModuleBase::cal_ylm_real_op (math_op.h) with the reference's constants; device
+    tensors (``p`` may be None)."""
+    import math
+    ctx.pre()
+    check(ctx.lib.apw_cal_ylm_real_op(ctx.handle, ng, lmax, math.sqrt(2.0), math.pi, 0.5 * math.pi, 4.0 * math.pi,
+                                      1.0 / math.sqrt(4.0 * math.pi), _ptr(g), None if p is None else _ptr(p),
+                                      _ptr(ylm)))
+
+
+def cal_sk_op(ctx, ik, ntype, nx, ny, nz, rho_nx, rho_ny, rho_nz, npw, npwx, fftny, eigts1_nc, eigts2_nc, eigts3_nc,
+              atom_na, igl2isz, is2fftixy, two_pi, kvec_c, atom_tau, eigts1, eigts2, eigts3, sk):
+    """hamilt::cal_sk_op (wf_op.h); all arrays device tensors."""
+    ctx.pre()
+    check(ctx.lib.apw_cal_sk_op(ctx.handle, ik, ntype, nx, ny, nz, rho_nx, rho_ny, rho_nz, npw, npwx, fftny,
+                                eigts1_nc, eigts2_nc, eigts3_nc, _ptr(atom_na), _ptr(igl2isz), _ptr(is2fftixy),
+                                float(two_pi), _ptr(kvec_c), _ptr(atom_tau), _ptr(eigts1), _ptr(eigts2),
+                                _ptr(eigts3), _ptr(sk)))
+
+
+def cal_vnl_op(ctx, ntype, npw, npwx, nhm, tab_2, tab_3, atom_na, atom_nb, atom_nh, dq, tpiba, neg_imag_unit, gk,
+               ylm, indv, nhtol, nhtolm, tab, vkb1, sk, vkb_in):
+    """hamilt::cal_vnl_op (vnl_op.h); all arrays device tensors."""
+    ctx.pre()
+    ni = complex(neg_imag_unit)
+    check(ctx.lib.apw_cal_vnl_op(ctx.handle, ntype, npw, npwx, nhm, tab_2, tab_3, _ptr(atom_na), _ptr(atom_nb),
+                                 _ptr(atom_nh), float(dq), float(tpiba), ni.real, ni.imag, _ptr(gk), _ptr(ylm),
+                                 _ptr(indv), _ptr(nhtol), _ptr(nhtolm), _ptr(tab), _ptr(vkb1), _ptr(sk),
+                                 _ptr(vkb_in)))
+
+
+def getvnl(ctx, ntype, npw, npwx, nhm, tab_2, tab_3, atom_na, atom_nb, atom_nh, dq, tpiba, gk, tau, indv, nhtol,
+           nhtolm, tab, vkb):
+    """pseudopot_cell_vnl::getvnl for one k-point in one pass.  ``atom_na/nb/nh``,
+    ``indv/nhtol/nhtolm`` (ntype x nhm) and ``tau`` (nat x 3, lat0 units) are host arrays;
+    ``gk`` (npw x 3, 2 pi / lat0), ``tab`` and ``vkb`` (nkb x npwx c128) device tensors."""
+    ctx.pre()
+    keep = [_hptr(atom_na, np.int32), _hptr(atom_nb, np.int32), _hptr(atom_nh, np.int32), _hptr(tau, np.float64),
+            _hptr(indv, np.float64), _hptr(nhtol, np.float64), _hptr(nhtolm, np.float64)]
+    na, nb, nh, ta, iv, hl, hlm = (k[1] for k in keep)
+    check(ctx.lib.apw_getvnl(ctx.handle, ntype, npw, npwx, nhm, tab_2, tab_3, na, nb, nh, float(dq), float(tpiba),
+                             _ptr(gk), ta, iv, hl, hlm, _ptr(tab), _ptr(vkb)))
+
+
+def write_wfc_pw(ctx, filename, ik_global, nkstot, kvec_c, weight, nbands, ecutwfc, lat0, tpiba, G, nx, ny, nz, npw,
+                 ig2ixyz, psi, ldpsi):
+    """ModuleIO::write_wfc_pw for one k-point: ``filename`` ends in .dat (binary) or .txt;
+    ``kvec_c`` (3), ``G`` (3 x 3) and ``ig2ixyz`` (npw) host arrays, ``psi`` a CUDA c128
+    tensor (band b at offset b * ldpsi)."""
+    ctx.pre()
+    kv, pk = _hptr(kvec_c, np.float64)
+    g, pg = _hptr(np.asarray(G, dtype=np.float64).reshape(9), np.float64)
+    ix, pi = _hptr(ig2ixyz[:npw], np.int32)
+    check(ctx.lib.apw_write_wfc_pw(ctx.handle, os.fsencode(filename), ik_global, nkstot, pk, float(weight), nbands,
+                                   float(ecutwfc), float(lat0), float(tpiba), pg, nx, ny, nz, npw, pi,
+                                   _ptr(_dev_c128(psi)), ldpsi))
+
+
+def read_wfc_pw(ctx, filename, ik, nkstot, nbands, kvec_c, lat0, tpiba, G, nx, ny, nz, npw, ig2ixyz, wfc, ldwfc):
+    """ModuleIO::read_wfc_pw for one k-point into the CUDA c128 tensor ``wfc``."""
+    ctx.pre()
+    kv, pk = _hptr(kvec_c, np.float64)
+    g, pg = _hptr(np.asarray(G, dtype=np.float64).reshape(9), np.float64)
+    ix, pi = _hptr(ig2ixyz[:npw], np.int32)
+    check(ctx.lib.apw_read_wfc_pw(ctx.handle, os.fsencode(filename), ik, nkstot, nbands, pk, float(lat0),
+                                  float(tpiba), pg, nx, ny, nz, npw, pi, _ptr(_dev_c128(wfc)), ldwfc))
+
+
+def dngvx_op(ctx, nstart, ldh, hcc, scc, m, eigenvalue, vcc):
+    ctx.pre()
+    check(ctx.lib.apw_dngvx_op(ctx.handle, nstart, ldh, _ptr(hcc), _ptr(scc), m, _ptr(eigenvalue), _ptr(vcc)))
+
+
+def dngvd_op(ctx, nstart, ldh, hcc, scc, eigenvalue, vcc):
+    ctx.pre()
+    check(ctx.lib.apw_dngvd_op(ctx.handle, nstart, ldh, _ptr(hcc), _ptr(scc), _ptr(eigenvalue), _ptr(vcc)))
 
 
 def probe_mfma_layout(ctx):

@@ -169,6 +169,43 @@ def _projectors(lat: Lattice, basis: PWBasisK, ik: int, tau: np.ndarray, widths:
     return vkb
 
 
+# reference Ylm row (nhtolm, Y_lm order of cal_ylm_real_op) of each of the 8 projectors and
+# the sign that maps it onto the x, y, z components used above: Y_1 rows are z (lm 1),
+# -x (lm 2) and -y (lm 3)
+_PROJ_LM = np.array([0, 0, 2, 3, 1, 2, 3, 1])
+_PROJ_SIGN = np.array([1.0, 1.0, -1.0, -1.0, 1.0, -1.0, -1.0, 1.0])
+
+
+def projector_tables(sysm: "PWSystem", dq: float = 0.01) -> dict:
+    """The same projectors in the form pseudopot_cell_vnl::getvnl consumes
+    (VNL_in_pw.cpp:412-524): a radial table tab[it][nb][iq] = beta_nb(iq dq) sampled every
+    dq bohr^-1 (the reference's default, input_parameter dq = 0.01) up to past the cutoff,
+    indv / nhtol / nhtolm per projector, k+G in 2 pi / lat0 units and tau in lat0 units.
+    getvnl of these reproduces ``sysm.vkb`` row by row up to ``sign`` and the
+    interpolation error; the radial part of an l = 1 channel carries the |q| of
+    |q| Y_1m(q^) = y1 q_m."""
+    lat, basis, ik = sysm.lat, sysm.basis, sysm.ik
+    npw = sysm.npw
+    gk = np.ascontiguousarray(basis.gcar_k[ik, :npw])                       # 2 pi / lat0
+    qmax = float(np.sqrt((gk ** 2).sum(-1)).max()) * lat.tpiba
+    nqx = int(qmax / dq) + 8
+    q = np.arange(nqx) * dq
+    widths = np.asarray(sysm.meta["widths"])
+    ch_l = [0, 0, 1, 1]
+    inv_sqrt_omega = 1.0 / math.sqrt(lat.omega)
+    tab = np.zeros((1, 4, nqx))
+    for ch, (w, l) in enumerate(zip(widths, ch_l)):
+        integral = math.sqrt(math.pi) / (4.0 * w ** 3) if l == 0 else 3.0 * math.sqrt(math.pi) / (8.0 * w ** 5)
+        norm = math.sqrt((2.0 * math.pi) ** 3 / integral)
+        tab[0, ch] = norm * inv_sqrt_omega * np.exp(-0.5 * (w * q) ** 2) * (q if l == 1 else 1.0)
+    return {"ntype": 1, "npw": npw, "npwx": sysm.npwx, "nhm": NH_PER_ATOM, "tab_2": 4, "tab_3": nqx,
+            "atom_na": np.array([sysm.nat], dtype=np.int32), "atom_nb": np.array([4], dtype=np.int32),
+            "atom_nh": np.array([NH_PER_ATOM], dtype=np.int32), "dq": dq, "tpiba": lat.tpiba, "gk": gk,
+            "tau": sysm.tau / lat.lat0, "indv": _PROJ_CHANNEL.astype(np.float64),
+            "nhtol": _PROJ_L.astype(np.float64), "nhtolm": _PROJ_LM.astype(np.float64),
+            "tab": tab.reshape(-1), "sign": _PROJ_SIGN}
+
+
 def build_system(name: str, lat: Lattice, pos_direct: np.ndarray, ecutwfc: float, nband: int,
                  david_ndim: int, seed: int, grid=None, kvec_d=None, ik: int = 0,
                  basis: PWBasisK | None = None) -> PWSystem:

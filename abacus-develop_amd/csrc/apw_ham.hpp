@@ -17,7 +17,7 @@ struct Ham {
     DevBuf<float> veff32, vkb32;
     DevBuf<int> kb_iat, kb_off, kb_nh;
     // work buffers (grow-only)
-    DevBuf<double> S, X, becp, ps, psi32, h32;
+    DevBuf<double> S, X, becp, ps, psi32, h32, wts;
     GemmWorkspace gws;
 
     void create(apw_ctx* c, const apw_ham_desc* d);

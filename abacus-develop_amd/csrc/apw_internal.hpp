@@ -122,10 +122,14 @@ struct EigWorkspace {
     DevBuf<double> v;      // n x n c128 Householder vectors
     DevBuf<double> tau;    // n c128
     DevBuf<double> d, e;   // tridiagonal
-    DevBuf<double> p;      // 4n c128 exchange buffers of the global-memory reduction
-    DevBuf<double> ring;   // 2n(n-1) c128 exchange ring of the LDS-resident reduction
+    DevBuf<double> ring;   // 2n^2 c128 exchange ring of the reduction
     DevBuf<double> z;      // n x nband real eigenvectors of T
-    DevBuf<unsigned> bar;  // grid barrier counter + abort flag
+    DevBuf<unsigned> bar;  // [1]: abort word of the exchange polls
+};
+
+// work arrays of the generalized (pencil) solver
+struct GenWorkspace {
+    DevBuf<double> a, r, t, x, c, t2, y;
 };
 
 }  // namespace apw
@@ -139,6 +143,11 @@ struct apw_ctx {
     apw::DevBuf<double> scratch;
     apw::GemmWorkspace gws;
     apw::EigWorkspace ews;   // subspace eigensolver workspace of apw_dnevx_op
+    apw::GenWorkspace gen;   // and of apw_dngvx_op / apw_dngvd_op
+    apw::DevBuf<int> status;
+    apw::DevBuf<double> vnl_tau;   // atom positions of apw_getvnl
+    apw::DevBuf<double> io_stage;  // bands of a wavefunction file, file order
+    apw::DevBuf<int> io_perm;      // file position of each plane wave of the basis
     apw::Prof prof;
 };
 
@@ -180,5 +189,16 @@ void heevx_lowest(hipStream_t s, EigWorkspace& ws, int num_cu, int n, int ldh, c
 constexpr int CHOL_MAX = 128;
 void chol_inv_upper(hipStream_t s, int n, const double* g, int ldg, double* rinv, int ldr, int* status,
                     double min_norm, Prof* prof = nullptr);
+
+// ---------------------------------------------------------------- generalized problem
+// The same factor and inverse for any n (blocks of CHOL_MAX, GEMM updates); x gets U^-1
+// (n x n, ld ldx, zeros below).  status[0] != 0 when a diagonal block breaks down.
+void chol_inv_blocked(hipStream_t s, GemmWorkspace& gws, GenWorkspace& w, int num_cu, int n, const double* g,
+                      int ldg, double* x, int ldx, int* status, double min_norm, Prof* prof = nullptr);
+// Lowest nband pairs of H x = lambda S x (hcc, scc: n x n Hermitian, both triangles, ld
+// ldh; S positive definite): eig device double[nband], vcc device c128 (ld ldv), x^H S x = I.
+void hegvx_lowest(hipStream_t s, GemmWorkspace& gws, EigWorkspace& ews, GenWorkspace& w, int num_cu, int n, int ldh,
+                  const double* hcc, const double* scc, int nband, double* eig, double* vcc, int ldv, int* status,
+                  Prof* prof = nullptr);
 
 }  // namespace apw

@@ -76,26 +76,6 @@ __global__ void k_zero_u32(unsigned* p, int n) {
     for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0u;
 }
 
-__device__ void grid_barrier(unsigned* bar, unsigned& target, unsigned nwg, unsigned* abort_flag) {
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        target += nwg;
-        __threadfence();
-        atomicAdd(bar, 1u);
-        unsigned spins = 0;
-        while (__hip_atomic_load(bar, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
-            if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) break;
-            if (++spins > SPIN_LIMIT) {
-                atomicExch(abort_flag, 1u);
-                break;
-            }
-            __builtin_amdgcn_s_sleep(2);
-        }
-        __threadfence();
-    }
-    __syncthreads();
-}
-
 // Reflector (zlarfg convention) of x[0..mx): P^H x = beta e_1, P = I - tau v v^H, v[0] = 1.
 // x is read from xs (any memory), v written to vo (LDS, may alias xs shifted); every thread
 // of the block calls it and gets tau and beta.
@@ -123,124 +103,6 @@ __device__ void make_reflector(const double2* xs, int mx, double2* vo, double2& 
     __syncthreads();
     for (int i = threadIdx.x; i < mx; i += NT) vo[i] = (i == 0) ? make_double2(1.0, 0.0) : cmul(xs[i], sc);
     __syncthreads();
-}
-
-// Householder tridiagonalisation (lower), cooperative, one grid barrier per column.
-// Column c of the trailing matrix is owned by global wave c % nwaves for the whole
-// reduction, so its reads (p = A v) and writes (rank-2 update) never cross waves.  What
-// crosses workgroups is p (double-buffered by column parity, so a workgroup one column
-// ahead cannot overwrite what a slower one still reads) and the column that yields the
-// next reflector: its owner leaves it untouched and every workgroup recomputes its
-// rank-2-updated values itself.  LDS: v, y, c (n c128 each) + reduction scratch.
-// bar[0]: barrier counter, bar[1]: abort flag (set on a barrier timeout).
-__global__ __launch_bounds__(NT) void k_tridiag(double2* __restrict__ A, int n, double2* __restrict__ V,
-                                                double2* __restrict__ tau, double* __restrict__ d,
-                                                double* __restrict__ e, double2* __restrict__ P, unsigned* bar) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    double2* v = reinterpret_cast<double2*>(smem);
-    double2* y = v + n;
-    double2* cn = y + n;
-    double* red = reinterpret_cast<double*>(cn + n);
-    const unsigned nwg = gridDim.x;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int gwave = blockIdx.x * (NT / 64) + wave;
-    const int nwaves = nwg * (NT / 64);
-    unsigned target = 0;
-    // reflector of column 0
-    double2 t;
-    double beta;
-    make_reflector(A + 1, n - 1, v, t, beta, red);
-    if (blockIdx.x == 0) {
-        for (int i = threadIdx.x; i < n - 1; i += NT) V[1 + i] = v[i];
-        if (threadIdx.x == 0) {
-            tau[0] = t;
-            d[0] = A[0].x;
-            e[0] = beta;
-        }
-    }
-    for (int k = 0; k < n - 1; ++k) {
-        const int m = n - k - 1;           // trailing matrix rows/cols k+1 .. n-1
-        double2* Pk = P + (size_t)(k & 1) * n;
-        // ---- p = A_trail v on owned columns c = k+1+jj ----
-        {
-            int c0 = k + 1 + ((gwave - (k + 1)) % nwaves + nwaves) % nwaves;
-            for (int c = c0; c < n; c += nwaves) {
-                const double2* col = A + (size_t)c * n + (k + 1);
-                double pr = 0, pi = 0;
-                for (int i = lane; i < m; i += 64) {
-                    const double2 q = cconjmul(col[i], v[i]);   // conj(A[i][c]) v_i = A[c][i] v_i
-                    pr += q.x;
-                    pi += q.y;
-                }
-                pr = wave_sum(pr);
-                pi = wave_sum(pi);
-                if (lane == 0) Pk[c - (k + 1)] = make_double2(pr, pi);
-            }
-        }
-        grid_barrier(bar, target, nwg, bar + 1);
-        // ---- y = tau p - (|tau|^2 mu / 2) v,  mu = v^H p ----
-        double mr = 0;
-        for (int i = threadIdx.x; i < m; i += NT) {
-            const double2 p = Pk[i];
-            y[i] = p;
-            mr += cconjmul(v[i], p).x;
-        }
-        const double mu = block_sum2(mr, 0.0, red).x;
-        const double c2 = 0.5 * (t.x * t.x + t.y * t.y) * mu;
-        for (int i = threadIdx.x; i < m; i += NT) {
-            const double2 tp = cmul(t, y[i]);
-            y[i] = make_double2(tp.x - c2 * v[i].x, tp.y - c2 * v[i].y);
-        }
-        __syncthreads();
-        // ---- updated column k+1 (every workgroup): cn[i] = A[k+1+i][k+1] - y_i conj(v_0) - v_i conj(y_0)
-        {
-            const double2* col = A + (size_t)(k + 1) * n + (k + 1);
-            const double2 v0 = v[0], y0 = y[0];
-            for (int i = threadIdx.x; i < m; i += NT) {
-                const double2 a1 = cconjmul(v0, y[i]);
-                const double2 a2 = cconjmul(y0, v[i]);
-                double2 a = col[i];
-                a.x -= a1.x + a2.x;
-                a.y -= a1.y + a2.y;
-                cn[i] = a;
-            }
-        }
-        __syncthreads();
-        // ---- rank-2 update of owned columns k+2 .. n-1, rows k+2 .. n-1 ----
-        {
-            int c0 = k + 2 + ((gwave - (k + 2)) % nwaves + nwaves) % nwaves;
-            for (int c = c0; c < n; c += nwaves) {
-                double2* col = A + (size_t)c * n + (k + 1);
-                const int jj = c - (k + 1);
-                const double2 vj = v[jj], yj = y[jj];
-                for (int i = 1 + lane; i < m; i += 64) {
-                    const double2 a1 = cconjmul(vj, y[i]);
-                    const double2 a2 = cconjmul(yj, v[i]);
-                    double2 a = col[i];
-                    a.x -= a1.x + a2.x;
-                    a.y -= a1.y + a2.y;
-                    col[i] = a;
-                }
-            }
-        }
-        __threadfence_block();
-        __syncthreads();
-        // ---- next reflector from the updated column (every workgroup, redundantly) ----
-        if (m >= 2) {
-            make_reflector(cn + 1, m - 1, v, t, beta, red);
-            if (blockIdx.x == 0) {
-                for (int i = threadIdx.x; i < m - 1; i += NT) V[(size_t)(k + 1) * n + (k + 2) + i] = v[i];
-                if (threadIdx.x == 0) {
-                    tau[k + 1] = t;
-                    d[k + 1] = cn[0].x;
-                    e[k + 1] = beta;
-                }
-            }
-        } else if (blockIdx.x == 0 && threadIdx.x == 0) {
-            d[k + 1] = cn[0].x;
-        }
-        __syncthreads();
-    }
 }
 
 // Cross-workgroup exchange of the LDS-resident reduction: a ring with one slot per
@@ -278,23 +140,31 @@ __device__ inline bool ring_get(const double2* slot, double2& v) {
 // Both go through the ring: slot k * 2n + i holds p_i of step k, k * 2n + n + i the
 // published column.  abort_flag: raised when a poll exceeds its spin bound.
 // LDS: ncol x n columns, v, y, c (n each), ncol spare, reduction scratch.
-__global__ __launch_bounds__(NT) void k_tridiag_lds(const double2* __restrict__ A, int n, int ncol,
+template <bool GCOL>
+__global__ __launch_bounds__(NT) void k_tridiag_lds(double2* __restrict__ A, int n, int ncol,
                                                     double2* __restrict__ V, double2* __restrict__ tau,
                                                     double* __restrict__ d, double* __restrict__ e,
                                                     double2* __restrict__ ring, unsigned* abort_flag) {
     extern __shared__ __align__(16) unsigned char smem[];
-    double2* Al = reinterpret_cast<double2*>(smem);   // Al[lc * n + i] = A[i][c], c = blockIdx.x + lc * nwg
-    double2* v = Al + (size_t)ncol * n;
+    // own column slot lc (c = blockIdx.x + lc * nwg): LDS copy, or (GCOL) the column of A
+    // itself, which no other workgroup writes after the start
+    double2* Al = reinterpret_cast<double2*>(smem);
+    double2* v = Al + (GCOL ? 0 : (size_t)ncol * n);
     double2* y = v + n;
     double2* cn = y + n;
     double* red = reinterpret_cast<double*>(cn + n + ncol);
     __shared__ int stop;
     const unsigned nwg = gridDim.x;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    for (int lc = 0; lc < ncol; ++lc) {
-        const int c = blockIdx.x + lc * nwg;
-        if (c >= n) break;
-        for (int i = threadIdx.x; i < n; i += NT) Al[(size_t)lc * n + i] = A[(size_t)c * n + i];
+    auto colp = [&](int lc) -> double2* {
+        return GCOL ? A + (size_t)(blockIdx.x + lc * nwg) * n : Al + (size_t)lc * n;
+    };
+    if (!GCOL) {
+        for (int lc = 0; lc < ncol; ++lc) {
+            const int c = blockIdx.x + lc * nwg;
+            if (c >= n) break;
+            for (int i = threadIdx.x; i < n; i += NT) Al[(size_t)lc * n + i] = A[(size_t)c * n + i];
+        }
     }
     double2 t;
     double beta;
@@ -315,7 +185,7 @@ __global__ __launch_bounds__(NT) void k_tridiag_lds(const double2* __restrict__ 
         for (int lc = lc0 + wave; lc < ncol; lc += NT / 64) {
             const int c = blockIdx.x + lc * nwg;
             if (c >= n) break;
-            const double2* col = Al + (size_t)lc * n + (k + 1);
+            const double2* col = colp(lc) + (k + 1);
             double pr = 0, pi = 0;
             for (int i = lane; i < m; i += 64) {
                 const double2 q = cconjmul(col[i], v[i]);
@@ -327,7 +197,7 @@ __global__ __launch_bounds__(NT) void k_tridiag_lds(const double2* __restrict__ 
             if (lane == 0) ring_put(slot + (c - (k + 1)), make_double2(pr, pi));
         }
         if ((k + 1) % (int)nwg == (int)blockIdx.x) {   // owner of column k+1 publishes it
-            const double2* col = Al + (size_t)((k + 1) / nwg) * n + (k + 1);
+            const double2* col = colp((k + 1) / nwg) + (k + 1);
             for (int i = threadIdx.x; i < m; i += NT) ring_put(slot + n + i, col[i]);
         }
         // ---- gather p -> y and the published column -> cn ----
@@ -381,7 +251,7 @@ __global__ __launch_bounds__(NT) void k_tridiag_lds(const double2* __restrict__ 
             for (int lc = lc1 + wave; lc < ncol; lc += NT / 64) {
                 const int c = blockIdx.x + lc * nwg;
                 if (c >= n) break;
-                double2* col = Al + (size_t)lc * n + (k + 1);
+                double2* col = colp(lc) + (k + 1);
                 const int jj = c - (k + 1);
                 const double2 vj = v[jj], yj = y[jj];
                 for (int i = 1 + lane; i < m; i += 64) {
@@ -637,15 +507,34 @@ __global__ __launch_bounds__(NT) void k_tridiag_reg(const double2* __restrict__ 
 
 // Sturm count: number of eigenvalues of T strictly below x
 // d and e2 (squared off-diagonals) in LDS; every lane of a wave reads the same address.
+// The operands of four steps are loaded before the four dependent updates, so the chain
+// waits on the divisions only.
 __device__ int sturm(const double* d, const double* e2, int n, double x, double pivmin) {
     int c = 0;
     double q = d[0] - x;
     if (fabs(q) <= pivmin) q = -pivmin;
     if (q < 0) ++c;
-    for (int i = 1; i < n; ++i) {
+    int i = 1;
+    for (; i + 3 < n; i += 4) {
+        const double d0 = d[i], d1 = d[i + 1], d2 = d[i + 2], d3 = d[i + 3];
+        const double f0 = e2[i - 1], f1 = e2[i], f2 = e2[i + 1], f3 = e2[i + 2];
+        q = d0 - x - f0 / q;
+        if (fabs(q) <= pivmin) q = -pivmin;
+        c += q < 0;
+        q = d1 - x - f1 / q;
+        if (fabs(q) <= pivmin) q = -pivmin;
+        c += q < 0;
+        q = d2 - x - f2 / q;
+        if (fabs(q) <= pivmin) q = -pivmin;
+        c += q < 0;
+        q = d3 - x - f3 / q;
+        if (fabs(q) <= pivmin) q = -pivmin;
+        c += q < 0;
+    }
+    for (; i < n; ++i) {
         q = d[i] - x - e2[i - 1] / q;
         if (fabs(q) <= pivmin) q = -pivmin;
-        if (q < 0) ++c;
+        c += q < 0;
     }
     return c;
 }
@@ -941,10 +830,10 @@ __global__ __launch_bounds__(NT) void k_backtransform(int n, const double2* __re
 }
 
 // vcc[:, j] = Q z_j with one wave per column, no workgroup barriers: lane l holds rows
-// l + 64 r (r < RPL) of z in registers; reflector k is read straight from V into
-// registers one reflector ahead of its use, so its load overlaps the previous update.
-// Rows below k + 1 are skipped a register slot at a time (uniform across the wave).
-template <int RPL>
+// l + 64 r (r < RPL) of z in registers; reflectors are read straight from V into a ring of
+// D register sets, D - 1 reflectors ahead of their use, so each load has D - 1 reflector
+// applications to arrive.  Rows below k + 1 are skipped a register slot at a time.
+template <int RPL, int D>
 __global__ __launch_bounds__(NT) void k_backtransform_w(int n, int nband, const double2* __restrict__ V,
                                                         const double2* __restrict__ tau,
                                                         const double* __restrict__ Z, double2* __restrict__ vcc,
@@ -952,29 +841,23 @@ __global__ __launch_bounds__(NT) void k_backtransform_w(int n, int nband, const 
     const int lane = threadIdx.x & 63;
     const int j = blockIdx.x * (NT / 64) + (threadIdx.x >> 6);
     if (j >= nband) return;   // whole wave; the kernel has no workgroup barrier
-    double2 z[RPL], vc[RPL], vn[RPL];
+    double2 z[RPL], vb[D][RPL], tb[D];
 #pragma unroll
     for (int r = 0; r < RPL; ++r) {
         const int i = lane + 64 * r;
         z[r] = make_double2(i < n ? Z[(size_t)j * n + i] : 0.0, 0.0);
     }
-    // reflector k occupies rows k+1 .. n-1 of column k of V
-    auto load = [&](int k, double2* dst) {
+    // reflector k occupies rows k+1 .. n-1 of column k of V; k < 0 loads nothing useful
+    auto load = [&](int k, double2 (&dst)[RPL], double2& t) {
+        const int kk = k < 0 ? 0 : k;
 #pragma unroll
         for (int r = 0; r < RPL; ++r) {
             const int i = lane + 64 * r;
-            dst[r] = V[(size_t)k * n + (i < n ? i : 0)];
+            dst[r] = V[(size_t)kk * n + (i < n ? i : 0)];
         }
+        t = k < 0 ? make_double2(0.0, 0.0) : tau[kk];
     };
-    int k = n - 2;
-    load(k, vc);
-    double2 tc = tau[k];
-    for (; k >= 0; --k) {
-        double2 tn = make_double2(0.0, 0.0);
-        if (k > 0) {
-            load(k - 1, vn);
-            tn = tau[k - 1];
-        }
+    auto apply = [&](int k, const double2 (&vc)[RPL], double2 tc) {
         const int rmin = (k + 1) >> 6;
         double sr = 0, si = 0;
 #pragma unroll
@@ -1000,9 +883,18 @@ __global__ __launch_bounds__(NT) void k_backtransform_w(int n, int nband, const 
                 z[r].y -= q.y;
             }
         }
+    };
 #pragma unroll
-        for (int r = 0; r < RPL; ++r) vc[r] = vn[r];
-        tc = tn;
+    for (int q = 0; q < D; ++q) load(n - 2 - q, vb[q], tb[q]);
+    for (int k0 = n - 2; k0 >= 0; k0 -= D) {
+#pragma unroll
+        for (int q = 0; q < D; ++q) {
+            const int k = k0 - q;
+            if (k >= 0) {
+                apply(k, vb[q], tb[q]);
+                load(k - D, vb[q], tb[q]);
+            }
+        }
     }
 #pragma unroll
     for (int r = 0; r < RPL; ++r) {
@@ -1077,8 +969,10 @@ __global__ __launch_bounds__(1024) void k_chol_inv(int n, const double2* __restr
             }
         }
     }
-    if (bad) {
+    if (bad) {   // zero R^-1 so that work queued behind a failed factorisation stays finite
         if (threadIdx.x == 0) status[0] = bad;
+        for (size_t t = threadIdx.x; t < (size_t)n * n; t += blockDim.x)
+            rinv[t % n + (t / n) * (size_t)ldr] = make_double2(0.0, 0.0);
         return;
     }
     __syncthreads();
@@ -1149,7 +1043,6 @@ void heevx_lowest(hipStream_t s, EigWorkspace& ws, int num_cu, int n, int ldh, c
     ws.tau.reserve((size_t)2 * n);
     ws.d.reserve(n);
     ws.e.reserve(n);
-    ws.p.reserve((size_t)8 * n);   // 4n c128: p and the published column, each double-buffered
     ws.z.reserve((size_t)n * nband);
     ws.bar.reserve(64 + 256);
     double2* A = reinterpret_cast<double2*>(ws.a.get());
@@ -1159,15 +1052,12 @@ void heevx_lowest(hipStream_t s, EigWorkspace& ws, int num_cu, int n, int ldh, c
     if (n > 1) {
         hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(256), 0, s, ws.bar.get(), 64 + 256);
         APW_LAUNCH_CHECK();
-        // exchange buffers: 4n c128 (p and the published column, double-buffered); flags
-        // for up to 256 workgroups after the counter / abort words
-        APW_CHECK(ws.p.n >= (size_t)8 * n && ws.bar.n >= 64 + 256 && ws.v.n >= (size_t)2 * n * n,
-                  "heevx: workspace too small for n=%d", n);
+        // bar[1]: abort word of the exchange polls; V: n x n reflectors
+        APW_CHECK(ws.bar.n >= 2 && ws.v.n >= (size_t)2 * n * n, "heevx: workspace too small for n=%d", n);
         double2* Vp = reinterpret_cast<double2*>(ws.v.get());
         double2* taup = reinterpret_cast<double2*>(ws.tau.get());
         double* dp = ws.d.get();
         double* ep = ws.e.get();
-        double2* Pp = reinterpret_cast<double2*>(ws.p.get());
         unsigned* barp = ws.bar.get();
         // algorithmic: per column a hemv read and a rank-2 read+write of the trailing matrix
         ProfScope ps(prof, s, "eig_tridiag", 16.0 * 3.0 * n * (double)n * n / 3.0, 16.0 / 3.0 * n * (double)n * n);
@@ -1218,42 +1108,41 @@ void heevx_lowest(hipStream_t s, EigWorkspace& ws, int num_cu, int n, int ldh, c
             int ncol_r = nc;
             void* args[] = {&Ac, &n, &ncol_r, &Vp, &taup, &dp, &ep, &ring, &abortp};
             APW_HIP(hipLaunchCooperativeKernel(fn, dim3(w), dim3(NT), args, (unsigned)lds, s));
-        } else if (nwg > 0) {
-            const size_t lds = fixed + (size_t)ncol * (n + 1) * sizeof(double2);
+        } else {
+            // columns in LDS when the share fits, else in A itself (one column per
+            // workgroup slot, L2/Infinity-Cache resident); the ring as above
+            const bool gcol = nwg <= 0;
+            if (gcol) {
+                nwg = std::min(num_cu, 256);
+                ncol = (n + nwg - 1) / nwg;
+            }
+            const size_t lds = gcol ? fixed + (size_t)ncol * sizeof(double2)
+                                    : fixed + (size_t)ncol * (n + 1) * sizeof(double2);
             const size_t nring = (size_t)2 * n * (n - 1);
             ws.ring.reserve(2 * nring);
             double2* ring = reinterpret_cast<double2*>(ws.ring.get());
             hipLaunchKernelGGL(k_fill_ring, dim3((unsigned)std::min<size_t>(1024, (nring + 255) / 256)), dim3(256), 0, s,
                                ring, nring);
             APW_LAUNCH_CHECK();
-            APW_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_tridiag_lds),
-                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-            const double2* Ac = A;
+            const void* fn = gcol ? reinterpret_cast<const void*>(k_tridiag_lds<true>)
+                                  : reinterpret_cast<const void*>(k_tridiag_lds<false>);
+            APW_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
             unsigned* abortp = barp + 1;
-            void* args[] = {&Ac, &n, &ncol, &Vp, &taup, &dp, &ep, &ring, &abortp};
-            APW_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_tridiag_lds), dim3(nwg), dim3(NT), args,
-                                               (unsigned)lds, s));
-        } else {
-            const size_t lds = fixed;
-            APW_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_tridiag), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                        (int)lds));
-            int occ = 0;
-            APW_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(k_tridiag), NT, lds));
-            nwg = std::max(1, std::min(n / 16, std::min(num_cu, 64)));
-            nwg = std::min(nwg, std::max(1, occ) * num_cu);
-            void* args[] = {&A, &n, &Vp, &taup, &dp, &ep, &Pp, &barp};
-            APW_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_tridiag), dim3(nwg), dim3(NT), args,
-                                               (unsigned)lds, s));
+            void* args[] = {&A, &n, &ncol, &Vp, &taup, &dp, &ep, &ring, &abortp};
+            APW_HIP(hipLaunchCooperativeKernel(fn, dim3(nwg), dim3(NT), args, (unsigned)lds, s));
         }
     } else {
         // n == 1: T = [Re a00]; the real part is the first double of A
         APW_HIP(hipMemcpyAsync(ws.d.get(), A, sizeof(double), hipMemcpyDeviceToDevice, s));
     }
     {
-    ProfScope ps(prof, s, "eig_bisect_invit", 0.0, 0.0);
+    ProfScope ps(prof, s, "eig_bisect", 0.0, 0.0);
     hipLaunchKernelGGL(k_bisect, dim3((nband + 3) / 4), dim3(256), (size_t)2 * n * sizeof(double), s, ws.d.get(),
                        ws.e.get(), n, nband, eig);
     APW_LAUNCH_CHECK();
+    }
+    {
+    ProfScope ps(prof, s, "eig_invit", 0.0, 0.0);
     const size_t lds_inv = (size_t)8 * n * sizeof(double) + 64 * sizeof(double) + (size_t)n * sizeof(int);
     APW_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_invit), hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)lds_inv));
@@ -1268,17 +1157,21 @@ void heevx_lowest(hipStream_t s, EigWorkspace& ws, int num_cu, int n, int ldh, c
     double2* out = reinterpret_cast<double2*>(vcc);
     const dim3 gw((nband + NT / 64 - 1) / (NT / 64));
     if (n < 2) {
-        hipLaunchKernelGGL(k_backtransform_w<1>, gw, dim3(NT), 0, s, n, nband, Vc, tc, ws.z.get(), out, ldv);
+        hipLaunchKernelGGL((k_backtransform_w<1, 4>), gw, dim3(NT), 0, s, n, nband, Vc, tc, ws.z.get(), out, ldv);
     } else if (n <= 128) {
-        hipLaunchKernelGGL(k_backtransform_w<2>, gw, dim3(NT), 0, s, n, nband, Vc, tc, ws.z.get(), out, ldv);
+        hipLaunchKernelGGL((k_backtransform_w<2, 4>), gw, dim3(NT), 0, s, n, nband, Vc, tc, ws.z.get(), out, ldv);
     } else if (n <= 256) {
-        hipLaunchKernelGGL(k_backtransform_w<4>, gw, dim3(NT), 0, s, n, nband, Vc, tc, ws.z.get(), out, ldv);
+        hipLaunchKernelGGL((k_backtransform_w<4, 4>), gw, dim3(NT), 0, s, n, nband, Vc, tc, ws.z.get(), out, ldv);
+    } else if (n <= 384) {
+        hipLaunchKernelGGL((k_backtransform_w<6, 4>), gw, dim3(NT), 0, s, n, nband, Vc, tc, ws.z.get(), out, ldv);
     } else if (n <= 512) {
-        hipLaunchKernelGGL(k_backtransform_w<8>, gw, dim3(NT), 0, s, n, nband, Vc, tc, ws.z.get(), out, ldv);
+        hipLaunchKernelGGL((k_backtransform_w<8, 4>), gw, dim3(NT), 0, s, n, nband, Vc, tc, ws.z.get(), out, ldv);
+    } else if (n <= 640) {
+        hipLaunchKernelGGL((k_backtransform_w<10, 4>), gw, dim3(NT), 0, s, n, nband, Vc, tc, ws.z.get(), out, ldv);
     } else if (n <= 768) {
-        hipLaunchKernelGGL(k_backtransform_w<12>, gw, dim3(NT), 0, s, n, nband, Vc, tc, ws.z.get(), out, ldv);
+        hipLaunchKernelGGL((k_backtransform_w<12, 3>), gw, dim3(NT), 0, s, n, nband, Vc, tc, ws.z.get(), out, ldv);
     } else if (n <= 1024) {
-        hipLaunchKernelGGL(k_backtransform_w<16>, gw, dim3(NT), 0, s, n, nband, Vc, tc, ws.z.get(), out, ldv);
+        hipLaunchKernelGGL((k_backtransform_w<16, 2>), gw, dim3(NT), 0, s, n, nband, Vc, tc, ws.z.get(), out, ldv);
     } else {
         const size_t lds_bt = (size_t)n * sizeof(double2) + 64 * sizeof(double);
         APW_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_backtransform),

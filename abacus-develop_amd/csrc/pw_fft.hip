@@ -425,6 +425,63 @@ __global__ __launch_bounds__(NT) void k_xpass(cplx<R>* __restrict__ X, int nxr, 
     }
 }
 
+// ---- density pass: x-backward of every band of the chunk, rho += w_b |psi_b(r)|^2 ------
+// grid (ny, ceil(nz/L)); each workgroup owns lines (iy, iz0 .. iz0+L) for all nb bands and
+// accumulates in LDS in band order (deterministic, no atomics), then adds to rho once.
+// LDS 2 * nx * (L+1) + nx complex + nx * L doubles
+template <int NF>
+__global__ __launch_bounds__(NT) void k_xrho(const cplx<double>* __restrict__ X, int nxr, int ny, int nz, int nb,
+                                             const int* __restrict__ slot_ix, const double* __restrict__ w,
+                                             double* __restrict__ rho, int add, FftPlan px,
+                                             const cplx<double>* __restrict__ tw, int Lrt) {
+    using C = cplx<double>;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int nx = APW_LEN(NF, px);
+    const int L = APW_LINES(NF, Lrt);
+    const int ldl = L + 1;
+    C* x = reinterpret_cast<C*>(smem);
+    C* y = x + nx * ldl;
+    C* tws = y + nx * ldl;
+    double* acc = reinterpret_cast<double*>(tws + nx);
+    const int iy = blockIdx.x;
+    const int iz0 = blockIdx.y * L;
+    load_tw<double>(tws, tw, nx);
+    for (int t = threadIdx.x; t < nx * L; t += NT) acc[t] = 0.0;
+    for (int b = 0; b < nb; ++b) {
+        const C* Xb = X + (size_t)b * nxr * ny * nz;
+        __syncthreads();
+        for (int t = threadIdx.x; t < nx * ldl; t += NT) {
+            C z;
+            z.x = 0;
+            z.y = 0;
+            x[t] = z;
+        }
+        __syncthreads();
+        for (int t = threadIdx.x; t < nxr * L; t += NT) {
+            const int q = t / L, dz = t - (t / L) * L;
+            const int iz = iz0 + dz;
+            if (iz < nz) x[slot_ix[q] * ldl + dz] = Xb[((size_t)q * ny + iy) * nz + iz];
+        }
+        __syncthreads();
+        const C* res = fft_lines<double, NF, +1>(x, y, tws, px, L, ldl);
+        const double wb = w[b];
+        for (int t = threadIdx.x; t < nx * L; t += NT) {
+            const int ix = t / L, dz = t - (t / L) * L;
+            const C v = res[ix * ldl + dz];
+            acc[t] += wb * (v.x * v.x + v.y * v.y);
+        }
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < nx * L; t += NT) {
+        const int ix = t / L, dz = t - (t / L) * L;
+        const int iz = iz0 + dz;
+        if (iz < nz) {
+            double* rp = rho + ((size_t)ix * ny + iy) * nz + iz;
+            *rp = add ? *rp + acc[t] : acc[t];
+        }
+    }
+}
+
 // ---- pass 4: y-forward, keeping the stick rows -----------------------------------------
 template <class R, int NF>
 __global__ __launch_bounds__(NT) void k_yfwd(const cplx<R>* __restrict__ X, int ns, int nz, int nxr,
@@ -725,6 +782,28 @@ void box_to_sphere(hipStream_t s, const PwGrid& g, const double* box, double* hp
     APW_LAUNCH_CHECK();
     pass_forward_half<double>(s, g, S, X, nb, reinterpret_cast<double2*>(hpsi), ldh, nullptr, 0, nullptr, 0.0, mode,
                               scale, nullptr);
+}
+
+void psi_to_rho(hipStream_t s, const PwGrid& g, const double* psi, int ld, int nb, const double* w, double* rho,
+                bool add, double* S, double* X, Prof* prof) {
+    using C = cplx<double>;
+    if (nb <= 0) return;
+    pass_backward_half<double>(s, g, reinterpret_cast<const double2*>(psi), ld, S, X, nb, prof);
+    const int Lx = lines_for(g.nx);
+    const double nX = (double)g.nxr * g.ny * g.nz, nxyz = (double)g.nx * g.ny * g.nz;
+    // algorithmic: the x slots of every band in, rho read (when adding) and written once
+    ProfScope ps(prof, s, "fft_x_rho", nb * sizeof(C) * nX + (add ? 16.0 : 8.0) * nxyz,
+                 nb * (fft_flops(g.nx) * g.ny * g.nz + 3.0 * nxyz));
+    const size_t lds = lds_bytes<double>(g.nx, Lx) + (size_t)g.nx * Lx * sizeof(double);
+#define APW_DO_LAUNCH(NF)                                                                                       \
+    APW_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_xrho<NF>),                                     \
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));                         \
+    hipLaunchKernelGGL((k_xrho<NF>), dim3(g.ny, (g.nz + Lx - 1) / Lx), dim3(NT), lds, s,                       \
+                       reinterpret_cast<const C*>(X), g.nxr, g.ny, g.nz, nb, g.slot_ix.get(), w, rho, add ? 1 : 0, \
+                       g.px, tw_ptr<double>(g, g.twoff_x), Lx)
+    APW_DISPATCH(g.nx)
+#undef APW_DO_LAUNCH
+    APW_LAUNCH_CHECK();
 }
 
 }  // namespace apw

@@ -59,4 +59,9 @@ void sphere_to_box(hipStream_t s, const PwGrid& g, const double* psi, int ld, do
 void box_to_sphere(hipStream_t s, const PwGrid& g, const double* box, double* hpsi, int ldh, int nb,
                    GatherMode mode, double scale, double* S, double* X);
 
+// rho(r) (+)= sum_b w[b] |FFT^+(psi_b)(r)|^2 over nb bands (w: device doubles), bands in
+// order, no atomics.  S, X hold nb bands of intermediates.
+void psi_to_rho(hipStream_t s, const PwGrid& g, const double* psi, int ld, int nb, const double* w, double* rho,
+                bool add, double* S, double* X, Prof* prof = nullptr);
+
 }  // namespace apw

@@ -37,6 +37,39 @@ __global__ void k_ekinetic(int npw, int max_npw, double tpiba2, const double* __
     }
 }
 
+// elecstate_pw_op (nspin 1/2): rho[ir] += w1 |wfcr[ir]|^2
+__global__ void k_elecstate(int nrxx, double w1, double* __restrict__ rho, const double2* __restrict__ wfcr) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nrxx; i += gridDim.x * blockDim.x) {
+        const double2 a = wfcr[i];
+        rho[i] += w1 * (a.x * a.x + a.y * a.y);
+    }
+}
+
+// non-collinear form: charge from both spinor components; mode 2 = DOMAG (all of m),
+// 1 = DOMAG_Z (m_z accumulated, m_x = m_y = 0), 0 = m set to 0
+__global__ void k_elecstate_nc(int nrxx, int mode, double w1, double* __restrict__ r0, double* __restrict__ r1,
+                               double* __restrict__ r2, double* __restrict__ r3, const double2* __restrict__ a,
+                               const double2* __restrict__ b) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nrxx; i += gridDim.x * blockDim.x) {
+        const double2 u = a[i], v = b[i];
+        const double nu = u.x * u.x + u.y * u.y, nv = v.x * v.x + v.y * v.y;
+        r0[i] += w1 * (nu + nv);
+        if (mode == 2) {
+            r1[i] += w1 * 2.0 * (u.x * v.x + u.y * v.y);
+            r2[i] += w1 * 2.0 * (u.x * v.y - v.x * u.y);
+            r3[i] += w1 * (nu - nv);
+        } else if (mode == 1) {
+            r1[i] = 0.0;
+            r2[i] = 0.0;
+            r3[i] += w1 * (nu - nv);
+        } else {
+            r1[i] = 0.0;
+            r2[i] = 0.0;
+            r3[i] = 0.0;
+        }
+    }
+}
+
 __global__ void k_veff(int size, double2* __restrict__ out, const double* __restrict__ in) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < size; i += gridDim.x * blockDim.x) {
         double2 o = out[i];
@@ -345,6 +378,23 @@ int apw_ham_set_precision(apw_ham* ham, int precision) {
     });
 }
 
+int apw_ham_set_vkb(apw_ham* ham, const double* vkb, int ldvkb) {
+    return guarded([&] {
+        APW_CHECK(ham && vkb, "apw_ham_set_vkb: null argument");
+        apw::Ham& h = ham->h;
+        APW_CHECK(h.nkb > 0, "apw_ham_set_vkb: the Hamiltonian has no projectors");
+        APW_CHECK(ldvkb >= h.npw, "apw_ham_set_vkb: ldvkb %d < npw %d", ldvkb, h.npw);
+        apw::DeviceGuard g(h.ctx->device);
+        // rows of npw plane waves into the npwx-strided buffer, then the FP32 shadow
+        APW_HIP(hipMemcpy2DAsync(h.vkb.get(), (size_t)2 * h.npwx * sizeof(double), vkb, (size_t)2 * ldvkb * sizeof(double),
+                                 (size_t)2 * h.npw * sizeof(double), h.nkb, hipMemcpyDeviceToDevice, h.ctx->stream));
+        const size_t n = (size_t)2 * h.npwx * h.nkb;
+        hipLaunchKernelGGL(apw::k_d2f, dim3(apw::grid_for(n)), dim3(256), 0, h.ctx->stream, n, h.vkb.get(), h.vkb32.get());
+        APW_LAUNCH_CHECK();
+        return APW_OK;
+    });
+}
+
 int apw_hpsi(apw_ham* ham, const double* psi, int ldpsi, double* hpsi, int ldhpsi, int nbands) {
     return guarded([&] {
         APW_CHECK(ham && psi && hpsi, "apw_hpsi: null argument");
@@ -434,6 +484,67 @@ int apw_veff_pw_op(apw_ctx* ctx, int size, double* out, const double* in) {
         hipLaunchKernelGGL(apw::k_veff, dim3(apw::grid_for(size)), dim3(256), 0, ctx->stream, size,
                            reinterpret_cast<double2*>(out), in);
         APW_LAUNCH_CHECK();
+        return APW_OK;
+    });
+}
+
+int apw_elecstate_pw_op(apw_ctx* ctx, int spin, int nrxx, double w1, double* const* rho, const double* wfcr) {
+    return guarded([&] {
+        APW_CHECK(ctx && rho && wfcr && nrxx >= 0 && spin >= 0, "apw_elecstate_pw_op: bad argument");
+        APW_CHECK(rho[spin], "apw_elecstate_pw_op: null rho[spin]");
+        if (nrxx == 0) return APW_OK;
+        apw::DeviceGuard g(ctx->device);
+        hipLaunchKernelGGL(apw::k_elecstate, dim3(apw::grid_for(nrxx)), dim3(256), 0, ctx->stream, nrxx, w1,
+                           rho[spin], reinterpret_cast<const double2*>(wfcr));
+        APW_LAUNCH_CHECK();
+        return APW_OK;
+    });
+}
+
+int apw_elecstate_pw_op_nc(apw_ctx* ctx, int domag, int domag_z, int nrxx, double w1, double* const* rho,
+                           const double* wfcr, const double* wfcr_another_spin) {
+    return guarded([&] {
+        APW_CHECK(ctx && rho && wfcr && wfcr_another_spin && nrxx >= 0, "apw_elecstate_pw_op_nc: bad argument");
+        APW_CHECK(rho[0] && rho[1] && rho[2] && rho[3], "apw_elecstate_pw_op_nc: null rho component");
+        if (nrxx == 0) return APW_OK;
+        apw::DeviceGuard g(ctx->device);
+        const int mode = domag ? 2 : (domag_z ? 1 : 0);
+        hipLaunchKernelGGL(apw::k_elecstate_nc, dim3(apw::grid_for(nrxx)), dim3(256), 0, ctx->stream, nrxx, mode, w1,
+                           rho[0], rho[1], rho[2], rho[3], reinterpret_cast<const double2*>(wfcr),
+                           reinterpret_cast<const double2*>(wfcr_another_spin));
+        APW_LAUNCH_CHECK();
+        return APW_OK;
+    });
+}
+
+int apw_psi_to_rho(apw_ham* ham, const double* psi, int ldpsi, int nbands, const double* wg, double omega,
+                   double* rho, int add) {
+    return guarded([&] {
+        APW_CHECK(ham && rho && (nbands == 0 || (psi && wg)), "apw_psi_to_rho: null argument");
+        APW_CHECK(nbands >= 0 && omega > 0.0, "apw_psi_to_rho: nbands=%d omega=%g", nbands, omega);
+        apw::Ham& h = ham->h;
+        APW_CHECK(ldpsi >= h.npw, "apw_psi_to_rho: ldpsi=%d < npw=%d", ldpsi, h.npw);
+        apw::DeviceGuard g(h.ctx->device);
+        hipStream_t s = h.ctx->stream;
+        const size_t nxyz = (size_t)h.grid.nx * h.grid.ny * h.grid.nz;
+        if (nbands == 0) {
+            if (!add) APW_HIP(hipMemsetAsync(rho, 0, nxyz * sizeof(double), s));
+            APW_HIP(hipStreamSynchronize(s));
+            return APW_OK;
+        }
+        // weights wg_b / omega (elecstate_pw.cpp:200), uploaded once
+        std::vector<double> w(nbands);
+        for (int b = 0; b < nbands; ++b) w[b] = wg[b] / omega;
+        h.wts.reserve(nbands);
+        APW_HIP(hipMemcpyAsync(h.wts.get(), w.data(), nbands * sizeof(double), hipMemcpyHostToDevice, s));
+        for (int b0 = 0; b0 < nbands; b0 += h.chunk) {
+            const int n = std::min(h.chunk, nbands - b0);
+            h.S.reserve(2 * h.grid.bytes_S(1) * n);
+            h.X.reserve(2 * h.grid.bytes_X(1) * n);
+            apw::psi_to_rho(s, h.grid, psi + (size_t)2 * b0 * ldpsi, ldpsi, n, h.wts.get() + b0, rho,
+                            add != 0 || b0 > 0, h.S.get(), h.X.get(), &h.ctx->prof);
+        }
+        APW_HIP(hipStreamSynchronize(s));
         return APW_OK;
     });
 }

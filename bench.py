@@ -123,6 +123,93 @@ def cpu_baseline(workload, s, fp64_hpsi_bands, fp64_iters):
                       f"{fp64_hpsi_bands} bands (H|psi> only; subspace work not counted)"}
 
 
+def measure_next(pw, ctx, ham, s, psi0_dev, eig_david, reps=5):
+    """The components on either side of the Davidson path, on the same workload: projector
+    generation (getvnl, HBM-write bound: nkb x npw c128), the charge density of the nband
+    start vectors (psiToRho, five FFT passes per band), the Diago_DavSubspace solve from the
+    same start vectors, and one write + read of the bands as a .dat wavefunction file."""
+    import tempfile
+    import numpy as np
+    import torch
+    from apw import synth
+    out = {}
+    # ---- getvnl
+    t = synth.projector_tables(s)
+    gk = torch.from_numpy(np.ascontiguousarray(t["gk"])).cuda()
+    tab = torch.from_numpy(t["tab"]).cuda()
+    vkb = torch.empty((s.nkb, s.npwx), dtype=torch.complex128, device="cuda")
+    args = (ctx, t["ntype"], t["npw"], t["npwx"], t["nhm"], t["tab_2"], t["tab_3"], t["atom_na"], t["atom_nb"],
+            t["atom_nh"], t["dq"], t["tpiba"], gk, t["tau"], t["indv"], t["nhtol"], t["nhtolm"], tab, vkb)
+    pw.getvnl(*args)
+    ctx.synchronize()
+    ctx.set_profiling(True)
+    for _ in range(reps):
+        pw.getvnl(*args)
+    ctx.synchronize()
+    e = ctx.profile()["getvnl"]
+    ctx.set_profiling(False)
+    avg = e["ms"] * 1e-3 / e["launches"]
+    gbs = e["bytes"] / e["launches"] / avg / 1e9
+    sign = np.tile(t["sign"], s.nat)[:, None]
+    err = float(np.abs(vkb[:8].cpu().numpy() * sign[:8] - s.vkb[:8]).max() / np.abs(s.vkb).max())
+    out["getvnl"] = {"avg_launch_us": round(avg * 1e6, 2), "bytes_per_launch": e["bytes"] / e["launches"],
+                     "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                  "frac": round(gbs / HBM_PEAK_GBS, 4)},
+                     "rel_err_vs_analytic_projectors": err}
+    # ---- psiToRho of the nband start vectors (weights 2 / nband each)
+    wg = np.full(s.nband, 2.0 / s.nband)
+    rho = torch.zeros(s.nxyz, dtype=torch.float64, device="cuda")
+    ham.psi_to_rho(psi0_dev, s.nband, wg, s.lat.omega, rho=rho, ldpsi=s.npwx)
+    ctx.synchronize()
+    ctx.set_profiling(True)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ham.psi_to_rho(psi0_dev, s.nband, wg, s.lat.omega, rho=rho, ldpsi=s.npwx)
+    ctx.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    prof = ctx.profile()
+    ctx.set_profiling(False)
+    out["psi_to_rho"] = {"bands": s.nband, "time_ms": round(dt * 1e3, 3),
+                         "bands_per_s": round(s.nband / dt, 1),
+                         "kernels": {k: {"avg_us": round(1e3 * v["ms"] / max(1, v["launches"]), 2),
+                                         "GB_s": round(v["bytes"] / max(v["ms"], 1e-9) / 1e6, 1)}
+                                     for k, v in prof.items()}}
+    # ---- Diago_DavSubspace from the same start vectors
+    # HSolverPW::update_precondition for dav_subspace: |k+G|^2 tpiba2 (hsolver_pw.cpp:570-577)
+    pre = np.ones(s.npwx)
+    pre[: s.npw] = np.asarray(s.gk2[: s.npw]) * s.tpiba2
+    ds = pw.DiagoDavSubspace(ctx, pre, s.nband, s.npw, s.david_ndim, 1e-9, 500)
+    psi = psi0_dev.clone()
+    eig = np.zeros(s.nband)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    it = ds.diag(ham, psi, s.npwx, eig)
+    ctx.synchronize()
+    dt = time.perf_counter() - t0
+    st = dict(ds.stats)
+    ds.close()
+    out["dav_subspace"] = {"time_s": round(dt, 4), "iterations": it, "hpsi_bands": st["hpsi_bands"],
+                           "max_abs_eig_diff_vs_david_Ry": float(np.abs(eig - eig_david).max())}
+    # ---- wavefunction file round trip
+    kc = s.lat.gcar(s.basis.kvec_d[s.ik])
+    with tempfile.TemporaryDirectory() as td:
+        fn = os.path.join(td, "WAVEFUNC1.dat")
+        t0 = time.perf_counter()
+        pw.write_wfc_pw(ctx, fn, s.ik, 1, kc, 1.0, s.nband, s.meta["ecutwfc"], s.lat.lat0, s.lat.tpiba, s.lat.G,
+                        s.nx, s.ny, s.nz, s.npw, s.ig2ixyz, psi0_dev, s.npwx)
+        tw = time.perf_counter() - t0
+        back = torch.empty_like(psi0_dev)
+        t0 = time.perf_counter()
+        pw.read_wfc_pw(ctx, fn, s.ik, 1, s.nband, kc, s.lat.lat0, s.lat.tpiba, s.lat.G, s.nx, s.ny, s.nz, s.npw,
+                       s.ig2ixyz, back, s.npwx)
+        tr = time.perf_counter() - t0
+        same = bool(torch.equal(back[:, : s.npw], psi0_dev[:, : s.npw]))
+    mb = 16.0 * s.npw * s.nband / 1e6
+    out["wfc_io"] = {"MB": round(mb, 1), "write_MB_s": round(mb / tw, 1), "read_MB_s": round(mb / tr, 1),
+                     "round_trip_identical": same, "filesystem": "tempfile default (host)"}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -130,6 +217,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="si64")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-next", action="store_true", help="skip the getvnl / psiToRho / DavSubspace / wfc-file lines")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -214,6 +302,9 @@ def main():
                         "GB_s": round(v["bytes"] / max(v["ms"], 1e-9) / 1e6, 1),
                         "TFLOP_s": round(v["flops"] / max(v["ms"], 1e-9) / 1e9, 2)} for k, v in top},
     }
+    if not args.no_next:
+        out["next_components"] = measure_next(pw, ctx, ham, s, psi0, eig64)
+        print(f"[bench] next components: {json.dumps(out['next_components'])}", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.workload, s, st64["hpsi_bands"], it64)
     print(json.dumps(out), flush=True)

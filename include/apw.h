@@ -97,6 +97,12 @@ int apw_ham_destroy(apw_ham* ham);
 /* Switch the H|psi> arithmetic (APW_FP64 / APW_FP32_MIXED); psi and hpsi stay c128. */
 int apw_ham_set_precision(apw_ham* ham, int precision);
 
+/* Replace the projectors with device-resident ones (e.g. from apw_getvnl): vkb device c128,
+ * nkb rows of ldvkb (>= npw), the first npw entries of each row are used.  This is where
+ * pseudopot_cell_vnl::getvnl(ctx, ik, vkb) (VNL_in_pw.cpp:412-524) hands ppcell->vkb to the
+ * nonlocal operator (nonlocal_pw.cpp:58-64). */
+int apw_ham_set_vkb(apw_ham* ham, const double* vkb, int ldvkb);
+
 /* Operator::hPsi (operator.cpp:39-88): hpsi[:, b] = H psi[:, b] for b < nbands
  * (overwrite; the reference zeroes then accumulates every act()).  Device c128. */
 int apw_hpsi(apw_ham* ham, const double* psi, int ldpsi, double* hpsi, int ldhpsi, int nbands);
@@ -113,6 +119,73 @@ int apw_act_nonlocal(apw_ham* ham, int nbands, int nbasis, const double* psi, do
  *                box' = FFT^- (in).   in/out device c128 (box: nx*ny*nz). */
 int apw_recip_to_real(apw_ham* ham, const double* in, double* out, int add, double factor);
 int apw_real_to_recip(apw_ham* ham, const double* in, double* out, int add, double factor);
+
+/* ------------------------------------------------------------------------------
+ * Charge density from the bands (ElecStatePW::psiToRho / rhoBandK, nspin 1,
+ * elecstate_pw.cpp:142-214): rho(r) = sum_b wg[b] / omega * |psi_b(r)|^2, psi_b(r) the
+ * unnormalised backward FFT of band b scattered into the box (recip_to_real).
+ * psi: device c128, band b at psi + b*ldpsi (ldpsi >= npw); wg: HOST double[nbands];
+ * rho: device double[nx*ny*nz], overwritten (add = 0) or accumulated (add != 0).
+ * Bands are summed in order (deterministic).  Synchronous. */
+int apw_psi_to_rho(apw_ham* ham, const double* psi, int ldpsi, int nbands, const double* wg,
+                   double omega, double* rho, int add);
+
+/* ------------------------------------------------------------------------------
+ * Projector generation for one k-point (pseudopot_cell_vnl::getvnl, GPU branch,
+ * VNL_in_pw.cpp:412-524: Ylm_Real, get_sk, cal_vnl_op).
+ * ------------------------------------------------------------------------------ */
+/* cal_ylm_real_op<double, DEVICE_GPU> (module_base/kernels/math_op.h:12-39): real
+ * spherical harmonics of ng points g (device, ng x 3) into ylm ((lmax+1)^2 x ng), the
+ * Legendre values into p ((lmax+1)^2 x ng, may be NULL).  lmax <= 6. */
+int apw_cal_ylm_real_op(apw_ctx* ctx, int ng, int lmax, double sqrt2, double pi, double pi_half,
+                        double four_pi, double sqrt_inverse_four_pi, const double* g, double* p,
+                        double* ylm);
+/* cal_sk_op<double, DEVICE_GPU> (kernels/wf_op.h:10-58): structure factor sk (nat x npw
+ * c128) from the eigts tables.  All arrays device. */
+int apw_cal_sk_op(apw_ctx* ctx, int ik, int ntype, int nx, int ny, int nz, int rho_nx, int rho_ny,
+                  int rho_nz, int npw, int npwx, int fftny, int eigts1_nc, int eigts2_nc,
+                  int eigts3_nc, const int* atom_na, const int* igl2isz, const int* is2fftixy,
+                  double two_pi, const double* kvec_c, const double* atom_tau, const double* eigts1,
+                  const double* eigts2, const double* eigts3, double* sk);
+/* cal_vnl_op<double, DEVICE_GPU> (kernels/vnl_op.h:10-58): vkb_in (nkb x npwx c128) from
+ * gk (npw x 3), ylm, the radial table tab (ntype x tab_2 x tab_3) and sk; vkb1 (nhm x npw)
+ * is scratch, as in the reference.  NEG_IMAG_UNIT passed as (re, im).  All arrays device.
+ * Fails (APW_EINVAL) when |k+G| tpiba / dq reaches past the table. */
+int apw_cal_vnl_op(apw_ctx* ctx, int ntype, int npw, int npwx, int nhm, int tab_2, int tab_3,
+                   const int* atom_na, const int* atom_nb, const int* atom_nh, double dq,
+                   double tpiba, double neg_imag_re, double neg_imag_im, const double* gk,
+                   const double* ylm, const double* indv, const double* nhtol,
+                   const double* nhtolm, const double* tab, double* vkb1, const double* sk,
+                   double* vkb_in);
+/* getvnl(ctx, ik, vkb_in) in one pass: the three functors above fused, the structure
+ * factor as exp(-2 pi i (k+G).tau).  Sizes and atom_na/nb/nh, indv/nhtol/nhtolm (ntype x
+ * nhm) and tau (nat x 3, cartesian, lat0 units) are HOST arrays; gk (k+G cartesian, 2 pi /
+ * lat0 units, npw x 3), tab and vkb are device.  ntype <= 16, nhm <= 32, l <= 3. */
+int apw_getvnl(apw_ctx* ctx, int ntype, int npw, int npwx, int nhm, int tab_2, int tab_3,
+               const int* atom_na, const int* atom_nb, const int* atom_nh, double dq, double tpiba,
+               const double* gk, const double* tau, const double* indv, const double* nhtol,
+               const double* nhtolm, const double* tab, double* vkb);
+
+/* ------------------------------------------------------------------------------
+ * Plane-wave wavefunction files of one k-point, npol 1 (the reference's file layouts).
+ * ------------------------------------------------------------------------------ */
+/* ModuleIO::write_wfc_pw (module_io/write_wfc_pw.cpp:13-283): filename ending ".dat"
+ * (binary records, out_wfc_pw 2) or ".txt" (text, out_wfc_pw 1), created or truncated.
+ * ik_global: 0-based k index among nkstot; kvec_c[3] (2 pi / lat0), weight, ecutwfc, lat0,
+ * tpiba and G (3 x 3 reciprocal vectors, row-major, 2 pi / lat0) go into the header;
+ * ig2ixyz HOST int[npw] (iz + iy*nz + ix*ny*nz of each plane wave, basis order);
+ * psi DEVICE c128, band b at psi + b*ldpsi. */
+int apw_write_wfc_pw(apw_ctx* ctx, const char* filename, int ik_global, int nkstot,
+                     const double* kvec_c, double weight, int nbands, double ecutwfc, double lat0,
+                     double tpiba, const double* G, int nx, int ny, int nz, int npw,
+                     const int* ig2ixyz, const double* psi, int ldpsi);
+/* ModuleIO::read_wfc_pw (module_io/read_wfc_pw.cpp:12-341): ".dat" files only (the
+ * reference reads nothing from ".txt").  The header must match ik + 1, nkstot, npw, nbands,
+ * kvec_c, lat0, tpiba and G exactly (the reference's checks); the file's plane waves are
+ * mapped onto this basis through their Miller indices.  wfc DEVICE c128 (ld ldwfc) out. */
+int apw_read_wfc_pw(apw_ctx* ctx, const char* filename, int ik, int nkstot, int nbands,
+                    const double* kvec_c, double lat0, double tpiba, const double* G, int nx,
+                    int ny, int nz, int npw, const int* ig2ixyz, double* wfc, int ldwfc);
 
 /* ------------------------------------------------------------------------------
  * Block Davidson (DiagoDavid, diago_david.cpp).
@@ -152,6 +225,35 @@ int apw_david_diag(apw_david* dav, apw_ham* ham, apw_hpsi_cb hpsi_cb, apw_spsi_c
                    int david_maxiter, int ntry_max, int notconv_max, int precision,
                    apw_diag_stats* stats);
 
+/* dngvx_op<complex<double>, DEVICE_GPU> (dngvd_op.h:67-81): lowest m eigenpairs of the
+ * Hermitian-definite pencil A x = lambda B x (nstart x nstart, ld ldh, both triangles;
+ * the CPU op calls zhegvx).  eigenvalue device double[m]; vcc device c128 (ld ldh),
+ * B-orthonormal.  dngvd_op (dngvd_op.h:24-45, zhegvd): all nstart pairs. */
+int apw_dngvx_op(apw_ctx* ctx, int nstart, int ldh, const double* hcc, const double* scc,
+                 int m, double* eigenvalue, double* vcc);
+int apw_dngvd_op(apw_ctx* ctx, int nstart, int ldh, const double* hcc, const double* scc,
+                 double* eigenvalue, double* vcc);
+
+/* ------------------------------------------------------------------------------
+ * Davidson with pencil Rayleigh-Ritz (Diago_DavSubspace, diago_dav_subspace.cpp; the
+ * "dav_subspace" method of HSolverPW, hsolver_pw.cpp:434-471).
+ *   create: precondition HOST double[dim] (HSolverPW passes |k+G|^2 tpiba2 for this
+ *   method, hsolver_pw.cpp:570-577), nband, dim, david_ndim (> 1, david_ndim*nband < dim),
+ *   diag_thr (eigenvalue-change threshold), diag_nmax (iterations per diag_once),
+ *   need_subspace (Rayleigh-Ritz of the start vectors first).
+ *   diag: as apw_david_diag (FP64 only); is_occupied HOST int[nband] or NULL (all
+ *   occupied; unoccupied bands use max(5 thr, 1e-5)); scf selects test_exit_cond's rule.
+ * Returns the total iteration count or a negative status.
+ * ------------------------------------------------------------------------------ */
+typedef struct apw_dav_subspace apw_dav_subspace;
+int apw_dav_subspace_create(apw_ctx* ctx, const double* precondition, int nband, int dim,
+                            int david_ndim, double diag_thr, int diag_nmax, int need_subspace,
+                            apw_dav_subspace** out);
+int apw_dav_subspace_destroy(apw_dav_subspace* dav);
+int apw_dav_subspace_diag(apw_dav_subspace* dav, apw_ham* ham, apw_hpsi_cb hpsi_cb, void* user,
+                          int ldpsi, double* psi, double* eig, const int* is_occupied, int scf,
+                          apw_diag_stats* stats);
+
 /* ------------------------------------------------------------------------------
  * Device functors of the path (parity entry points; same argument meaning as the
  * reference functor named beside each).  Device pointers unless noted.
@@ -161,6 +263,16 @@ int apw_ekinetic_pw_op(apw_ctx* ctx, int nband, int npw, int max_npw, double tpi
                        const double* gk2, double* hpsi, const double* psi);
 /* veff_pw_op<double, DEVICE_GPU> (cuda/veff_op.cu:14-23): out[ir] *= in[ir] */
 int apw_veff_pw_op(apw_ctx* ctx, int size, double* out, const double* in);
+/* elecstate_pw_op<double, DEVICE_GPU> (elecstate_op.h:11-31, cuda/elecstate_op.cu):
+ * rho[spin][ir] += w1 * |wfcr[ir]|^2.  rho: HOST array of device pointers (one per
+ * spin component); wfcr device c128[nrxx]. */
+int apw_elecstate_pw_op(apw_ctx* ctx, int spin, int nrxx, double w1, double* const* rho,
+                        const double* wfcr);
+/* The non-collinear overload (elecstate_op.cpp:27-81): rho[0] += w1(|a|^2 + |b|^2);
+ * domag: rho[1] += 2w1 Re(conj(a)b), rho[2] += 2w1 Im(conj(a)b), rho[3] += w1(|a|^2 - |b|^2);
+ * domag_z only: rho[1] = rho[2] = 0, rho[3] += w1(|a|^2 - |b|^2); neither: rho[1..3] = 0. */
+int apw_elecstate_pw_op_nc(apw_ctx* ctx, int domag, int domag_z, int nrxx, double w1,
+                           double* const* rho, const double* wfcr, const double* wfcr_another_spin);
 /* nonlocal_pw_op<double, DEVICE_GPU> (cuda/nonlocal_op.cu:14-38,95); sum and iat are
  * HOST in/out counters advanced as the reference does. */
 int apw_nonlocal_pw_op(apw_ctx* ctx, int l1, int l2, int l3, int* sum, int* iat, int spin,

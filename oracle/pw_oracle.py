@@ -20,6 +20,19 @@ How each part is pinned (checks live in tests/test_oracle.py):
 * DiagoDavid       -- converged eigenvalues vs dense LAPACK eigh; iteration-level
                       trace vs the compiled reference DiagoDavid (oracle/_ref) when
                       that library is present
+* dnevx            -- Householder reduction vs LAPACK zhetrd; eigenpairs vs LAPACK
+* elecstate_pw_op(_nc) -- reference golden vectors elecstate_op_test.cpp
+* psi_to_rho       -- sum rule (integral of rho = sum of weights for normalised
+                      bands), one-band closed form (plane wave -> constant density)
+* dngvx / dngvd    -- scipy.linalg.eigh (LAPACK zhegvd) on the same pencil
+* DiagoDavSubspace -- converged eigenvalues vs dense eigh; iteration count and
+                      eigenvalues vs the compiled reference Diago_DavSubspace
+* cal_ylm_real     -- reference golden vector math_op_test.cpp (lmax 1) and
+                      scipy.special.sph_harm_y for l <= 4 (sqrt2 Re / Im of Y_l^m)
+* cal_sk           -- reference golden vector wf_op_test.cpp and the closed form
+                      exp(-2 pi i (k+G).tau)
+* cal_vnl / getvnl -- reference golden vector vnl_op_test.cpp; the cubic
+                      interpolation is exact on cubic polynomials
 """
 from __future__ import annotations
 
@@ -83,6 +96,32 @@ def set_real_to_recip_output_op(npw_k, nxyz, add, factor, box_index, inp, out):
     else:
         out[:npw_k] = inp[idx] / float(nxyz)
     return out
+
+
+def elecstate_pw_op(spin, nrxx, w1, rho, wfcr):
+    """rho[spin][ir] += w1 * |wfcr[ir]|^2 -- elecstate_op.cpp:8-25 (nspin 1/2 form)."""
+    rho[spin][:nrxx] += w1 * (wfcr[:nrxx].real ** 2 + wfcr[:nrxx].imag ** 2)
+
+
+def elecstate_pw_op_nc(domag, domag_z, nrxx, w1, rho, wfcr, wfcr_b):
+    """Non-collinear form, elecstate_op.cpp:27-81: charge from both spinor components;
+    magnetisation (rho[1..3]) accumulated with DOMAG, only m_z with DOMAG_Z (m_x, m_y set
+    to 0), else m set to 0."""
+    a, b = wfcr[:nrxx], wfcr_b[:nrxx]
+    na = a.real ** 2 + a.imag ** 2
+    nb = b.real ** 2 + b.imag ** 2
+    rho[0][:nrxx] += w1 * (na + nb)
+    if domag:
+        rho[1][:nrxx] += w1 * 2.0 * (a.real * b.real + a.imag * b.imag)
+        rho[2][:nrxx] += w1 * 2.0 * (a.real * b.imag - b.real * a.imag)
+        rho[3][:nrxx] += w1 * (na - nb)
+    elif domag_z:
+        rho[1][:nrxx] = 0.0
+        rho[2][:nrxx] = 0.0
+        rho[3][:nrxx] += w1 * (na - nb)
+    else:
+        for i in (1, 2, 3):
+            rho[i][:nrxx] = 0.0
 
 
 def fft3d_backward(box, nx, ny, nz):
@@ -417,6 +456,223 @@ class DiagoDavidOracle:
         return nband
 
 
+# ---------------------------------------------------------------------------
+# Diago_DavSubspace -- source/module_hsolver/diago_dav_subspace.cpp
+# ---------------------------------------------------------------------------
+
+def dngvx(nbase, hcc, scc, nband):
+    """Lowest nband eigenpairs of the Hermitian-definite pencil (hcc, scc) on the nbase x
+    nbase leading blocks (dngvx_op -> zhegvx ITYPE 1, dngvd_op.cpp:238-330); both
+    matrices are held fully.  Eigenvectors are S-orthonormal; their phases are those of
+    scipy (the DavSubspace trajectory does not depend on them: refresh() resets vcc)."""
+    h = hcc[:nbase, :nbase]
+    sm = scc[:nbase, :nbase]
+    h = 0.5 * (h + h.conj().T)
+    sm = 0.5 * (sm + sm.conj().T)
+    return sla.eigh(h, sm, subset_by_index=[0, nband - 1])
+
+
+def dngvd(n, hcc, scc):
+    """All eigenpairs of the pencil (dngvd_op -> zhegvd, dngvd_op.cpp:82-140)."""
+    return dngvx(n, hcc, scc, n)
+
+
+class DiagoDavSubspaceOracle:
+    """Restatement of Diago_DavSubspace<complex<double>, CPU> (diago_dav_subspace.cpp).
+
+    Differences from DiagoDavid that the algorithm itself makes: the correction vectors
+    are preconditioned with the shifted QE form 0.5 (1 + x + sqrt(1 + (x-1)^2)),
+    x = |precondition - e| (:372-393), normalised but not orthogonalised (:397-414); the
+    projected problem is the pencil (hcc, scc) (:577-705), except the first solve of
+    each diag_once, which uses hcc alone (dnevx, :612-624); refresh() resets vcc to the
+    identity (:709-798)."""
+
+    def __init__(self, precondition, nband, dim, david_ndim, diag_thr, diag_nmax, need_subspace=False):
+        assert david_ndim > 1 and david_ndim * nband < dim        # :37-38
+        self.n_band, self.dim, self.nbase_x = nband, dim, nband * david_ndim
+        self.diag_thr, self.iter_nmax, self.is_subspace = diag_thr, diag_nmax, need_subspace
+        self.pre = np.asarray(precondition[:dim], dtype=np.float64)
+        nbx = self.nbase_x
+        # persistent work arrays (zero-initialised in the constructor, :41-56)
+        self.psi_iter = np.zeros((dim, nbx), dtype=np.complex128)
+        self.hphi = np.zeros((dim, nbx), dtype=np.complex128)
+        self.hcc = np.zeros((nbx, nbx), dtype=np.complex128)
+        self.scc = np.zeros((nbx, nbx), dtype=np.complex128)
+        self.vcc = np.zeros((nbx, nbx), dtype=np.complex128)
+        self.notconv = 0
+        self.trace = []
+
+    @staticmethod
+    def test_exit_cond(ntry, notconv, scf):
+        """:841-858"""
+        return ntry <= 5 and ((not scf and notconv > 0) or (scf and notconv > 5))
+
+    def diag(self, hpsi_func, psi_in, eigenvalue, is_occupied=None, scf=True):
+        """:804-839; psi_in (nband, ld) row = band, updated in place; returns the iterations."""
+        is_occupied = [True] * self.n_band if is_occupied is None else is_occupied
+        self.notconv = 0
+        total, ntry = 0, 0
+        while True:
+            if self.is_subspace or ntry > 0:
+                self.diagH_subspace(psi_in, eigenvalue, hpsi_func)
+            total += self.diag_once(hpsi_func, psi_in, eigenvalue, is_occupied)
+            ntry += 1
+            if not self.test_exit_cond(ntry, self.notconv, scf):
+                break
+        return total
+
+    def diagH_subspace(self, psi_in, en, hpsi_func):
+        """:860-958: Rayleigh-Ritz of the nband input vectors, pencil solved in full."""
+        nb, dim = self.n_band, self.dim
+        X = np.ascontiguousarray(psi_in[:nb, :dim].T)
+        hphi = np.zeros((dim, nb), dtype=np.complex128)
+        hpsi_func(X, hphi, nb, dim, 0, nb - 1)
+        hcc = X.conj().T @ hphi
+        scc = X.conj().T @ X
+        w, v = dngvd(nb, hcc, scc)
+        en[:nb] = w
+        psi_in[:nb, :dim] = (X @ v).T
+
+    def diag_once(self, hpsi_func, psi_in, eig_in, is_occupied):
+        """:86-291"""
+        nb, dim, nbx = self.n_band, self.dim, self.nbase_x
+        psi_iter, hphi, hcc, scc, vcc = self.psi_iter, self.hphi, self.hcc, self.scc, self.vcc
+        eig_iter = np.zeros(nbx)
+        unconv = list(range(nb))
+        notconv = nb
+        nbase = 0
+        psi_iter[:, :nb] = psi_in[:nb, :dim].T
+        # H on all nbase_x columns (:122-125); only the first nb are used before being
+        # overwritten
+        hpsi_func(psi_iter, hphi, nbx, dim, 0, nbx - 1)
+        nbase = self._cal_elem(nbase, notconv)
+        self._diag_zhegvx(nbase, eig_iter, init=True, is_subspace=self.is_subspace)
+        eig_in[:nb] = eig_iter[:nb]
+        it = 0
+        while True:
+            it += 1
+            self._cal_grad(hpsi_func, nbase, notconv, unconv, eig_iter)
+            nbase = self._cal_elem(nbase, notconv)
+            self._diag_zhegvx(nbase, eig_iter, init=False, is_subspace=False)
+            notconv = 0
+            unconv = [0] * nb
+            for m in range(nb):
+                thr = self.diag_thr if is_occupied[m] else max(self.diag_thr * 5.0, 1e-5)
+                if not abs(eig_iter[m] - eig_in[m]) < thr:
+                    unconv[notconv] = m
+                    notconv += 1
+                eig_in[m] = eig_iter[m]
+            self.notconv = notconv
+            self.trace.append((nbase, notconv, eig_iter[:nb].copy()))
+            if notconv == 0 or nbase + notconv + 1 > nbx or it == self.iter_nmax:
+                psi_in[:nb, :] = 0
+                psi_in[:nb, :dim] = (psi_iter[:, :nbase] @ vcc[:nbase, :nb]).T
+                psi_iter[:, :nb] = psi_in[:nb, :dim].T
+                nbase = self._refresh(nbase, eig_in)
+                if notconv == 0 or it == self.iter_nmax:
+                    break
+        return it
+
+    def _cal_grad(self, hpsi_func, nbase, notconv, unconv, eig_iter):
+        """:294-429"""
+        dim, nbx = self.dim, self.nbase_x
+        psi_iter, hphi, vcc = self.psi_iter, self.hphi, self.vcc
+        for i in range(notconv):
+            if unconv[i] != i:
+                vcc[:nbase, i] = vcc[:nbase, unconv[i]]
+                eig_iter[i] = eig_iter[unconv[i]]
+        new = psi_iter[:, :nbase] @ vcc[:nbase, :notconv]
+        new = new * (-eig_iter[:notconv])[None, :]
+        new = new + hphi[:, :nbase] @ vcc[:nbase, :notconv]
+        for m in range(notconv):
+            x = np.abs(self.pre - eig_iter[m])
+            new[:, m] /= 0.5 * (1.0 + x + np.sqrt(1.0 + (x - 1.0) ** 2))
+        for m in range(notconv):
+            nrm = float(np.real(np.vdot(new[:, m], new[:, m])))
+            assert nrm > 0.0
+            new[:, m] /= np.sqrt(nrm)
+        psi_iter[:, nbase:nbase + notconv] = new
+        hpsi_func(psi_iter, hphi[:, nbase:nbase + notconv], nbx, dim, nbase, nbase + notconv - 1)
+
+    def _cal_elem(self, nbase, notconv):
+        """:431-575: new columns of hcc, scc (rows 0 .. nbase+notconv), Hermitian fill."""
+        psi_iter, hphi, hcc, scc = self.psi_iter, self.hphi, self.hcc, self.scc
+        n1 = nbase + notconv
+        hcc[:n1, nbase:n1] = psi_iter[:, :n1].conj().T @ hphi[:, nbase:n1]
+        scc[:n1, nbase:n1] = psi_iter[:, :n1].conj().T @ psi_iter[:, nbase:n1]
+        last = nbase
+        for i in range(n1):
+            if i >= last:
+                hcc[i, i] = hcc[i, i].real
+                scc[i, i] = scc[i, i].real
+            for j in range(max(i + 1, last), n1):
+                hcc[j, i] = np.conj(hcc[i, j])
+                scc[j, i] = np.conj(scc[i, j])
+        hcc[n1:, n1:] = 0
+        scc[n1:, n1:] = 0
+        return n1
+
+    def _diag_zhegvx(self, nbase, eig_iter, init, is_subspace):
+        """:577-705"""
+        nb = self.n_band
+        if is_subspace:
+            for m in range(nb):
+                eig_iter[m] = self.hcc[m, m].real
+                self.vcc[m, m] = 1.0
+            return
+        if init:
+            w, v = dnevx(nbase, self.hcc, nb)
+        else:
+            w, v = dngvx(nbase, self.hcc, self.scc, nb)
+        eig_iter[:nb] = w
+        self.vcc[:nbase, :nb] = v
+        self.hcc[:nbase, nbase:] = 0
+        self.hcc[nbase:, :nbase] = 0
+        self.scc[:nbase, nbase:] = 0
+        self.scc[nbase:, :nbase] = 0
+
+    def _refresh(self, nbase, eig_in):
+        """:709-798"""
+        nb = self.n_band
+        self.hphi[:, :nb] = self.hphi[:, :nbase] @ self.vcc[:nbase, :nb]
+        self.hcc[:nb, :nb] = 0
+        self.scc[:nb, :nb] = 0
+        self.vcc[:nb, :nb] = 0
+        for i in range(nb):
+            self.hcc[i, i] = eig_in[i]
+            self.scc[i, i] = 1.0
+            self.vcc[i, i] = 1.0
+        return nb
+
+
+def dav_subspace_precondition(sysm):
+    """HSolverPW::update_precondition for dav_subspace (hsolver_pw.cpp:562-579): the
+    kinetic energy |k+G|^2 tpiba2 on the sphere, 1.0 beyond npw."""
+    pre = np.ones(sysm.npwx)
+    pre[: sysm.npw] = np.asarray(sysm.gk2[: sysm.npw]) * sysm.tpiba2
+    return pre
+
+
+def dav_subspace_solve(sysm, thr=1e-9, maxiter=500, need_subspace=False, scf=True, nband=None, psi0=None):
+    """Run the oracle DavSubspace on a synthetic system; returns (eig, psi, iters, oracle)."""
+    ham = HamiltOracle(sysm)
+    nband = nband or sysm.nband
+    dim = sysm.npw
+
+    def hpsi_func(X, HXv, nvec, d, i0, i1):
+        nb = i1 - i0 + 1
+        cols = X[:, i0:i1 + 1]
+        out = ham.hpsi(np.ascontiguousarray(cols.T).reshape(-1), nb, d)
+        HXv[:, :] = out.reshape(nb, d).T
+
+    psi = np.array(sysm.psi0[:nband] if psi0 is None else psi0, dtype=np.complex128, copy=True)
+    eig = np.zeros(nband)
+    dav = DiagoDavSubspaceOracle(dav_subspace_precondition(sysm), nband, dim, sysm.david_ndim, thr, maxiter,
+                                 need_subspace)
+    iters = dav.diag(hpsi_func, psi, eig, scf=scf)
+    return eig, psi, iters, dav
+
+
 def david_solve(sysm, thr=1e-9, maxiter=500, ntry_max=5, notconv_max=0, nband=None, psi0=None):
     """Run the oracle Davidson on a synthetic system; returns (eig, psi, iters, oracle)."""
     ham = HamiltOracle(sysm)
@@ -434,3 +690,259 @@ def david_solve(sysm, thr=1e-9, maxiter=500, ntry_max=5, notconv_max=0, nband=No
     dav = DiagoDavidOracle(sysm.precondition, nband, dim, sysm.david_ndim)
     iters = dav.diag(hpsi_func, psi, eig, thr, maxiter, ntry_max, notconv_max)
     return eig, psi, iters, dav
+
+
+def psi_to_rho(sysm, psi, wg, omega, rho=None):
+    """ElecStatePW::rhoBandK for nspin 1 (elecstate_pw.cpp:142-214): for every band with
+    nonzero weight, psi(G) -> psi(r) by recip_to_real (pw_transform_k.cpp:404-476: box
+    scatter, unnormalised backward FFT) and rho(r) += wg_b / omega * |psi_b(r)|^2.
+    ``psi`` is (nband, >= npw) in the sphere order; returns rho (nxyz real)."""
+    npw, nxyz = sysm.npw, sysm.nxyz
+    rho = np.zeros(nxyz) if rho is None else rho
+    bi = np.asarray(sysm.ig2ixyz, dtype=np.int64)[:npw]
+    for ib in range(len(wg)):
+        w1 = wg[ib] / omega
+        if w1 == 0.0:
+            continue
+        box = np.zeros(nxyz, dtype=np.complex128)
+        box[bi] = psi[ib, :npw]
+        wr = fft3d_backward(box, sysm.nx, sysm.ny, sysm.nz)
+        elecstate_pw_op(0, nxyz, w1, [rho], wr)
+    return rho
+
+
+# ---------------------------------------------------------------------------
+# projector generation: Ylm_Real, structure factor, beta(k+G) (getvnl)
+# ---------------------------------------------------------------------------
+
+def cal_ylm_real(ng, lmax, g):
+    """Real spherical harmonics Y_lm(g^) for l <= lmax at ng points g (ng x 3, flat)
+    -- cal_ylm_real_op, module_base/kernels/math_op.cpp:27-126.
+
+    Associated Legendre P_l^m(cos theta) by the recursion on l (Condon-Shortley phase,
+    P_l^l = (-1)^l (2l-1)!! (1-x^2)^(l/2)); phi from atan with the quadrant fix-up and
+    phi = +-pi/2 when |g_x| <= 1e-9; cos theta = 0 when |g| < 1e-9.  Output rows
+    lm = l^2 + {0: m=0, 2m-1: cos(m phi), 2m: sin(m phi)}, each row ng long:
+    Y_l0 = c P_l0, Y_lm^{cos,sin} = c sqrt((l-m)!/(l+m)!) sqrt2 P_lm {cos, sin}(m phi),
+    c = sqrt((2l+1)/(4 pi)).  Returns ((lmax+1)^2, ng)."""
+    g = np.asarray(g, dtype=np.float64).reshape(ng, 3)
+    ylm = np.zeros(((lmax + 1) ** 2, ng))
+    if lmax == 0:
+        ylm[0] = 1.0 / np.sqrt(4.0 * np.pi)
+        return ylm
+    gmod = np.sqrt((g ** 2).sum(1))
+    safe = np.where(gmod < 1e-9, 1.0, gmod)
+    cost = np.where(gmod < 1e-9, 0.0, g[:, 2] / safe)
+    gx, gy = g[:, 0], g[:, 1]
+    with np.errstate(divide="ignore", invalid="ignore"):
+        at = np.arctan(gy / np.where(np.abs(gx) > 1e-9, gx, 1.0))
+    phi = np.where(gx > 1e-9, at, np.where(gx < -1e-9, at + np.pi, 0.5 * np.pi * np.where(gy >= 0.0, 1.0, -1.0)))
+    P = np.zeros((lmax + 1, lmax + 1, ng))          # P[m, l]
+    x2 = np.maximum(0.0, 1.0 - cost * cost)
+    lm = -1
+    for l in range(lmax + 1):
+        c = np.sqrt((2 * l + 1) / (4.0 * np.pi))
+        if l == 0:
+            P[0, 0] = 1.0
+        elif l == 1:
+            P[0, 1] = cost
+            P[1, 1] = -np.sqrt(x2)
+        else:
+            l1, l2, l3 = l - 1, l - 2, 2 * l - 1
+            for m in range(l2 + 1):
+                P[m, l] = (cost * l3 * P[m, l1] - (l1 + m) * P[m, l2]) / (l - m)
+            P[l1, l] = cost * l3 * P[l1, l1]
+            semif = 1
+            for i in range(l3, 2, -2):
+                semif *= i
+            P[l, l] = semif * x2 ** (l / 2.0) * (-1.0 if l % 2 == 1 else 1.0)
+        lm += 1
+        ylm[lm] = c * P[0, l]
+        for m in range(1, l + 1):
+            same = c * np.sqrt(float(np.prod(np.arange(1, l - m + 1, dtype=np.float64))) /
+                               float(np.prod(np.arange(1, l + m + 1, dtype=np.float64)))) * np.sqrt(2.0)
+            lm += 1
+            ylm[lm] = same * P[m, l] * np.cos(m * phi)
+            lm += 1
+            ylm[lm] = same * P[m, l] * np.sin(m * phi)
+    return ylm
+
+
+def cal_sk(ik, ntype, nx, ny, nz, rho_nx, rho_ny, rho_nz, npw, npwx, fftny, eigts1_nc, eigts2_nc, eigts3_nc,
+           atom_na, igl2isz, is2fftixy, two_pi, kvec_c, atom_tau, eigts1, eigts2, eigts3):
+    """Structure factor sk[iat*npw + ig] = e^{-2 pi i k.tau} eigts1[ix] eigts2[iy] eigts3[iz]
+    with (ix, iy, iz) the signed Miller indices of G shifted by rho_n* -- cal_sk_op,
+    kernels/wf_op.cpp:8-74 (= exp(-2 pi i (k+G).tau) when the eigts are e^{-2 pi i G_j.tau})."""
+    igl2isz = np.asarray(igl2isz, dtype=np.int64)
+    is2fftixy = np.asarray(is2fftixy, dtype=np.int64)
+    kvec_c = np.asarray(kvec_c, dtype=np.float64)
+    atom_tau = np.asarray(atom_tau, dtype=np.float64)
+    nat = int(np.sum(atom_na[:ntype]))
+    sk = np.zeros(nat * npw, dtype=np.complex128)
+    isz = igl2isz[ik * npwx: ik * npwx + npw]
+    iz = isz % nz
+    ixy = is2fftixy[isz // nz]
+    ix, iy = ixy // fftny, ixy % fftny
+    ix = np.where(ix >= nx // 2 + 1, ix - nx, ix) + rho_nx
+    iy = np.where(iy >= ny // 2 + 1, iy - ny, iy) + rho_ny
+    iz = np.where(iz >= nz // 2 + 1, iz - nz, iz) + rho_nz
+    for iat in range(nat):
+        arg = two_pi * float(np.dot(kvec_c[ik * 3: ik * 3 + 3], atom_tau[iat * 3: iat * 3 + 3]))
+        kphase = np.cos(arg) - 1j * np.sin(arg)
+        sk[iat * npw:(iat + 1) * npw] = (kphase * eigts1[iat * eigts1_nc + ix] * eigts2[iat * eigts2_nc + iy]
+                                         * eigts3[iat * eigts3_nc + iz])
+    return sk
+
+
+def polynomial_interpolation(tab, it, nb, tab_2, tab_3, dq, x):
+    """Four-point Lagrange interpolation of tab[it, nb, :] (spacing dq) at x
+    -- _polynomial_interpolation, kernels/vnl_tools.hpp:7-27."""
+    pos = np.asarray(x, dtype=np.float64) / dq
+    iq = pos.astype(np.int64)
+    x0 = pos - iq
+    x1, x2, x3 = 1.0 - x0, 2.0 - x0, 3.0 - x0
+    base = (it * tab_2 + nb) * tab_3 + iq
+    t = np.asarray(tab, dtype=np.float64)
+    return (t[base] * x1 * x2 * x3 / 6.0 + t[base + 1] * x0 * x2 * x3 / 2.0
+            - t[base + 2] * x1 * x0 * x3 / 2.0 + t[base + 3] * x1 * x2 * x0 / 6.0)
+
+
+def cal_vnl(ntype, npw, npwx, nhm, tab_2, tab_3, atom_na, atom_nb, atom_nh, dq, tpiba, gk, ylm, indv, nhtol,
+            nhtolm, tab, sk, vkb=None):
+    """beta projectors vkb[jkb*npwx + ig] = ylm[lm(ih), ig] vq_nb(|k+G| tpiba) sk[iat, ig] (-i)^l(ih),
+    jkb running over types, atoms, ih -- cal_vnl_op, kernels/vnl_op.cpp:8-96 (getvnl,
+    VNL_in_pw.cpp:412-524).  Columns past npw are left as given."""
+    nat = int(np.sum(atom_na[:ntype]))
+    gk = np.asarray(gk, dtype=np.float64).reshape(-1, 3)[:npw]
+    gnorm = np.sqrt((gk ** 2).sum(1)) * tpiba
+    if vkb is None:
+        vkb = np.zeros(int(sum(atom_na[t] * atom_nh[t] for t in range(ntype))) * npwx, dtype=np.complex128)
+    pref_tab = np.array([1.0, -1j, -1.0, 1j])
+    ylm = np.asarray(ylm, dtype=np.float64).reshape(-1, npw)
+    sk = np.asarray(sk).reshape(nat, npw)
+    jkb = iat = 0
+    for it in range(ntype):
+        nh = atom_nh[it]
+        vkb1 = np.zeros((nh, npw))
+        for nb in range(atom_nb[it]):
+            vq = polynomial_interpolation(tab, it, nb, tab_2, tab_3, dq, gnorm)
+            for ih in range(nh):
+                if nb == int(indv[it * nhm + ih]):
+                    vkb1[ih] = ylm[int(nhtolm[it * nhm + ih])] * vq
+        for _ in range(atom_na[it]):
+            for ih in range(nh):
+                pref = pref_tab[int(nhtol[it * nhm + ih]) % 4]
+                vkb[jkb * npwx: jkb * npwx + npw] = vkb1[ih] * sk[iat] * pref
+                jkb += 1
+            iat += 1
+    return vkb
+
+
+def getvnl(ntype, npw, npwx, nhm, tab_2, tab_3, atom_na, atom_nb, atom_nh, dq, tpiba, gk, tau, indv, nhtol,
+           nhtolm, tab, vkb=None):
+    """getvnl for one k-point (VNL_in_pw.cpp:412-524): Ylm_Real of k+G up to the largest l
+    in nhtol, sk = exp(-2 pi i (k+G).tau) (gk: k+G cartesian in 2 pi / a, tau: cartesian
+    in a), then cal_vnl."""
+    lmaxkb = int(max(int(nhtol[it * nhm + ih]) for it in range(ntype) for ih in range(atom_nh[it])))
+    gk = np.asarray(gk, dtype=np.float64).reshape(-1, 3)[:npw]
+    ylm = cal_ylm_real(npw, lmaxkb, gk)
+    tau = np.asarray(tau, dtype=np.float64).reshape(-1, 3)
+    sk = np.exp(-2j * np.pi * (tau @ gk.T)).reshape(-1)
+    return cal_vnl(ntype, npw, npwx, nhm, tab_2, tab_3, atom_na, atom_nb, atom_nh, dq, tpiba, gk, ylm, indv,
+                   nhtol, nhtolm, tab, sk, vkb)
+
+
+# ---------------------------------------------------------------------------
+# wavefunction files (module_io/write_wfc_pw.cpp, read_wfc_pw.cpp), npol 1
+# ---------------------------------------------------------------------------
+
+def _miller(ig2ixyz, ny, nz):
+    idx = np.asarray(ig2ixyz, dtype=np.int64)
+    return np.stack([idx // (ny * nz), (idx // nz) % ny, idx % nz], axis=1).astype(np.int32)
+
+
+def write_wfc_pw(path, ik_global, nkstot, kvec_c, weight, nbands, ecutwfc, lat0, tpiba, G, nx, ny, nz, ig2ixyz,
+                 psi):
+    """ModuleIO::write_wfc_pw for one k-point (write_wfc_pw.cpp:86-218): ``.dat`` records
+    [int bytes][payload][int bytes] -- header (4 int + 7 double = 72 B), G (9 double),
+    (ix, iy, iz) per plane wave, one complex record per band; ``.txt`` the same as text
+    (setw(10), precision 6 header; setw(15) scientific values, four per line).
+    ``psi``: (nbands, npw) complex."""
+    import struct
+    npw = len(ig2ixyz)
+    mil = _miller(ig2ixyz, ny, nz)
+    G = np.asarray(G, dtype=np.float64).reshape(9)
+    psi = np.asarray(psi, dtype=np.complex128)[:nbands, :npw]
+    if path.endswith(".dat"):
+        with open(path, "wb") as f:
+            f.write(struct.pack("<3i4d2i3di", 72, ik_global + 1, nkstot, *kvec_c, weight, npw, nbands, ecutwfc, lat0,
+                                tpiba, 72))
+            f.write(struct.pack("<i9di", 72, *G, 72))
+            f.write(struct.pack("<i", npw * 12) + mil.astype("<i4").tobytes() + struct.pack("<i", npw * 12))
+            for b in range(nbands):
+                f.write(struct.pack("<i", npw * 16) + psi[b].astype("<c16").tobytes() + struct.pack("<i", npw * 16))
+        return
+    w = lambda x: "%10s" % _cfmt(x)   # noqa: E731
+    lines = ["".join("%10s" % h for h in ("Kpoint", "nKpoint", "kv.x", "kv.y", "kv.z", "weight", "ngtot", "nband",
+                                            "ecut", "lat0", "2pi/lat0")),
+             "".join(w(v) for v in (ik_global + 1, nkstot, *kvec_c, weight, npw, nbands, ecutwfc, lat0, tpiba)),
+             "", "<Reciprocal Lattice Vector>"]
+    lines += ["".join(w(G[3 * r + c]) for c in range(3)) for r in range(3)]
+    lines += ["<Reciprocal Lattice Vector>", "", "<G vectors>"]
+    lines += ["".join("%10d" % v for v in m) for m in mil]
+    lines += ["<G vectors>", ""]
+    out = "\n".join(lines) + "\n"
+    for b in range(nbands):
+        out += "\n< Band %d >\n" % (b + 1)
+        vals = []
+        for ig in range(npw):
+            if ig % 4 == 0 and ig != 0:
+                vals.append("\n")
+            vals.append("%15.6e%15.6e" % (psi[b, ig].real, psi[b, ig].imag))
+        out += "".join(vals) + "\n< Band %d >\n" % (b + 1)
+    with open(path, "w") as f:
+        f.write(out)
+
+
+def _cfmt(x):
+    """C++ iostream default float format at precision 6 (printf %g), ints as is."""
+    if isinstance(x, (int, np.integer)):
+        return "%d" % x
+    return "%g" % x
+
+
+def read_wfc_pw(path, ik, nkstot, nbands, kvec_c, lat0, tpiba, G, nx, ny, nz, ig2ixyz):
+    """ModuleIO::read_wfc_pw for one k-point (read_wfc_pw.cpp:96-323, .dat only): header
+    checks (exact comparisons), then wfc[b, i] = file[b, order[ig2ixyz[i]]] with order the
+    file position of each box index (ix ny + iy) nz + iz.  Raises ValueError with the
+    reference's message on a mismatch."""
+    import struct
+    npw = len(ig2ixyz)
+    with open(path, "rb") as f:
+        data = f.read()
+    h = struct.unpack_from("<3i4d2i3di", data, 0)
+    ik_in, nks_in, kv, npw_in, nb_in = h[1], h[2], h[3:6], h[7], h[8]
+    lat0_in, tpiba_in = h[10], h[11]
+    if ik_in != ik + 1 or nks_in != nkstot or npw_in != npw or nb_in != nbands:
+        raise ValueError("ikstot_in != ikstot || nkstot_in != nkstot || npwtot_in != npwtot || "
+                         "nbands_in != GlobalV::NBANDS")
+    if tuple(kv) != tuple(float(x) for x in kvec_c):
+        raise ValueError("k vector in file is not the same as the one in memory")
+    if lat0_in != lat0 or tpiba_in != tpiba:
+        raise ValueError("lat0_in != pw_wfc->lat0 || tpiba_in != pw_wfc->tpiba")
+    off = 80
+    g_in = struct.unpack_from("<i9di", data, off)[1:10]
+    if tuple(g_in) != tuple(np.asarray(G, dtype=np.float64).reshape(9)):
+        raise ValueError("G_in != G")
+    off += 80 + 4
+    mil = np.frombuffer(data, dtype="<i4", count=3 * npw, offset=off).reshape(npw, 3)
+    off += 12 * npw + 4
+    order = np.full(nx * ny * nz, -1, dtype=np.int64)
+    order[(mil[:, 0].astype(np.int64) * ny + mil[:, 1]) * nz + mil[:, 2]] = np.arange(npw)
+    perm = order[np.asarray(ig2ixyz, dtype=np.int64)]
+    wfc = np.zeros((nbands, npw), dtype=np.complex128)
+    for b in range(nbands):
+        off += 4
+        wfc[b] = np.frombuffer(data, dtype="<c16", count=npw, offset=off)[perm]
+        off += 16 * npw + 4
+    return wfc

@@ -33,6 +33,10 @@ def load(detsign: bool = False):
         _lib.ref_david_diag.argtypes = [C.c_int] * 5 + [P, P, C.c_double, P, C.c_int, P, C.c_int, P, P, C.c_int, P,
                                                         P, C.c_int, C.c_int, C.c_int, P, P, C.c_double, C.c_int,
                                                         C.c_int, C.c_int, P, P, C.c_int, P, C.c_int, P]
+        _lib.ref_dav_subspace_diag.restype = C.c_int
+        _lib.ref_dav_subspace_diag.argtypes = ([C.c_int] * 5 + [P, P, C.c_double, P, C.c_int, P, C.c_int, P, P,
+                                                                C.c_int, P, P, C.c_int, C.c_int, C.c_int, P, P,
+                                                                C.c_double, C.c_int, C.c_int, C.c_int, P])
         _lib.ref_hpsi.restype = None
         _lib.ref_hpsi.argtypes = [C.c_int] * 5 + [P, P, C.c_double, P, C.c_int, P, C.c_int, P, P, C.c_int, P, P, P,
                                                   C.c_int, C.c_int]
@@ -92,4 +96,26 @@ def david(s, thr=1e-9, maxiter=500, ntry_max=5, notconv_max=0, psi0=None, nband=
     david.last_trace = trace[:ncall].tolist()
     david.last_etrace = etrace[: ncall * nband].reshape(ncall, nband)
     david.last_dump = dump
+    return eig, psi, it, int(counts[0]), int(counts[1])
+
+
+def dav_subspace(s, thr=1e-9, maxiter=500, need_subspace=False, scf=True, psi0=None, nband=None, precondition=None):
+    """Reference Diago_DavSubspace::diag from the system's start vectors, with the
+    dav_subspace preconditioner of HSolverPW (|k+G|^2 tpiba2 unless given).  Returns
+    (eigenvalues, psi, iterations, hpsi_calls, hpsi_bands)."""
+    lib = load(False)
+    a = _arrays(s)
+    nband = nband or s.nband
+    if precondition is None:
+        pre = np.ones(s.npwx)
+        pre[: s.npw] = np.asarray(s.gk2[: s.npw]) * s.tpiba2
+    else:
+        pre = np.ascontiguousarray(precondition, dtype=np.float64)
+    psi = np.array(s.psi0[:nband] if psi0 is None else psi0, dtype=np.complex128, copy=True, order="C")
+    eig = np.zeros(nband)
+    counts = np.zeros(2, dtype=np.int64)
+    it = lib.ref_dav_subspace_diag(s.nx, s.ny, s.nz, s.npw, s.npwx, _p(a["ig2ixyz"]), _p(a["gk2"]), s.tpiba2,
+                                   _p(a["veff"]), s.nkb, _p(a["vkb"]), len(a["na"]), _p(a["na"]), _p(a["nh"]),
+                                   s.deeq.shape[-1], _p(a["deeq"]), _p(pre), nband, s.david_ndim, s.npwx, _p(psi),
+                                   _p(eig), thr, maxiter, int(need_subspace), int(scf), _p(counts))
     return eig, psi, it, int(counts[0]), int(counts[1])

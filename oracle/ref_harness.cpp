@@ -20,6 +20,7 @@
 #include "module_hamilt_pw/hamilt_pwdft/kernels/nonlocal_op.h"
 #include "module_hamilt_pw/hamilt_pwdft/kernels/veff_op.h"
 #include "module_hsolver/diago_david.h"
+#include "module_hsolver/diago_dav_subspace.h"
 #include "module_hsolver/kernels/math_kernel_op.h"
 
 // FFTW3 API (www.fftw.org/fftw3_doc), resolved from MKL's libmkl_rt.
@@ -161,6 +162,53 @@ int ref_david_diag(int nx, int ny, int nz, int npw, int npwx, const int* ig2ixyz
     for (int i = 0; trace && i < (int)h.trace.size() && i < trace_cap; ++i) trace[i] = h.trace[i];
     for (size_t i = 0; etrace && i < h.etrace.size() && i < (size_t)trace_cap * nband; ++i) etrace[i] = h.etrace[i];
     if (dump_out && !h.dump.empty()) std::memcpy(static_cast<void*>(dump_out), h.dump.data(), h.dump.size() * sizeof(T));
+    return it;
+}
+
+// Runs the reference Diago_DavSubspace<complex<double>, CPU>::diag from psi (nband x
+// ldpsi, band-major c128), H applied by the reference functors.  Returns the iteration
+// count; eig receives the eigenvalues (in: used only as the previous values of the first
+// check, as in HSolverPW).
+int ref_dav_subspace_diag(int nx, int ny, int nz, int npw, int npwx, const int* ig2ixyz, const double* gk2,
+                          double tpiba2, const double* veff, int nkb, const double* vkb, int ntype, const int* na,
+                          const int* nh, int nhm, const double* deeq, const double* precondition, int nband,
+                          int david_ndim, int ldpsi, double* psi, double* eig, double thr, int maxiter,
+                          int need_subspace, int scf, long* counts) {
+    RefHam h;
+    h.nx = nx;
+    h.ny = ny;
+    h.nz = nz;
+    h.npw = npw;
+    h.npwx = npwx;
+    h.nkb = nkb;
+    h.ntype = ntype;
+    h.nhm = nhm;
+    h.tpiba2 = tpiba2;
+    h.ig2ixyz = ig2ixyz;
+    h.gk2 = gk2;
+    h.veff = veff;
+    h.vkb = reinterpret_cast<const T*>(vkb);
+    h.na = na;
+    h.nh = nh;
+    h.deeq = deeq;
+    h.nat = 0;
+    for (int it = 0; it < ntype; ++it) h.nat += na[it];
+    h.init();
+    h.neig = nband;
+    const int dim = npw;
+    auto hpsi_func = [&h](T* X, T* HX, const int nvec, const int d, const int i0, const int i1) {
+        (void)nvec;
+        h.hpsi(X + (size_t)i0 * d, HX, i1 - i0 + 1, d);
+    };
+    std::vector<double> pre(precondition, precondition + npwx);
+    hsolver::diag_comm_info comm(0, 1);
+    hsolver::Diago_DavSubspace<T, CPU> dav(pre, nband, dim, david_ndim, thr, maxiter, need_subspace != 0, comm);
+    const std::vector<bool> occ(nband, true);
+    const int it = dav.diag(hpsi_func, reinterpret_cast<T*>(psi), ldpsi, eig, occ, scf != 0);
+    if (counts) {
+        counts[0] = h.hpsi_calls;
+        counts[1] = h.hpsi_bands;
+    }
     return it;
 }
 

@@ -189,6 +189,45 @@ def test_cgemm_vs_numpy(torch, ctx, ta, tb, m, n, k):
     assert (np.abs(got - ref) <= bound).all(), np.abs(got - ref).max()
 
 
+# ------------------------------------------------------------------ density (psiToRho)
+def test_elecstate_golden(torch, ctx):
+    """elecstate_op_test.cpp vectors through the device functors (nspin 1 and the
+    non-collinear DOMAG_Z form; the latter's expected values are for weight w2)."""
+    from apw import pw
+    g = _load("ref_elecstate_op.json")
+    n = g["nrxx"]
+    rho = [torch.zeros(n, dtype=torch.float64, device="cuda")]
+    pw.elecstate_pw_op(ctx, 0, n, g["w1"], rho, dev(torch, _c(g["wfcr"])))
+    assert np.abs(host(ctx, rho[0]) - np.array(g["expected_rho"])).max() < 6e-9
+    rho4 = [torch.zeros(n, dtype=torch.float64, device="cuda") for _ in range(4)]
+    pw.elecstate_pw_op_nc(ctx, g["DOMAG"], g["DOMAG_Z"], n, g["w2"], rho4, dev(torch, _c(g["wfcr_2"])),
+                          dev(torch, _c(g["wfcr_another_spin_2"])))
+    got = np.concatenate([host(ctx, r) for r in rho4])
+    assert np.abs(got - np.array(g["expected_rho_2"])).max() < 6e-9
+
+
+@pytest.mark.parametrize("name,nb", [("si8", 24), ("kpts16:37", 10)])
+def test_psi_to_rho_vs_oracle(torch, ctx, name, nb):
+    """Fused band-density pass vs the oracle's rhoBandK (random bands and weights, one
+    zero weight), then accumulation on top of an existing density."""
+    from apw import pw
+    s = configs.make("kpts16", ik=37) if name.startswith("kpts16") else configs.make(name)
+    rng = np.random.default_rng(11)
+    X = np.zeros((nb, s.npwx), complex)
+    X[:, : s.npw] = rng.standard_normal((nb, s.npw)) + 1j * rng.standard_normal((nb, s.npw))
+    wg = rng.uniform(0.0, 2.0, nb)
+    wg[3] = 0.0
+    omega = float(s.lat.omega) if hasattr(s.lat, "omega") else 1000.0
+    ref = O.psi_to_rho(s, X, wg, omega)
+    ham = pw.HamiltPW(ctx, s)
+    rho = ham.psi_to_rho(dev(torch, X), nb, wg, omega)
+    got = host(ctx, rho)
+    assert np.abs(got - ref).max() < 1e-12 * np.abs(ref).max()
+    ham.psi_to_rho(dev(torch, X), nb, wg, omega, rho=rho, add=True)
+    assert np.abs(host(ctx, rho) - 2 * ref).max() < 2e-12 * np.abs(ref).max()
+    ham.close()
+
+
 # ------------------------------------------------------------------ eigensolver
 @pytest.mark.parametrize("n,nband,ld", [(1, 1, 1), (2, 1, 3), (48, 24, 48), (200, 60, 256), (640, 160, 640),
                                         (1024, 64, 1024), (1300, 40, 1300)])
