@@ -7,8 +7,8 @@
 //   * update shapes         C(npw x n) += A(npw x k) B(k x n) with k <= ~2e3
 //     (back-projection, cal_grad, refresh, final psi) -> plain 2D tiling.
 //
-// Tile: 64 x 64, 128 x 64 or 64 x 128 (m x n) per workgroup, k step 16 through two LDS
-// stages, 4 waves in a 2 x 2 arrangement, each wave owning TM x TN MFMA 16x16 sub-tiles
+// Tile: 64 x 64, 128 x 64 or 64 x 128 (m x n) per workgroup, k step 8 (16 for the wide
+// tile) through two LDS stages, 4 waves in a 2 x 2 arrangement, each wave owning TM x TN MFMA 16x16 sub-tiles
 // with separate real and imaginary accumulators (a complex product = 4 real MFMAs).  Operand roles are swapped
 // (MFMA "A" <- op(B) indexed by n, MFMA "B" <- op(A) indexed by m) so that the 16 lanes of
 // a group hold 16 consecutive rows m of one column n: the column-major stores of C are then
@@ -16,6 +16,7 @@
 // column l % 16; drow is (l / 16) + 4 r for the f64 instruction and 4 (l / 16) + r for the
 // f32 one (checked on the device by the layout probe below and its unit test).
 #include <type_traits>
+#include <utility>
 
 #include "apw_internal.hpp"
 
@@ -47,14 +48,14 @@ struct Mfma<float> {
     __device__ static inline int drow(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 
-constexpr int BK = 16, NT = 256;
+constexpr int BKMAX = 16, NT = 256;
 
 // op codes: 0 = N, 1 = T, 2 = C.  Each of the 4 waves (2 x 2) owns (16 TM) x (16 TN) of the
 // (32 TM) x (32 TN) tile.  Two LDS stages: while the MFMAs consume stage s, the k-tile
 // after the next is loaded into registers and the next one is written to stage s^1, so a
 // k step costs one workgroup barrier.  LDS rows are padded by one element so that the
 // k-fastest operand stores (16 lanes down a column) fall in distinct banks.
-template <class R, int TA, int TB, int TM, int TN>
+template <class R, int TA, int TB, int TM, int TN, int BK, bool NFAST>
 __global__ __launch_bounds__(NT) void zgemm_kernel(int M, int N, int K, int kchunk,
                                                    const typename Mfma<R>::c_t* __restrict__ A, int lda,
                                                    const typename Mfma<R>::c_t* __restrict__ B, int ldb,
@@ -73,7 +74,9 @@ __global__ __launch_bounds__(NT) void zgemm_kernel(int M, int N, int K, int kchu
     const int lane = tid & 63;
     const int wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1;
-    const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
+    // NFAST: consecutive workgroups walk n for a fixed m, so the m-tile of A they share is
+    // read from HBM once and re-read from cache
+    const int m0 = (NFAST ? blockIdx.y : blockIdx.x) * BM, n0 = (NFAST ? blockIdx.x : blockIdx.y) * BN;
     const int kbeg = blockIdx.z * kchunk;
     const int kend = min(K, kbeg + kchunk);
 
@@ -276,11 +279,12 @@ int opcode(char t) {
     }
 }
 
-template <class R, int TA, int TB, int TM, int TN>
+template <class R, int TA, int TB, int TM, int TN, int BK, bool NFAST>
 void launch(hipStream_t s, dim3 grid, int M, int N, int K, int kchunk, const R* A, int lda, const R* B, int ldb,
             R* C, int ldc, double ar, double ai, double br, double bi, R* part) {
     using c_t = typename Mfma<R>::c_t;
-    hipLaunchKernelGGL((zgemm_kernel<R, TA, TB, TM, TN>), grid, dim3(NT), 0, s, M, N, K, kchunk,
+    if (NFAST) std::swap(grid.x, grid.y);
+    hipLaunchKernelGGL((zgemm_kernel<R, TA, TB, TM, TN, BK, NFAST>), grid, dim3(NT), 0, s, M, N, K, kchunk,
                        reinterpret_cast<const c_t*>(A), lda, reinterpret_cast<const c_t*>(B), ldb,
                        reinterpret_cast<c_t*>(C), ldc, ar, ai, br, bi, reinterpret_cast<c_t*>(part));
 }
@@ -321,7 +325,7 @@ void gemm(hipStream_t s, GemmWorkspace& ws, int num_cu, char ta, char tb, int m,
         splits = std::min(splits, 256);
     }
     int kchunk = (k + splits - 1) / splits;
-    kchunk = (kchunk + BK - 1) / BK * BK;
+    kchunk = (kchunk + BKMAX - 1) / BKMAX * BKMAX;
     splits = (k + kchunk - 1) / kchunk;
     R* part = nullptr;
     if (splits > 1) {
@@ -340,14 +344,14 @@ void gemm(hipStream_t s, GemmWorkspace& ws, int num_cu, char ta, char tb, int m,
 #define APW_GEMM_CASE(X, Y)                                                                                  \
     if (oa == X && ob == Y) {                                                                               \
         if (shape == 1)                                                                                     \
-            launch<R, X, Y, 4, 2>(s, grid, m, n, k, kchunk, A, lda, B, ldb, C, ldc, alpha_re, alpha_im, beta_re, \
-                                  beta_im, part);                                                           \
+            launch<R, X, Y, 4, 2, 8, true>(s, grid, m, n, k, kchunk, A, lda, B, ldb, C, ldc, alpha_re, alpha_im,  \
+                                           beta_re, beta_im, part);                                         \
         else if (shape == 2)                                                                                \
-            launch<R, X, Y, 2, 4>(s, grid, m, n, k, kchunk, A, lda, B, ldb, C, ldc, alpha_re, alpha_im, beta_re, \
-                                  beta_im, part);                                                           \
+            launch<R, X, Y, 2, 4, 16, false>(s, grid, m, n, k, kchunk, A, lda, B, ldb, C, ldc, alpha_re,         \
+                                             alpha_im, beta_re, beta_im, part);                             \
         else                                                                                                \
-            launch<R, X, Y, 2, 2>(s, grid, m, n, k, kchunk, A, lda, B, ldb, C, ldc, alpha_re, alpha_im, beta_re, \
-                                  beta_im, part);                                                           \
+            launch<R, X, Y, 2, 2, 8, false>(s, grid, m, n, k, kchunk, A, lda, B, ldb, C, ldc, alpha_re, alpha_im, \
+                                            beta_re, beta_im, part);                                        \
     }
     APW_GEMM_CASE(0, 0) APW_GEMM_CASE(0, 1) APW_GEMM_CASE(0, 2)
     APW_GEMM_CASE(1, 0) APW_GEMM_CASE(1, 1) APW_GEMM_CASE(1, 2)

@@ -218,7 +218,20 @@ def main():
     ap.add_argument("--workload", default="si64")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-next", action="store_true", help="skip the getvnl / psiToRho / DavSubspace / wfc-file lines")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device copy (the reference for HBM-byte counters) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = 1 << 27                                    # 2^27 doubles = 1 GiB each way
+        x = torch.ones(n, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration": {"bytes_read_per_copy": 8 * n, "bytes_written_per_copy": 8 * n,
+                                               "copies": 3}}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(k-points/cells shard across GPUs with no exchange inside Davidson, see DESIGN.md)",
