@@ -505,37 +505,52 @@ __global__ __launch_bounds__(NT) void k_tridiag_reg(const double2* __restrict__ 
     }
 }
 
-// Sturm count: number of eigenvalues of T strictly below x
-// d and e2 (squared off-diagonals) in LDS; every lane of a wave reads the same address.
-// The operands of four steps are loaded before the four dependent updates, so the chain
-// waits on the divisions only.
-__device__ int sturm(const double* d, const double* e2, int n, double x, double pivmin) {
-    int c = 0;
-    double q = d[0] - x;
-    if (fabs(q) <= pivmin) q = -pivmin;
-    if (q < 0) ++c;
+// Sturm count: number of eigenvalues of T strictly below x, from the sign changes of the
+// characteristic polynomials of the leading blocks, p_i = (d_i - x) p_{i-1} - e_{i-1}^2 p_{i-2}
+// (division-free; the ratio q_i = p_i / p_{i-1} of the LAPACK form is never formed).  A zero
+// p_i counts as a negative ratio and takes the sign opposite to p_{i-1}, as LAPACK's
+// replacement of q = 0 by -pivmin does.  p and p_{i-1} are rescaled by a power of two every
+// 8 steps (the recurrence is homogeneous).  d and e2 (squared off-diagonals) in LDS; every
+// lane of a wave reads the same address.
+__device__ int sturm(const double* d, const double* e2, int n, double x) {
+    double pm = 1.0, p = d[0] - x;
+    int c;
+    double sg;   // sign of the last p, zeros resolved as above
+    if (p == 0.0) {
+        c = 1;
+        sg = -1.0;
+    } else {
+        c = p < 0.0;
+        sg = p < 0.0 ? -1.0 : 1.0;
+    }
+    auto step = [&](double di, double fi) {
+        const double pn = fma(di - x, p, -fi * pm);
+        pm = p;
+        p = pn;
+        if (p == 0.0) {
+            ++c;
+            sg = -sg;
+        } else {
+            const double sp = p < 0.0 ? -1.0 : 1.0;
+            c += sp != sg;
+            sg = sp;
+        }
+    };
     int i = 1;
-    for (; i + 3 < n; i += 4) {
-        const double d0 = d[i], d1 = d[i + 1], d2 = d[i + 2], d3 = d[i + 3];
-        const double f0 = e2[i - 1], f1 = e2[i], f2 = e2[i + 1], f3 = e2[i + 2];
-        q = d0 - x - f0 / q;
-        if (fabs(q) <= pivmin) q = -pivmin;
-        c += q < 0;
-        q = d1 - x - f1 / q;
-        if (fabs(q) <= pivmin) q = -pivmin;
-        c += q < 0;
-        q = d2 - x - f2 / q;
-        if (fabs(q) <= pivmin) q = -pivmin;
-        c += q < 0;
-        q = d3 - x - f3 / q;
-        if (fabs(q) <= pivmin) q = -pivmin;
-        c += q < 0;
+    for (; i + 7 < n; i += 8) {
+        double dd[8], ff[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            dd[q] = d[i + q];
+            ff[q] = e2[i + q - 1];
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) step(dd[q], ff[q]);
+        const int ex = __builtin_amdgcn_frexp_exp(p != 0.0 ? p : pm);
+        p = __builtin_amdgcn_ldexp(p, -ex);
+        pm = __builtin_amdgcn_ldexp(pm, -ex);
     }
-    for (; i < n; ++i) {
-        q = d[i] - x - e2[i - 1] / q;
-        if (fabs(q) <= pivmin) q = -pivmin;
-        c += q < 0;
-    }
+    for (; i < n; ++i) step(d[i], e2[i - 1]);
     return c;
 }
 
@@ -575,7 +590,7 @@ __global__ __launch_bounds__(256) void k_bisect(const double* __restrict__ d, co
     for (int it = 0; it < 64; ++it) {
         if (hi - lo <= 2.0 * DBL_EPSILON * fmax(fabs(lo), fabs(hi)) + pivmin) break;
         const double x = lo + (hi - lo) * (double)(lane + 1) / 65.0;
-        const int c = sturm(ds, e2, n, x, pivmin);
+        const int c = sturm(ds, e2, n, x);
         const unsigned long long above = __ballot(c > j);
         const double x63 = __shfl(x, 63, 64);
         if (above == 0ull) {
@@ -699,31 +714,56 @@ __global__ __launch_bounds__(NT) void k_invit(const double* __restrict__ d, cons
         __syncthreads();
         for (int it = 0; it < 3; ++it) {
             if (threadIdx.x == 0) {
-                // forward: L (row interchanges per piv), carry the running entry
+                // forward: L (row interchanges per piv), carry the running entry; the
+                // operands of 8 steps are loaded before their dependent updates (each read
+                // of xv precedes the write of its slot, as in the step-by-step order)
                 double xk = xv[0];
                 double nx = n > 1 ? xv[1] : 0.0;
-                for (int k = 0; k < n - 1; ++k) {
-                    const double nx2 = k + 2 < n ? xv[k + 2] : 0.0;
-                    const double ck = C[k];
-                    if (piv[k] == 0) {
-                        xv[k] = xk;
-                        xk = nx - ck * xk;
-                    } else {
-                        xv[k] = nx;
-                        xk = xk - ck * nx;
-                    }
+                auto fwd = [&](double ck, int pk, double nx2, int k) {
+                    const bool sw = pk != 0;
+                    xv[k] = sw ? nx : xk;
+                    xk = sw ? fma(-ck, nx, xk) : fma(-ck, xk, nx);
                     nx = nx2;
+                };
+                int k = 0;
+                for (; k + 8 <= n - 1; k += 8) {
+                    double c8[8], x8[8];
+                    int p8[8];
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) {
+                        c8[q] = C[k + q];
+                        p8[q] = piv[k + q];
+                        x8[q] = k + q + 2 < n ? xv[k + q + 2] : 0.0;
+                    }
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) fwd(c8[q], p8[q], x8[q], k + q);
                 }
+                for (; k < n - 1; ++k) fwd(C[k], piv[k], k + 2 < n ? xv[k + 2] : 0.0, k);
                 xv[n - 1] = xk;
                 // backward: U (diagonal A, superdiagonals B, DD), carry x[k+1], x[k+2]
                 double x1 = xk * IA[n - 1], x2 = 0.0;
                 xv[n - 1] = x1;
-                for (int k = n - 2; k >= 0; --k) {
-                    const double xk0 = (xv[k] - B[k] * x1 - (k < n - 2 ? DD[k] * x2 : 0.0)) * IA[k];
-                    xv[k] = xk0;
+                auto bwd = [&](double xx, double bb, double dd, double ia, int kk) {
+                    const double x0 = (xx - bb * x1 - dd * x2) * ia;
+                    xv[kk] = x0;
                     x2 = x1;
-                    x1 = xk0;
+                    x1 = x0;
+                };
+                k = n - 2;
+                for (; k - 7 >= 0; k -= 8) {
+                    double xx[8], bb[8], dd[8], ia[8];
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) {
+                        const int kk = k - q;
+                        xx[q] = xv[kk];
+                        bb[q] = B[kk];
+                        dd[q] = kk < n - 2 ? DD[kk] : 0.0;
+                        ia[q] = IA[kk];
+                    }
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) bwd(xx[q], bb[q], dd[q], ia[q], k - q);
                 }
+                for (; k >= 0; --k) bwd(xv[k], B[k], k < n - 2 ? DD[k] : 0.0, IA[k], k);
             }
             __syncthreads();
             // scale to unit max-norm first (avoid overflow), then Gram-Schmidt against the

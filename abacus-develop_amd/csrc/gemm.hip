@@ -55,7 +55,7 @@ constexpr int BKMAX = 16, NT = 256;
 // after the next is loaded into registers and the next one is written to stage s^1, so a
 // k step costs one workgroup barrier.  LDS rows are padded by one element so that the
 // k-fastest operand stores (16 lanes down a column) fall in distinct banks.
-template <class R, int TA, int TB, int TM, int TN, int BK, bool NFAST>
+template <class R, int TA, int TB, int TM, int TN, int BK, int MAP>
 __global__ __launch_bounds__(NT) void zgemm_kernel(int M, int N, int K, int kchunk,
                                                    const typename Mfma<R>::c_t* __restrict__ A, int lda,
                                                    const typename Mfma<R>::c_t* __restrict__ B, int ldb,
@@ -74,9 +74,19 @@ __global__ __launch_bounds__(NT) void zgemm_kernel(int M, int N, int K, int kchu
     const int lane = tid & 63;
     const int wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1;
-    // NFAST: consecutive workgroups walk n for a fixed m, so the m-tile of A they share is
-    // read from HBM once and re-read from cache
-    const int m0 = (NFAST ? blockIdx.y : blockIdx.x) * BM, n0 = (NFAST ? blockIdx.x : blockIdx.y) * BN;
+    // tile of this workgroup.  MAP 0: blockIdx.x walks m.  MAP 2 (XCD-aware, 1-D grid in x):
+    // workgroup L runs on XCD L % 8, so each XCD takes whole m-tiles and walks their n-tiles
+    // back to back; the A tile they share stays in that XCD's L2.  The grid is padded to a
+    // multiple of 8 m-tiles; padding workgroups leave before any barrier.
+    int bm = blockIdx.x, bn = blockIdx.y;
+    if (MAP == 2) {
+        const int L = blockIdx.x;
+        const int ntn = (N + BN - 1) / BN, tmt = (M + BM - 1) / BM;
+        bm = (L % 8) + 8 * (L / (8 * ntn));
+        bn = (L / 8) % ntn;
+        if (bm >= tmt) return;
+    }
+    const int m0 = bm * BM, n0 = bn * BN;
     const int kbeg = blockIdx.z * kchunk;
     const int kend = min(K, kbeg + kchunk);
 
@@ -279,12 +289,13 @@ int opcode(char t) {
     }
 }
 
-template <class R, int TA, int TB, int TM, int TN, int BK, bool NFAST>
-void launch(hipStream_t s, dim3 grid, int M, int N, int K, int kchunk, const R* A, int lda, const R* B, int ldb,
+template <class R, int TA, int TB, int TM, int TN, int BK, int MAP>
+void launch(hipStream_t s, int splits, int M, int N, int K, int kchunk, const R* A, int lda, const R* B, int ldb,
             R* C, int ldc, double ar, double ai, double br, double bi, R* part) {
     using c_t = typename Mfma<R>::c_t;
-    if (NFAST) std::swap(grid.x, grid.y);
-    hipLaunchKernelGGL((zgemm_kernel<R, TA, TB, TM, TN, BK, NFAST>), grid, dim3(NT), 0, s, M, N, K, kchunk,
+    const int tm = (M + 32 * TM - 1) / (32 * TM), tn = (N + 32 * TN - 1) / (32 * TN);
+    const dim3 grid = MAP == 2 ? dim3((unsigned)(((tm + 7) / 8) * 8 * tn), 1, splits) : dim3(tm, tn, splits);
+    hipLaunchKernelGGL((zgemm_kernel<R, TA, TB, TM, TN, BK, MAP>), grid, dim3(NT), 0, s, M, N, K, kchunk,
                        reinterpret_cast<const c_t*>(A), lda, reinterpret_cast<const c_t*>(B), ldb,
                        reinterpret_cast<c_t*>(C), ldc, ar, ai, br, bi, reinterpret_cast<c_t*>(part));
 }
@@ -311,9 +322,13 @@ void gemm(hipStream_t s, GemmWorkspace& ws, int num_cu, char ta, char tb, int m,
         APW_LAUNCH_CHECK();
         return;
     }
-    // tile shape: 128 x 64 for tall C (m >> n, the npw x nb updates), 64 x 128 for wide, else 64 x 64
+    // tile shape: 128 x 64 or 128 x 32 for tall C (m >> n, the npw x nb updates; the 32-wide
+    // tile unless n is a multiple of 64: it wastes less of the last tile column and gives
+    // more workgroups), 64 x 128 for wide, else 64 x 64 (the split-K inner products, where
+    // the 32-wide tile measured slower)
     const int shape = (m >= 4 * n && m >= 256) ? 1 : ((n >= 4 * m && n >= 256) ? 2 : 0);
-    const int BM = shape == 1 ? 128 : 64, BN = shape == 2 ? 128 : 64;
+    const bool n64 = n % 64 == 0;
+    const int BM = shape == 1 ? 128 : 64, BN = shape == 2 ? 128 : ((shape == 1 && !n64) ? 32 : 64);
     const int tiles_m = (m + BM - 1) / BM, tiles_n = (n + BN - 1) / BN;
     const long tiles = (long)tiles_m * tiles_n;
     const int target = 2 * std::max(num_cu, 64);
@@ -333,7 +348,6 @@ void gemm(hipStream_t s, GemmWorkspace& ws, int num_cu, char ta, char tb, int m,
         ws.part.reserve(need);
         part = reinterpret_cast<R*>(ws.part.get());
     }
-    dim3 grid(tiles_m, tiles_n, splits);
     const double esz = 2.0 * sizeof(R);
     const bool beta_nz = (beta_re != 0.0 || beta_im != 0.0);
     const char* pname = std::is_same<R, double>::value ? (splits > 1 ? "zgemm_f64_splitk" : "zgemm_f64_tiled")
@@ -343,15 +357,18 @@ void gemm(hipStream_t s, GemmWorkspace& ws, int num_cu, char ta, char tb, int m,
                  8.0 * m * n * (double)k);
 #define APW_GEMM_CASE(X, Y)                                                                                  \
     if (oa == X && ob == Y) {                                                                               \
-        if (shape == 1)                                                                                     \
-            launch<R, X, Y, 4, 2, 8, true>(s, grid, m, n, k, kchunk, A, lda, B, ldb, C, ldc, alpha_re, alpha_im,  \
-                                           beta_re, beta_im, part);                                         \
+        if (shape == 1 && n64)                                                                              \
+            launch<R, X, Y, 4, 2, 8, 2>(s, splits, m, n, k, kchunk, A, lda, B, ldb, C, ldc, alpha_re, alpha_im,   \
+                                        beta_re, beta_im, part);                                            \
+        else if (shape == 1)                                                                                \
+            launch<R, X, Y, 4, 1, 8, 2>(s, splits, m, n, k, kchunk, A, lda, B, ldb, C, ldc, alpha_re, alpha_im,   \
+                                        beta_re, beta_im, part);                                            \
         else if (shape == 2)                                                                                \
-            launch<R, X, Y, 2, 4, 16, false>(s, grid, m, n, k, kchunk, A, lda, B, ldb, C, ldc, alpha_re,         \
-                                             alpha_im, beta_re, beta_im, part);                             \
+            launch<R, X, Y, 2, 4, 16, 0>(s, splits, m, n, k, kchunk, A, lda, B, ldb, C, ldc, alpha_re,           \
+                                         alpha_im, beta_re, beta_im, part);                                 \
         else                                                                                                \
-            launch<R, X, Y, 2, 2, 8, false>(s, grid, m, n, k, kchunk, A, lda, B, ldb, C, ldc, alpha_re, alpha_im, \
-                                            beta_re, beta_im, part);                                        \
+            launch<R, X, Y, 2, 2, 8, 0>(s, splits, m, n, k, kchunk, A, lda, B, ldb, C, ldc, alpha_re, alpha_im,   \
+                                        beta_re, beta_im, part);                                            \
     }
     APW_GEMM_CASE(0, 0) APW_GEMM_CASE(0, 1) APW_GEMM_CASE(0, 2)
     APW_GEMM_CASE(1, 0) APW_GEMM_CASE(1, 1) APW_GEMM_CASE(1, 2)
