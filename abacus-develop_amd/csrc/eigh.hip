@@ -15,6 +15,8 @@
 #include <cfloat>
 #include <cmath>
 
+#include <cstdlib>
+
 #include "apw_internal.hpp"
 
 namespace apw {
@@ -29,9 +31,35 @@ __device__ inline double2 cconjmul(double2 a, double2 b) {  // conj(a) * b
     return make_double2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
 }
 
+// Cross-lane moves through DPP (VALU latency, a few cycles) rather than LDS-routed
+// permutes (~100 cycles each): the sequential chains of the reduction and the
+// back-transform are made of these.
+template <int CTRL, int ROWMASK>
+__device__ inline double dpp_d(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROWMASK, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROWMASK, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
+// value of lane `lane` (wave-uniform) in every lane
+__device__ inline double lane_bcast(double v, int lane) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+    return __hiloint2double(hi, lo);
+}
+
+// sum over the 64 lanes, in every lane (fixed order: deterministic).  quad_perm [1,0,3,2]
+// and [2,3,0,1], half-row and row mirrors leave every lane of a 16-lane row with the row
+// sum; row_bcast15 (rows 1, 3) and row_bcast31 (rows 2, 3) carry it to lane 63.  Call from
+// wave-uniform control flow only.
 __device__ inline double wave_sum(double v) {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
+    v += dpp_d<0xB1, 0xF>(v);
+    v += dpp_d<0x4E, 0xF>(v);
+    v += dpp_d<0x141, 0xF>(v);
+    v += dpp_d<0x140, 0xF>(v);
+    v += dpp_d<0x142, 0xA>(v);
+    v += dpp_d<0x143, 0xC>(v);
+    return lane_bcast(v, 63);
 }
 
 // block-wide sum of two doubles (deterministic tree); red: LDS scratch of >= 2*NT/64 doubles
@@ -291,7 +319,7 @@ __device__ inline double2 row_bcast(const double2 (&x)[R], int i) {
 #pragma unroll
     for (int r = 0; r < R; ++r)
         if (r == (i >> 6)) t = x[r];
-    return make_double2(__shfl(t.x, i & 63, 64), __shfl(t.y, i & 63, 64));
+    return make_double2(lane_bcast(t.x, i & 63), lane_bcast(t.y, i & 63));
 }
 
 // Householder tridiagonalisation (lower), LDS-resident columns, vectors in registers.
@@ -592,13 +620,13 @@ __global__ __launch_bounds__(256) void k_bisect(const double* __restrict__ d, co
         const double x = lo + (hi - lo) * (double)(lane + 1) / 65.0;
         const int c = sturm(ds, e2, n, x);
         const unsigned long long above = __ballot(c > j);
-        const double x63 = __shfl(x, 63, 64);
+        const double x63 = lane_bcast(x, 63);
         if (above == 0ull) {
             lo = x63;
         } else {
             const int l = __ffsll((long long)above) - 1;
-            const double xl = __shfl(x, l, 64);
-            const double xlm = __shfl(x, l > 0 ? l - 1 : 0, 64);
+            const double xl = lane_bcast(x, l);
+            const double xlm = lane_bcast(x, l > 0 ? l - 1 : 0);
             if (l > 0) lo = xlm;
             hi = xl;
         }
@@ -1122,6 +1150,14 @@ void heevx_lowest(hipStream_t s, EigWorkspace& ws, int num_cu, int n, int ldh, c
                 if (fixed_r + (size_t)nc * n * sizeof(double2) <= TRIDIAG_LDS_BYTES) break;
             }
             APW_CHECK(w <= std::min(num_cu, 256), "heevx: no workgroup count fits n=%d", n);
+            // tuning override: more workgroups than the LDS minimum (fewer columns each)
+            if (const char* ev = std::getenv("APW_TRIDIAG_NWG")) {
+                const int want = std::atoi(ev);
+                if (want > w) {
+                    w = std::min(want, std::min(num_cu, 256));
+                    nc = (n + w - 1) / w;
+                }
+            }
             const size_t lds = fixed_r + (size_t)nc * n * sizeof(double2);
             const size_t nring = (size_t)2 * n * n;
             ws.ring.reserve(2 * nring);
