@@ -74,12 +74,16 @@ __global__ __launch_bounds__(NT) void zgemm_kernel(int M, int N, int K, int kchu
     const int lane = tid & 63;
     const int wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1;
-    // tile of this workgroup.  MAP 0: blockIdx.x walks m.  MAP 2 (XCD-aware, 1-D grid in x):
+    // tile of this workgroup.  MAP 0: blockIdx.x walks m; MAP 1: it walks n.  MAP 2 (XCD-aware,
+    // 1-D grid in x; measured slower for split-K, used for the tall updates only):
     // workgroup L runs on XCD L % 8, so each XCD takes whole m-tiles and walks their n-tiles
     // back to back; the A tile they share stays in that XCD's L2.  The grid is padded to a
     // multiple of 8 m-tiles; padding workgroups leave before any barrier.
     int bm = blockIdx.x, bn = blockIdx.y;
-    if (MAP == 2) {
+    if (MAP == 1) {   // blockIdx.x walks n
+        bm = blockIdx.y;
+        bn = blockIdx.x;
+    } else if (MAP == 2) {
         const int L = blockIdx.x;
         const int ntn = (N + BN - 1) / BN, tmt = (M + BM - 1) / BM;
         bm = (L % 8) + 8 * (L / (8 * ntn));
@@ -294,7 +298,8 @@ void launch(hipStream_t s, int splits, int M, int N, int K, int kchunk, const R*
             R* C, int ldc, double ar, double ai, double br, double bi, R* part) {
     using c_t = typename Mfma<R>::c_t;
     const int tm = (M + 32 * TM - 1) / (32 * TM), tn = (N + 32 * TN - 1) / (32 * TN);
-    const dim3 grid = MAP == 2 ? dim3((unsigned)(((tm + 7) / 8) * 8 * tn), 1, splits) : dim3(tm, tn, splits);
+    const dim3 grid = MAP == 2 ? dim3((unsigned)(((tm + 7) / 8) * 8 * tn), 1, splits)
+                               : (MAP == 1 ? dim3(tn, tm, splits) : dim3(tm, tn, splits));
     hipLaunchKernelGGL((zgemm_kernel<R, TA, TB, TM, TN, BK, MAP>), grid, dim3(NT), 0, s, M, N, K, kchunk,
                        reinterpret_cast<const c_t*>(A), lda, reinterpret_cast<const c_t*>(B), ldb,
                        reinterpret_cast<c_t*>(C), ldc, ar, ai, br, bi, reinterpret_cast<c_t*>(part));
@@ -322,23 +327,30 @@ void gemm(hipStream_t s, GemmWorkspace& ws, int num_cu, char ta, char tb, int m,
         APW_LAUNCH_CHECK();
         return;
     }
-    // tile shape: 128 x 64 or 128 x 32 for tall C (m >> n, the npw x nb updates; the 32-wide
-    // tile unless n is a multiple of 64: it wastes less of the last tile column and gives
-    // more workgroups), 64 x 128 for wide, else 64 x 64 (the split-K inner products, where
-    // the 32-wide tile measured slower)
-    const int shape = (m >= 4 * n && m >= 256) ? 1 : ((n >= 4 * m && n >= 256) ? 2 : 0);
-    const bool n64 = n % 64 == 0;
-    const int BM = shape == 1 ? 128 : 64, BN = shape == 2 ? 128 : ((shape == 1 && !n64) ? 32 : 64);
-    const int tiles_m = (m + BM - 1) / BM, tiles_n = (n + BN - 1) / BN;
-    const long tiles = (long)tiles_m * tiles_n;
+    // Tile choice (measured on the Davidson shapes).  Inner products (k >> m n, split over k
+    // with a fixed-order reduction): 64 x 32 tiles for n <= 32, 128 x 64 walking n for tall
+    // C, else 64 x 64.  Updates (no split): 128 x 64 or 128 x 32 for tall C (m >> n; the
+    // 32-wide tile unless n is a multiple of 64: less waste in the last tile column, more
+    // workgroups) in the XCD-aware order, 64 x 128 for wide, else 64 x 64.
     const int target = 2 * std::max(num_cu, 64);
-    int splits = 1;
-    if (tiles < target) {
-        splits = (int)((target + tiles - 1) / tiles);
-        const int kmax_splits = std::max(1, k / 512);
-        splits = std::min(splits, kmax_splits);
-        splits = std::min(splits, 256);
-    }
+    auto split_count = [&](int bm, int bn) {
+        const long tiles = (long)((m + bm - 1) / bm) * ((n + bn - 1) / bn);
+        int sp = 1;
+        if (tiles < target) {
+            sp = (int)((target + tiles - 1) / tiles);
+            sp = std::min(sp, std::max(1, k / 512));
+            sp = std::min(sp, 256);
+        }
+        return sp;
+    };
+    const bool tall = m >= 4 * n && m >= 256, wide = n >= 4 * m && n >= 256;
+    const bool inner = split_count(64, 64) > 1;
+    int variant;   // 0: 2x2 map0, 1: 2x1 map0, 2: 4x2 map1, 3: 4x2 map2, 4: 4x1 map2, 5: 2x4 bk16 map0
+    if (inner) variant = n <= 32 ? 1 : (tall ? 2 : 0);
+    else if (tall) variant = n % 64 == 0 ? 3 : 4;
+    else variant = wide ? 5 : 0;
+    static const int VBM[6] = {64, 64, 128, 128, 128, 64}, VBN[6] = {64, 32, 64, 64, 32, 128};
+    int splits = split_count(VBM[variant], VBN[variant]);
     int kchunk = (k + splits - 1) / splits;
     kchunk = (kchunk + BKMAX - 1) / BKMAX * BKMAX;
     splits = (k + kchunk - 1) / kchunk;
@@ -357,18 +369,20 @@ void gemm(hipStream_t s, GemmWorkspace& ws, int num_cu, char ta, char tb, int m,
                  8.0 * m * n * (double)k);
 #define APW_GEMM_CASE(X, Y)                                                                                  \
     if (oa == X && ob == Y) {                                                                               \
-        if (shape == 1 && n64)                                                                              \
-            launch<R, X, Y, 4, 2, 8, 2>(s, splits, m, n, k, kchunk, A, lda, B, ldb, C, ldc, alpha_re, alpha_im,   \
-                                        beta_re, beta_im, part);                                            \
-        else if (shape == 1)                                                                                \
-            launch<R, X, Y, 4, 1, 8, 2>(s, splits, m, n, k, kchunk, A, lda, B, ldb, C, ldc, alpha_re, alpha_im,   \
-                                        beta_re, beta_im, part);                                            \
-        else if (shape == 2)                                                                                \
-            launch<R, X, Y, 2, 4, 16, 0>(s, splits, m, n, k, kchunk, A, lda, B, ldb, C, ldc, alpha_re,           \
-                                         alpha_im, beta_re, beta_im, part);                                 \
-        else                                                                                                \
-            launch<R, X, Y, 2, 2, 8, 0>(s, splits, m, n, k, kchunk, A, lda, B, ldb, C, ldc, alpha_re, alpha_im,   \
-                                        beta_re, beta_im, part);                                            \
+        switch (variant) {                                                                                  \
+            case 1: launch<R, X, Y, 2, 1, 8, 0>(s, splits, m, n, k, kchunk, A, lda, B, ldb, C, ldc, alpha_re,    \
+                                               alpha_im, beta_re, beta_im, part); break;                    \
+            case 2: launch<R, X, Y, 4, 2, 8, 1>(s, splits, m, n, k, kchunk, A, lda, B, ldb, C, ldc, alpha_re,    \
+                                               alpha_im, beta_re, beta_im, part); break;                    \
+            case 3: launch<R, X, Y, 4, 2, 8, 2>(s, splits, m, n, k, kchunk, A, lda, B, ldb, C, ldc, alpha_re,    \
+                                               alpha_im, beta_re, beta_im, part); break;                    \
+            case 4: launch<R, X, Y, 4, 1, 8, 2>(s, splits, m, n, k, kchunk, A, lda, B, ldb, C, ldc, alpha_re,    \
+                                               alpha_im, beta_re, beta_im, part); break;                    \
+            case 5: launch<R, X, Y, 2, 4, 16, 0>(s, splits, m, n, k, kchunk, A, lda, B, ldb, C, ldc, alpha_re,   \
+                                                alpha_im, beta_re, beta_im, part); break;                   \
+            default: launch<R, X, Y, 2, 2, 8, 0>(s, splits, m, n, k, kchunk, A, lda, B, ldb, C, ldc, alpha_re,   \
+                                                alpha_im, beta_re, beta_im, part); break;                   \
+        }                                                                                                   \
     }
     APW_GEMM_CASE(0, 0) APW_GEMM_CASE(0, 1) APW_GEMM_CASE(0, 2)
     APW_GEMM_CASE(1, 0) APW_GEMM_CASE(1, 1) APW_GEMM_CASE(1, 2)
