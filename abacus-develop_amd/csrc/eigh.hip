@@ -329,8 +329,8 @@ __device__ inline double2 row_bcast(const double2 (&x)[R], int i) {
 // reflector are formed with wave reductions only, and one pass over each own column both
 // applies the rank-2 update of step k and accumulates p_{k+1} = A_{k+1}^H v_{k+1} (the
 // column is read and written once per step).  Workgroup barriers: the gather's only.
-template <int R>
-__global__ __launch_bounds__(NT) void k_tridiag_reg(const double2* __restrict__ A, int n, int ncol,
+template <int R, int NTR>
+__global__ __launch_bounds__(NTR) void k_tridiag_reg(const double2* __restrict__ A, int n, int ncol,
                                                     double2* __restrict__ V, double2* __restrict__ tau,
                                                     double* __restrict__ d, double* __restrict__ e,
                                                     double2* __restrict__ ring, unsigned* abort_flag) {
@@ -340,11 +340,11 @@ __global__ __launch_bounds__(NT) void k_tridiag_reg(const double2* __restrict__ 
     __shared__ int stop;
     const int nwg = (int)gridDim.x, bid = (int)blockIdx.x;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    constexpr int NW = NT / 64;
+    constexpr int NW = NTR / 64;
     for (int lc = 0; lc < ncol; ++lc) {
         const int c = bid + lc * nwg;
         if (c >= n) break;
-        for (int i = threadIdx.x; i < n; i += NT) Al[(size_t)lc * n + i] = A[(size_t)c * n + i];
+        for (int i = threadIdx.x; i < n; i += NTR) Al[(size_t)lc * n + i] = A[(size_t)c * n + i];
     }
     __syncthreads();
     double2 v[R], y[R], w[R];   // v_k, y_k, and the corrected column / v_{k+1}
@@ -458,7 +458,7 @@ __global__ __launch_bounds__(NT) void k_tridiag_reg(const double2* __restrict__ 
             unsigned spins = 0;
             while (true) {
                 bool mine = true;
-                for (int i = threadIdx.x; i < 2 * m; i += NT) {
+                for (int i = threadIdx.x; i < 2 * m; i += NTR) {
                     const int ii = i < m ? i : n + (i - m);
                     double2 val;
                     if (ring_get(slot + ii, val)) g[ii] = val;
@@ -1127,8 +1127,9 @@ void heevx_lowest(hipStream_t s, EigWorkspace& ws, int num_cu, int n, int ldh, c
         double* dp = ws.d.get();
         double* ep = ws.e.get();
         unsigned* barp = ws.bar.get();
-        // algorithmic: per column a hemv read and a rank-2 read+write of the trailing matrix
-        ProfScope ps(prof, s, "eig_tridiag", 16.0 * 3.0 * n * (double)n * n / 3.0, 16.0 / 3.0 * n * (double)n * n);
+        // algorithmic HBM bytes: the matrix read once (16 n^2) and the reflectors written
+        // (8 n^2); the per-column hemv and rank-2 update work on LDS-resident columns
+        ProfScope ps(prof, s, "eig_tridiag", 24.0 * n * (double)n, 16.0 / 3.0 * n * (double)n * n);
         // LDS-resident form: the fewest workgroups whose column share fits the LDS budget
         const size_t fixed = (size_t)3 * n * sizeof(double2) + 64 * sizeof(double);
         int nwg = -1, ncol = 0;
@@ -1166,24 +1167,28 @@ void heevx_lowest(hipStream_t s, EigWorkspace& ws, int num_cu, int n, int ldh, c
                                ring, nring);
             APW_LAUNCH_CHECK();
             const void* fn = nullptr;
+            // threads per workgroup: 512 for 6 <= R <= 12 (measured faster at n = 480, 640: each
+            // wave updates fewer columns); 256 otherwise (small n: the extra waves cost more in
+            // barriers than they save; R = 16: the register budget of v, y, w, 6 R doubles a lane)
+            int ntr = 256;
             switch (R <= 6 ? R : (R <= 8 ? 8 : (R <= 10 ? 10 : (R <= 12 ? 12 : 16)))) {
-                case 1: fn = reinterpret_cast<const void*>(k_tridiag_reg<1>); break;
-                case 2: fn = reinterpret_cast<const void*>(k_tridiag_reg<2>); break;
-                case 3: fn = reinterpret_cast<const void*>(k_tridiag_reg<3>); break;
-                case 4: fn = reinterpret_cast<const void*>(k_tridiag_reg<4>); break;
-                case 5: fn = reinterpret_cast<const void*>(k_tridiag_reg<5>); break;
-                case 6: fn = reinterpret_cast<const void*>(k_tridiag_reg<6>); break;
-                case 8: fn = reinterpret_cast<const void*>(k_tridiag_reg<8>); break;
-                case 10: fn = reinterpret_cast<const void*>(k_tridiag_reg<10>); break;
-                case 12: fn = reinterpret_cast<const void*>(k_tridiag_reg<12>); break;
-                default: fn = reinterpret_cast<const void*>(k_tridiag_reg<16>); break;
+                case 1: fn = reinterpret_cast<const void*>(k_tridiag_reg<1, 256>); ntr = 256; break;
+                case 2: fn = reinterpret_cast<const void*>(k_tridiag_reg<2, 256>); ntr = 256; break;
+                case 3: fn = reinterpret_cast<const void*>(k_tridiag_reg<3, 256>); ntr = 256; break;
+                case 4: fn = reinterpret_cast<const void*>(k_tridiag_reg<4, 256>); ntr = 256; break;
+                case 5: fn = reinterpret_cast<const void*>(k_tridiag_reg<5, 256>); ntr = 256; break;
+                case 6: fn = reinterpret_cast<const void*>(k_tridiag_reg<6, 512>); ntr = 512; break;
+                case 8: fn = reinterpret_cast<const void*>(k_tridiag_reg<8, 512>); ntr = 512; break;
+                case 10: fn = reinterpret_cast<const void*>(k_tridiag_reg<10, 512>); ntr = 512; break;
+                case 12: fn = reinterpret_cast<const void*>(k_tridiag_reg<12, 512>); ntr = 512; break;
+                default: fn = reinterpret_cast<const void*>(k_tridiag_reg<16, 256>); ntr = 256; break;
             }
             APW_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
             const double2* Ac = A;
             unsigned* abortp = barp + 1;
             int ncol_r = nc;
             void* args[] = {&Ac, &n, &ncol_r, &Vp, &taup, &dp, &ep, &ring, &abortp};
-            APW_HIP(hipLaunchCooperativeKernel(fn, dim3(w), dim3(NT), args, (unsigned)lds, s));
+            APW_HIP(hipLaunchCooperativeKernel(fn, dim3(w), dim3(ntr), args, (unsigned)lds, s));
         } else {
             // columns in LDS when the share fits, else in A itself (one column per
             // workgroup slot, L2/Infinity-Cache resident); the ring as above

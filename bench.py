@@ -51,11 +51,41 @@ def run_solve(pw, ctx, ham, s, psi0_dev, precision):
     return t1 - t0, it, eig, st
 
 
-def roofline_from_profile(prof):
-    """Dominant kernel (largest total device time) and its achieved rate vs the MI355X peak."""
-    if not prof:
-        return None, {}
-    name, e = max(prof.items(), key=lambda kv: kv[1]["ms"])
+# library profiler name -> kernel-name prefixes in the rocprofv3 traffic summary
+_PMC_FAMILY = {
+    "eig_tridiag": ("apw::k_tridiag_reg", "apw::k_tridiag_lds"),
+    "zgemm_f64_tiled": ("apw::zgemm_kernel<double",), "zgemm_f64_splitk": ("apw::zgemm_kernel<double",),
+    "cgemm_f32_tiled": ("apw::zgemm_kernel<float",), "cgemm_f32_splitk": ("apw::zgemm_kernel<float",),
+    "fft_x_veff_fused": ("apw::k_xpass",), "fft_y_bwd": ("apw::k_ybwd",), "fft_y_fwd": ("apw::k_yfwd",),
+    "fft_z_bwd_scatter": ("apw::k_zbwd",), "fft_z_fwd_gather": ("apw::k_zfwd",),
+    "eig_bisect": ("apw::k_bisect",), "eig_invit": ("apw::k_invit",),
+    "eig_backtransform": ("apw::k_backtransform",), "chol_inv": ("apw::k_chol_inv",),
+}
+# kernels whose time is set by dependent latency chains, not by a byte or flop roof
+_LATENCY_BOUND = {
+    "eig_tridiag": "latency-bound: one cross-workgroup exchange per column of the reduction",
+    "eig_bisect": "latency-bound: sequential Sturm recurrences", "eig_invit": "latency-bound: sequential solves",
+    "eig_backtransform": "latency-bound: one wave reduction per reflector", "chol_inv": "latency-bound",
+}
+
+
+def _pmc_traffic(name):
+    """HBM bytes per dispatch (FETCH_SIZE + WRITE_SIZE, copy-calibrated) of the kernels behind a
+    profiler entry, dispatch-weighted, from the committed rocprofv3 summary; None if absent."""
+    path = os.path.join(ROOT, "profiles", "si64_davidson_hbm_traffic.json")
+    if name not in _PMC_FAMILY or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        ks = json.load(f)["kernels"]
+    tot = cnt = 0.0
+    for k, v in ks.items():
+        if k.startswith(_PMC_FAMILY[name]):
+            tot += (v["fetch_bytes"] + v["write_bytes"]) * v["dispatches"]
+            cnt += v["dispatches"]
+    return round(tot / cnt) if cnt else None
+
+
+def _roofline(name, e):
     avg_s = e["ms"] * 1e-3 / max(1, e["launches"])
     if name.startswith(("zgemm", "cgemm")):
         bound, unit = "mfma", "TFLOP/s"
@@ -65,9 +95,21 @@ def roofline_from_profile(prof):
         bound, unit = "hbm", "GB/s"
         achieved = e["bytes"] / max(1, e["launches"]) / avg_s / 1e9
         peak = HBM_PEAK_GBS
-    return {"bound": bound, "achieved": round(achieved, 3), "peak": peak, "unit": unit,
-            "frac": round(achieved / peak, 4), "traffic": None, "kernel": name,
-            "avg_launch_us": round(avg_s * 1e6, 2), "launches": e["launches"]}, e
+    out = {"bound": bound, "achieved": round(achieved, 3), "peak": peak, "unit": unit,
+           "frac": round(achieved / peak, 4), "traffic": _pmc_traffic(name), "kernel": name,
+           "avg_launch_us": round(avg_s * 1e6, 2), "launches": e["launches"]}
+    if name in _LATENCY_BOUND:
+        out["note"] = _LATENCY_BOUND[name]
+    return out
+
+
+def roofline_from_profile(prof):
+    """Dominant kernel (largest total device time) and its achieved rate vs the MI355X peak;
+    achieved = algorithmic bytes (or flops) per launch / mean launch time (HIP events)."""
+    if not prof:
+        return None, {}
+    name, e = max(prof.items(), key=lambda kv: kv[1]["ms"])
+    return _roofline(name, e), e
 
 
 _CPU_SAMPLE = r"""
@@ -310,6 +352,7 @@ def main():
         "max_abs_eig_diff_mixed_vs_fp64_Ry": float(np.abs(eig_m - eig64).max()),
         "warmup_s": [round(t, 4) for t in warm],
         "roofline": roof,
+        "rooflines": [_roofline(k, v) for k, v in sorted(prof.items(), key=lambda kv: -kv[1]["ms"])[:8]],
         "kernels": {k: {"launches": v["launches"], "ms": round(v["ms"], 3),
                         "avg_us": round(1e3 * v["ms"] / max(1, v["launches"]), 2),
                         "GB_s": round(v["bytes"] / max(v["ms"], 1e-9) / 1e6, 1),
