@@ -241,16 +241,20 @@ __global__ __launch_bounds__(NT) void k_tridiag_lds(double2* __restrict__ A, int
                     else mine = false;
                 }
                 if (__syncthreads_and(mine)) break;
-                if (threadIdx.x == 0) {
-                    int st = __hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u;
-                    if (!st && ++spins > SPIN_LIMIT) {
-                        atomicExch(abort_flag, 1u);
-                        st = 1;
+                // the abort word and the spin bound every 32 failed polls: its load is a
+                // memory round trip that would otherwise lengthen every wait
+                if ((++spins & 31u) == 0) {
+                    if (threadIdx.x == 0) {
+                        int st = __hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u;
+                        if (!st && spins > SPIN_LIMIT) {
+                            atomicExch(abort_flag, 1u);
+                            st = 1;
+                        }
+                        stop = st;
                     }
-                    stop = st;
+                    __syncthreads();
+                    if (stop) break;
                 }
-                __syncthreads();
-                if (stop) break;
                 __builtin_amdgcn_s_sleep(1);
             }
         }
@@ -321,7 +325,6 @@ __device__ inline double2 row_bcast(const double2 (&x)[R], int i) {
         if (r == (i >> 6)) t = x[r];
     return make_double2(lane_bcast(t.x, i & 63), lane_bcast(t.y, i & 63));
 }
-
 // Householder tridiagonalisation (lower), LDS-resident columns, vectors in registers.
 // Same ownership and exchange as k_tridiag_lds; what changes is where the per-column work
 // happens.  Every wave holds the full vectors v_k, y_k and the next reflector v_{k+1} in
@@ -465,16 +468,20 @@ __global__ __launch_bounds__(NTR) void k_tridiag_reg(const double2* __restrict__
                     else mine = false;
                 }
                 if (__syncthreads_and(mine)) break;
-                if (threadIdx.x == 0) {
-                    int st = __hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u;
-                    if (!st && ++spins > SPIN_LIMIT) {
-                        atomicExch(abort_flag, 1u);
-                        st = 1;
+                // the abort word and the spin bound every 32 failed polls: its load is a
+                // memory round trip that would otherwise lengthen every wait
+                if ((++spins & 31u) == 0) {
+                    if (threadIdx.x == 0) {
+                        int st = __hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u;
+                        if (!st && spins > SPIN_LIMIT) {
+                            atomicExch(abort_flag, 1u);
+                            st = 1;
+                        }
+                        stop = st;
                     }
-                    stop = st;
+                    __syncthreads();
+                    if (stop) break;
                 }
-                __syncthreads();
-                if (stop) break;
                 __builtin_amdgcn_s_sleep(1);
             }
         }
@@ -551,18 +558,13 @@ __device__ int sturm(const double* d, const double* e2, int n, double x) {
         c = p < 0.0;
         sg = p < 0.0 ? -1.0 : 1.0;
     }
-    auto step = [&](double di, double fi) {
+    auto step = [&](double di, double fi) {   // branch-free: the sign logic stays off the p chain
         const double pn = fma(di - x, p, -fi * pm);
         pm = p;
         p = pn;
-        if (p == 0.0) {
-            ++c;
-            sg = -sg;
-        } else {
-            const double sp = p < 0.0 ? -1.0 : 1.0;
-            c += sp != sg;
-            sg = sp;
-        }
+        const double sn = p == 0.0 ? -sg : (p < 0.0 ? -1.0 : 1.0);
+        c += sn != sg;
+        sg = sn;
     };
     int i = 1;
     for (; i + 7 < n; i += 8) {
@@ -702,30 +704,23 @@ __global__ __launch_bounds__(NT) void k_invit(const double* __restrict__ d, cons
                 const double an2 = k + 2 < n ? ds[k + 2] - lam : 0.0;
                 const double ck2 = k + 1 < n - 1 ? es[k + 1] : 0.0;
                 const double bn2 = k + 3 < n ? es[k + 2] : 0.0;
-                if (fabs(ck) <= fabs(ak)) {
-                    piv[k] = 0;
-                    if (fabs(ak) < tiny) ak = (ak >= 0 ? tiny : -tiny);
-                    const double ia = 1.0 / ak;
-                    const double mult = ck * ia;
-                    A[k] = ak;
-                    IA[k] = ia;
-                    B[k] = bk;
-                    C[k] = mult;
-                    DD[k] = 0.0;
-                    ak = an - mult * bk;
-                    bk = bn;
-                } else {
-                    piv[k] = 1;
-                    const double ia = 1.0 / ck;
-                    const double mult = ak * ia;
-                    A[k] = ck;
-                    IA[k] = ia;
-                    B[k] = an;
-                    C[k] = mult;
-                    DD[k] = bn;
-                    ak = bk - mult * an;
-                    bk = -mult * bn;
-                }
+                // branch-free (selects keep the exec mask whole on the sequential chain):
+                // no interchange when |c_k| <= |a_k| (a tiny a_k is replaced by +-tiny), else
+                // rows k and k+1 are swapped
+                const bool keep = fabs(ck) <= fabs(ak);
+                const double akk = (keep && fabs(ak) < tiny) ? (ak >= 0 ? tiny : -tiny) : ak;
+                const double pv = keep ? akk : ck;
+                const double ia = 1.0 / pv;
+                const double mult = (keep ? ck : akk) * ia;
+                piv[k] = keep ? 0 : 1;
+                A[k] = pv;
+                IA[k] = ia;
+                B[k] = keep ? bk : an;
+                C[k] = mult;
+                DD[k] = keep ? 0.0 : bn;
+                const double akn = keep ? fma(-mult, bk, an) : fma(-mult, an, bk);
+                bk = keep ? bn : -mult * bn;
+                ak = akn;
                 an = an2;
                 ck = ck2;
                 bn = bn2;
