@@ -248,6 +248,150 @@ __device__ inline cplx<R>* fft_lines(cplx<R>* x, cplx<R>* y, const cplx<R>* tw, 
 #define APW_LEN(NF, p) ((NF) != 0 ? (NF) : (p).n)
 #define APW_LINES(NF, Lrt) ((NF) != 0 ? lines_for(NF) : (Lrt))
 
+// ---- register-resident transforms ------------------------------------------------------
+// A length-P DFT held in one thread's registers, with every index and constant fixed at
+// compile time.  The x pass splits its length N = NA * NB (four-step: NB transforms of
+// length NA, twiddles W_N^(n2 k1), one exchange through LDS, NA transforms of length NB),
+// so each line crosses LDS once per direction instead of once per radix stage.
+
+// cos and sin of 2 pi m / p, evaluated at compile time: exact rational reduction to the
+// first octant, then the Taylor series in double-double arithmetic (Dekker's products), so
+// the constants are the correctly rounded values to within one unit in the last place.
+struct CtDD { double hi, lo; };
+struct CtCis { double c, s; };
+constexpr CtDD ct_two_sum(double a, double b) {
+    const double s = a + b, bb = s - a;
+    return {s, (a - (s - bb)) + (b - bb)};
+}
+constexpr CtDD ct_split(double a) {
+    const double c = 134217729.0 * a;
+    const double hi = c - (c - a);
+    return {hi, a - hi};
+}
+constexpr CtDD ct_two_prod(double a, double b) {
+    const double p = a * b;
+    const CtDD x = ct_split(a), y = ct_split(b);
+    return {p, ((x.hi * y.hi - p) + x.hi * y.lo + x.lo * y.hi) + x.lo * y.lo};
+}
+constexpr CtDD ct_add(CtDD a, CtDD b) {
+    const CtDD s = ct_two_sum(a.hi, b.hi);
+    return ct_two_sum(s.hi, s.lo + a.lo + b.lo);
+}
+constexpr CtDD ct_mul(CtDD a, CtDD b) {
+    const CtDD p = ct_two_prod(a.hi, b.hi);
+    return ct_two_sum(p.hi, p.lo + a.hi * b.lo + a.lo * b.hi);
+}
+constexpr CtDD ct_div(CtDD a, double d) {
+    const double q = a.hi / d;
+    const CtDD p = ct_two_prod(q, d);
+    return ct_two_sum(q, ((a.hi - p.hi) - p.lo + a.lo) / d);
+}
+constexpr CtDD ct_series(CtDD x, bool sine) {   // sin or cos of x in [0, pi/4]
+    const CtDD x2 = ct_mul(x, x);
+    CtDD t = sine ? x : CtDD{1.0, 0.0};
+    CtDD s = t;
+    for (int k = 1; k <= 14; ++k) {
+        const double d = sine ? (2.0 * k) * (2.0 * k + 1.0) : (2.0 * k - 1.0) * (2.0 * k);
+        t = ct_div(ct_mul(t, x2), -d);
+        s = ct_add(s, t);
+    }
+    return s;
+}
+constexpr CtCis ct_cis(long m, long p) {
+    m %= p;
+    if (m < 0) m += p;
+    const long oct = 8 * m / p, r = 8 * m - oct * p;   // 2 pi m / p = (oct + r / p) pi / 4
+    const CtDD quarter_pi = {0.78539816339744830962, 3.0616169978683830179e-17};
+    const CtDD x = ct_mul(quarter_pi, ct_div(CtDD{(double)((oct % 2 == 0) ? r : p - r), 0.0}, (double)p));
+    const double c = ct_series(x, false).hi, s = ct_series(x, true).hi;
+    if (oct % 2 == 0) {
+        switch (oct / 2) {
+            case 0: return {c, s};
+            case 1: return {-s, c};
+            case 2: return {-c, -s};
+            default: return {s, -c};
+        }
+    }
+    switch ((oct + 1) / 2) {   // angle = q pi / 2 - x
+        case 1: return {s, c};
+        case 2: return {-c, s};
+        case 3: return {-s, -c};
+        default: return {c, -s};
+    }
+}
+
+template <int I, int N, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+    if constexpr (I < N) {
+        f(std::integral_constant<int, I>{});
+        static_for<I + 1, N>(f);
+    }
+}
+#define APW_CI(c) decltype(c)::value
+
+// v * exp(2 pi i M / P); quarter turns are exact
+template <class C, class R, int P, int M>
+__device__ __forceinline__ C rot(C v) {
+    constexpr int m = ((M % P) + P) % P;
+    C r;
+    if constexpr (m == 0) {
+        r = v;
+    } else if constexpr (2 * m == P) {
+        r.x = -v.x;
+        r.y = -v.y;
+    } else if constexpr (4 * m == P) {
+        r.x = -v.y;
+        r.y = v.x;
+    } else if constexpr (4 * m == 3 * P) {
+        r.x = v.y;
+        r.y = -v.x;
+    } else {
+        constexpr CtCis w = ct_cis(m, P);
+        r.x = v.x * (R)w.c - v.y * (R)w.s;
+        r.y = v.x * (R)w.s + v.y * (R)w.c;
+    }
+    return r;
+}
+
+constexpr int ct_first_radix(int p) { return p % 4 == 0 ? 4 : p % 5 == 0 ? 5 : p % 3 == 0 ? 3 : 2; }
+
+// v[k] <- sum_n v[n] exp(SG 2 pi i n k / P), natural order in and out (P = 2^a 3^b 5^c)
+template <class C, class R, int P, int SG>
+__device__ __forceinline__ void rdft(C* v) {
+    if constexpr (P == 2 || P == 3 || P == 4 || P == 5) {
+        dft_radix<C, R>(v, P, SG);
+    } else if constexpr (P > 1) {
+        constexpr int PA = ct_first_radix(P), PB = P / PA;
+        C a[P];
+        static_for<0, PB>([&](auto n2c) {
+            C t[PA];
+            static_for<0, PA>([&](auto n1c) { t[APW_CI(n1c)] = v[PB * APW_CI(n1c) + APW_CI(n2c)]; });
+            rdft<C, R, PA, SG>(t);
+            static_for<0, PA>([&](auto k1c) {
+                a[APW_CI(n2c) * PA + APW_CI(k1c)] = rot<C, R, P, SG * APW_CI(n2c) * APW_CI(k1c)>(t[APW_CI(k1c)]);
+            });
+        });
+        static_for<0, PA>([&](auto k1c) {
+            C t[PB];
+            static_for<0, PB>([&](auto n2c) { t[APW_CI(n2c)] = a[APW_CI(n2c) * PA + APW_CI(k1c)]; });
+            rdft<C, R, PB, SG>(t);
+            static_for<0, PB>([&](auto k2c) { v[APW_CI(k1c) + PA * APW_CI(k2c)] = t[APW_CI(k2c)]; });
+        });
+    }
+}
+
+// x-pass split N = NA * NB: the largest NA with NA <= NB <= 16 (0: none)
+constexpr int ct_split_a(int n) {
+    for (int a = 16; a >= 2; --a)
+        if (n % a == 0 && a * a <= n && n / a <= 16) return a;
+    return 0;
+}
+constexpr int XL = 16;   // lines (consecutive z) per workgroup of the register x pass
+constexpr int xreg_threads(int n) { return XL * (n / ct_split_a(n)); }
+#define APW_XREG_CHECK(N) static_assert(ct_split_a(N) != 0, "x length without a register split");
+APW_FFT_SIZES(APW_XREG_CHECK)
+#undef APW_XREG_CHECK
+
 // ---- pass 1: z-backward on sticks, gathered from the sphere --------------------------
 // grid (ceil(ns/L), nb); LDS: 2 * nz * (L+1) + nz complex
 template <class R, int NF>
@@ -482,6 +626,362 @@ __global__ __launch_bounds__(NT) void k_xrho(const cplx<double>* __restrict__ X,
     }
 }
 
+// ---- raw buffer access: a lane whose byte offset is outside [0, bytes) reads zeros and its
+// store is dropped, so gathers and scatters with holes need no branches (and no waits per
+// element).  Descriptor word 3 = 0x00020000 (32-bit data format), as for every gfx9 part.
+constexpr unsigned BUF_OOB = 0x80000000u;
+__device__ inline __amdgpu_buffer_rsrc_t make_rsrc(const void* p, unsigned bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, (int)bytes, 0x00020000);
+}
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+__device__ inline double2 buf_load(__amdgpu_buffer_rsrc_t r, unsigned off, double2*) {
+    const u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+    double2 d;
+    __builtin_memcpy(&d, &v, 16);
+    return d;
+}
+__device__ inline float2 buf_load(__amdgpu_buffer_rsrc_t r, unsigned off, float2*) {
+    const u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
+    float2 d;
+    __builtin_memcpy(&d, &v, 8);
+    return d;
+}
+__device__ inline void buf_store(double2 d, __amdgpu_buffer_rsrc_t r, unsigned off) {
+    u32x4 v;
+    __builtin_memcpy(&v, &d, 16);
+    __builtin_amdgcn_raw_buffer_store_b128(v, r, off, 0, 0);
+}
+__device__ inline void buf_store(float2 d, __amdgpu_buffer_rsrc_t r, unsigned off) {
+    u32x2 v;
+    __builtin_memcpy(&v, &d, 8);
+    __builtin_amdgcn_raw_buffer_store_b64(v, r, off, 0, 0);
+}
+
+// ---- pass 3, register form (every x length with a compile-time split) ------------------
+// Workgroup: XL consecutive z lines at one iy; thread (l, g): l = line, g < NB.
+//   step A (all threads, g = n2):  x = NB n1 + g, n1 < NA, DFT_NA, twiddle W_N^(+-g k1)
+//   LDS exchange
+//   step B (threads g < NA, g = k1): DFT_NB, giving x = g + NA k2
+// MODE 0 runs A, B, * veff, then the forward transform with the roles swapped (B', A'),
+// so the data crosses LDS twice in all.  MODE 1: A, B, store into the full box (add:
+// box += factor * result); MODE 2: load the full box, B', A'.
+// grid (ny, ceil(nz / XL), nb); static LDS N * XL complex
+// The sphere's x slots are at most two runs of consecutive x (sorted order): slot of x =
+// x - r.x for x in [r.x, r.x + r.y), r.y + x - r.z for x in [r.z, r.z + r.w), else none.
+__device__ inline int xrun_slot(int4 r, int x) {
+    return (unsigned)(x - r.x) < (unsigned)r.y ? x - r.x : ((unsigned)(x - r.z) < (unsigned)r.w ? r.y + x - r.z : -1);
+}
+
+template <class R, int N, int MODE>
+__global__ __launch_bounds__(xreg_threads(N)) void k_xreg(cplx<R>* __restrict__ X, int nxr, int ny, int nz, int4 xrun,
+                                                         const R* __restrict__ veff, double2* __restrict__ box,
+                                                         int add, double factor, const cplx<R>* __restrict__ tw) {
+    using C = cplx<R>;
+    constexpr int NA = ct_split_a(N), NB = N / NA;
+    __shared__ C buf[N * XL];
+    __shared__ C tws[N];
+    const int l = threadIdx.x % XL, g = threadIdx.x / XL;
+    const int iy = blockIdx.x, iz = blockIdx.y * XL + l;
+    const bool zin = iz < nz;
+    const size_t plane = (size_t)ny * nz, off = (size_t)iy * nz + (zin ? iz : 0);
+    const size_t band = blockIdx.z;
+    const bool mid = g < NA;
+    const __amdgpu_buffer_rsrc_t xr = make_rsrc(X + band * nxr * plane, (unsigned)(nxr * plane * sizeof(C)));
+    double2* boxb = (MODE != 0) ? box + band * (size_t)N * plane : nullptr;
+    // every global load is issued before the one wait at the barrier
+    unsigned qo[NA];   // byte offsets of x = NB n1 + g in this band's slots (BUF_OOB: no slot)
+    static_for<0, NA>([&](auto n1) {
+        const int q = xrun_slot(xrun, NB * APW_CI(n1) + g);
+        qo[APW_CI(n1)] = (zin && q >= 0) ? (unsigned)(((size_t)q * plane + off) * sizeof(C)) : BUF_OOB;
+    });
+    C v[NA], u[NB];
+    R ve[NB];
+    if constexpr (MODE != 2) {
+        static_for<0, NA>([&](auto n1) { v[APW_CI(n1)] = buf_load(xr, qo[APW_CI(n1)], (C*)nullptr); });
+    }
+    if constexpr (MODE == 0) {
+        if (mid) static_for<0, NB>([&](auto k2) { ve[APW_CI(k2)] = veff[(size_t)(g + NA * APW_CI(k2)) * plane + off]; });
+    }
+    if constexpr (MODE == 2) {
+        if (mid) {
+            static_for<0, NB>([&](auto k2) {
+                C t = to_c<R>(boxb[(size_t)(g + NA * APW_CI(k2)) * plane + off]);
+                if (!zin) t.x = t.y = 0;
+                u[APW_CI(k2)] = t;
+            });
+        }
+    }
+    for (int t = threadIdx.x; t < N; t += xreg_threads(N)) tws[t] = tw[t];
+    __syncthreads();
+    if constexpr (MODE != 2) {
+        rdft<C, R, NA, +1>(v);
+        static_for<1, NA>([&](auto k1) {
+            C w = tws[g * APW_CI(k1)];   // g k1 < N
+            w.y = -w.y;
+            v[APW_CI(k1)] = cmul(v[APW_CI(k1)], w);
+        });
+        static_for<0, NA>([&](auto k1) { buf[(APW_CI(k1) * NB + g) * XL + l] = v[APW_CI(k1)]; });
+        __syncthreads();
+        if (mid) {
+            static_for<0, NB>([&](auto n2) { u[APW_CI(n2)] = buf[(g * NB + APW_CI(n2)) * XL + l]; });
+            rdft<C, R, NB, +1>(u);
+        }
+        if constexpr (MODE == 1) {
+            if (mid && zin) {
+                static_for<0, NB>([&](auto k2) {
+                    double2* bp = boxb + (size_t)(g + NA * APW_CI(k2)) * plane + off;
+                    const C r = u[APW_CI(k2)];
+                    double2 o;
+                    if (add) {
+                        o = *bp;
+                        o.x += factor * (double)r.x;
+                        o.y += factor * (double)r.y;
+                    } else {
+                        o.x = (double)r.x;
+                        o.y = (double)r.y;
+                    }
+                    *bp = o;
+                });
+            }
+            return;
+        }
+        if (mid) {
+            static_for<0, NB>([&](auto k2) {
+                u[APW_CI(k2)].x *= ve[APW_CI(k2)];
+                u[APW_CI(k2)].y *= ve[APW_CI(k2)];
+            });
+        }
+        __syncthreads();   // every step-B read of buf is done
+    }
+    if (mid) {
+        rdft<C, R, NB, -1>(u);
+        static_for<1, NB>([&](auto m1) { u[APW_CI(m1)] = cmul(u[APW_CI(m1)], tws[g * APW_CI(m1)]); });
+        static_for<0, NB>([&](auto m1) { buf[(APW_CI(m1) * NA + g) * XL + l] = u[APW_CI(m1)]; });
+    }
+    __syncthreads();
+    static_for<0, NA>([&](auto k1) { v[APW_CI(k1)] = buf[(g * NA + APW_CI(k1)) * XL + l]; });
+    rdft<C, R, NA, -1>(v);
+    static_for<0, NA>([&](auto m2) { buf_store(v[APW_CI(m2)], xr, qo[APW_CI(m2)]); });
+}
+
+// ---- density pass, register form: rho (+)= sum_b w_b |x-backward(psi_b)|^2 ------------
+// grid (ny, ceil(nz / XL)); bands in order inside the workgroup, sums in registers.
+template <int N>
+__global__ __launch_bounds__(xreg_threads(N)) void k_xrho_reg(const cplx<double>* __restrict__ X, int nxr, int ny,
+                                                             int nz, int nb, int4 xrun, const double* __restrict__ w,
+                                                             double* __restrict__ rho, int add,
+                                                             const cplx<double>* __restrict__ tw) {
+    using C = cplx<double>;
+    constexpr int NA = ct_split_a(N), NB = N / NA;
+    __shared__ C buf[N * XL];
+    const int l = threadIdx.x % XL, g = threadIdx.x / XL;
+    const int iy = blockIdx.x, iz = blockIdx.y * XL + l;
+    const bool zin = iz < nz;
+    const size_t plane = (size_t)ny * nz, off = (size_t)iy * nz + (zin ? iz : 0);
+    const bool mid = g < NA;
+    unsigned qo[NA];
+    C twa[NA];
+    static_for<0, NA>([&](auto n1) {
+        const int q = xrun_slot(xrun, NB * APW_CI(n1) + g);
+        qo[APW_CI(n1)] = (zin && q >= 0) ? (unsigned)(((size_t)q * plane + off) * sizeof(C)) : BUF_OOB;
+    });
+    static_for<0, NA>([&](auto k1) {
+        C t = tw[g * APW_CI(k1)];
+        t.y = -t.y;
+        twa[APW_CI(k1)] = t;
+    });
+    double acc[NB];
+    static_for<0, NB>([&](auto k2) { acc[APW_CI(k2)] = 0.0; });
+    C v[NA], u[NB];
+    for (int b = 0; b < nb; ++b) {
+        const __amdgpu_buffer_rsrc_t xr = make_rsrc(X + (size_t)b * nxr * plane, (unsigned)(nxr * plane * sizeof(C)));
+        static_for<0, NA>([&](auto n1) { v[APW_CI(n1)] = buf_load(xr, qo[APW_CI(n1)], (C*)nullptr); });
+        rdft<C, double, NA, +1>(v);
+        static_for<1, NA>([&](auto k1) { v[APW_CI(k1)] = cmul(v[APW_CI(k1)], twa[APW_CI(k1)]); });
+        __syncthreads();   // the previous band's reads of buf are done
+        static_for<0, NA>([&](auto k1) { buf[(APW_CI(k1) * NB + g) * XL + l] = v[APW_CI(k1)]; });
+        __syncthreads();
+        if (mid) {
+            static_for<0, NB>([&](auto n2) { u[APW_CI(n2)] = buf[(g * NB + APW_CI(n2)) * XL + l]; });
+            rdft<C, double, NB, +1>(u);
+            const double wb = w[b];
+            static_for<0, NB>([&](auto k2) {
+                const C r = u[APW_CI(k2)];
+                acc[APW_CI(k2)] += wb * (r.x * r.x + r.y * r.y);
+            });
+        }
+    }
+    if (mid && zin) {
+        static_for<0, NB>([&](auto k2) {
+            double* rp = rho + (size_t)(g + NA * APW_CI(k2)) * plane + off;
+            *rp = add ? *rp + acc[APW_CI(k2)] : acc[APW_CI(k2)];
+        });
+    }
+}
+
+// ---- passes 2 and 4, register form: y transforms of one x slot, XL consecutive z -------
+// DIR +1: sticks -> X (every y); DIR -1: X -> the stick rows.  The slot's sticks are at most
+// two runs of consecutive y (yrun[slot], same convention as xrun_slot), numbered from
+// slot_start[slot] in y order.  Thread (l, g) as in k_xreg.  grid (nxr, ceil(nz / XL), nb)
+template <class R, int N, int DIR>
+__global__ __launch_bounds__(xreg_threads(N)) void k_yreg(const cplx<R>* __restrict__ in, cplx<R>* __restrict__ out,
+                                                         int ns, int nz, int nxr, const int* __restrict__ slot_start,
+                                                         const int4* __restrict__ yrun,
+                                                         const cplx<R>* __restrict__ tw) {
+    using C = cplx<R>;
+    constexpr int NA = ct_split_a(N), NB = N / NA;
+    __shared__ C buf[N * XL];
+    __shared__ C tws[N];
+    const int l = threadIdx.x % XL, g = threadIdx.x / XL;
+    const int slot = blockIdx.x, iz = blockIdx.y * XL + l;
+    const bool zin = iz < nz;
+    const int izc = zin ? iz : 0;
+    const size_t band = blockIdx.z;
+    const bool mid = g < NA;
+    const int st0 = slot_start[slot];
+    const int4 r = yrun[slot];
+    const size_t xoff = (band * nxr + slot) * (size_t)N * nz + izc;   // X at y = 0 of this line
+    unsigned so[NA];   // byte offsets of the sticks at y = NB n1 + g (BUF_OOB: none)
+    static_for<0, NA>([&](auto n1) {
+        const int q = xrun_slot(r, NB * APW_CI(n1) + g);
+        so[APW_CI(n1)] = (zin && q >= 0) ? (unsigned)(((size_t)(st0 + q) * nz + iz) * sizeof(C)) : BUF_OOB;
+    });
+    C v[NA], u[NB];
+    if constexpr (DIR > 0) {
+        const __amdgpu_buffer_rsrc_t sr = make_rsrc(in + band * ns * nz, (unsigned)((size_t)ns * nz * sizeof(C)));
+        static_for<0, NA>([&](auto n1) { v[APW_CI(n1)] = buf_load(sr, so[APW_CI(n1)], (C*)nullptr); });
+    } else {
+        if (mid) static_for<0, NB>([&](auto k2) { u[APW_CI(k2)] = in[xoff + (size_t)(g + NA * APW_CI(k2)) * nz]; });
+    }
+    for (int t = threadIdx.x; t < N; t += xreg_threads(N)) tws[t] = tw[t];
+    __syncthreads();
+    if constexpr (DIR > 0) {
+        rdft<C, R, NA, +1>(v);
+        static_for<1, NA>([&](auto k1) {
+            C w = tws[g * APW_CI(k1)];
+            w.y = -w.y;
+            v[APW_CI(k1)] = cmul(v[APW_CI(k1)], w);
+        });
+        static_for<0, NA>([&](auto k1) { buf[(APW_CI(k1) * NB + g) * XL + l] = v[APW_CI(k1)]; });
+        __syncthreads();
+        if (mid) {
+            static_for<0, NB>([&](auto n2) { u[APW_CI(n2)] = buf[(g * NB + APW_CI(n2)) * XL + l]; });
+            rdft<C, R, NB, +1>(u);
+            if (zin) static_for<0, NB>([&](auto k2) { out[xoff + (size_t)(g + NA * APW_CI(k2)) * nz] = u[APW_CI(k2)]; });
+        }
+    } else {
+        if (mid) {
+            rdft<C, R, NB, -1>(u);
+            static_for<1, NB>([&](auto m1) { u[APW_CI(m1)] = cmul(u[APW_CI(m1)], tws[g * APW_CI(m1)]); });
+            static_for<0, NB>([&](auto m1) { buf[(APW_CI(m1) * NA + g) * XL + l] = u[APW_CI(m1)]; });
+        }
+        __syncthreads();
+        static_for<0, NA>([&](auto k1) { v[APW_CI(k1)] = buf[(g * NA + APW_CI(k1)) * XL + l]; });
+        rdft<C, R, NA, -1>(v);
+        const __amdgpu_buffer_rsrc_t sr = make_rsrc(out + band * ns * nz, (unsigned)((size_t)ns * nz * sizeof(C)));
+        static_for<0, NA>([&](auto m2) { buf_store(v[APW_CI(m2)], sr, so[APW_CI(m2)]); });
+    }
+}
+
+// ---- passes 1 and 5, register form: z transforms of ZL sticks ---------------------------
+// Thread (g, l): g = threadIdx % NB runs along z (consecutive lanes, consecutive z), l =
+// stick.  DIR +1: psi (sphere, gathered through stick_ig) -> S; DIR -1: S -> hpsi with the
+// gather mode of k_zfwd.  LDS rows padded to odd strides (conflict-free 16-byte accesses).
+// grid (ceil(ns / ZL), nb)
+constexpr int ZL = 16;
+constexpr int zreg_threads(int n) { return ZL * (n / ct_split_a(n)); }
+constexpr int odd_up(int n) { return n | 1; }
+
+template <class R, int N, int DIR>
+__global__ __launch_bounds__(zreg_threads(N)) void k_zreg(const double2* __restrict__ psi, int ld,
+                                                         const int* __restrict__ stick_ig, int ns,
+                                                         cplx<R>* __restrict__ S, double2* __restrict__ hpsi, int ldh,
+                                                         const double* __restrict__ gk2, double tpiba2, int mode,
+                                                         double scale, const cplx<R>* __restrict__ tw) {
+    using C = cplx<R>;
+    constexpr int NA = ct_split_a(N), NB = N / NA;
+    constexpr int NBP = odd_up(NB), NAP = odd_up(NA);
+    constexpr int LDZ = odd_up(DIR > 0 ? NA * NBP : NB * NAP);
+    __shared__ C buf[ZL * LDZ];
+    __shared__ C tws[N];
+    const int g = threadIdx.x % NB, l = threadIdx.x / NB;
+    const int s = blockIdx.x * ZL + l;
+    const bool sin = s < ns;
+    const size_t band = blockIdx.y;
+    const bool mid = g < NA;
+    const int* sig = stick_ig + (size_t)(sin ? s : 0) * N;
+    C* Sl = S + (band * ns + (sin ? s : 0)) * (size_t)N;   // this stick's z line
+    C v[NA], u[NB];
+    unsigned po[NA];   // byte offsets into the band's sphere of z = NB n1 + g (BUF_OOB: no G)
+    static_for<0, NA>([&](auto n1) {
+        const int ig = sig[NB * APW_CI(n1) + g];
+        po[APW_CI(n1)] = (sin && ig >= 0) ? (unsigned)ig * 16u : BUF_OOB;
+    });
+    if constexpr (DIR > 0) {
+        const __amdgpu_buffer_rsrc_t pr = make_rsrc(psi + band * ld, (unsigned)((size_t)ld * 16));
+        static_for<0, NA>([&](auto n1) { v[APW_CI(n1)] = to_c<R>(buf_load(pr, po[APW_CI(n1)], (double2*)nullptr)); });
+    } else {
+        if (mid) static_for<0, NB>([&](auto k2) { u[APW_CI(k2)] = Sl[g + NA * APW_CI(k2)]; });
+    }
+    for (int t = threadIdx.x; t < N; t += zreg_threads(N)) tws[t] = tw[t];
+    __syncthreads();
+    C* row = buf + l * LDZ;
+    if constexpr (DIR > 0) {
+        rdft<C, R, NA, +1>(v);
+        static_for<1, NA>([&](auto k1) {
+            C w = tws[g * APW_CI(k1)];
+            w.y = -w.y;
+            v[APW_CI(k1)] = cmul(v[APW_CI(k1)], w);
+        });
+        static_for<0, NA>([&](auto k1) { row[APW_CI(k1) * NBP + g] = v[APW_CI(k1)]; });
+        __syncthreads();
+        if (mid) {
+            static_for<0, NB>([&](auto n2) { u[APW_CI(n2)] = row[g * NBP + APW_CI(n2)]; });
+            rdft<C, R, NB, +1>(u);
+            if (sin) static_for<0, NB>([&](auto k2) { Sl[g + NA * APW_CI(k2)] = u[APW_CI(k2)]; });
+        }
+    } else {
+        if (mid) {
+            rdft<C, R, NB, -1>(u);
+            static_for<1, NB>([&](auto m1) { u[APW_CI(m1)] = cmul(u[APW_CI(m1)], tws[g * APW_CI(m1)]); });
+            static_for<0, NB>([&](auto m1) { row[APW_CI(m1) * NAP + g] = u[APW_CI(m1)]; });
+        }
+        __syncthreads();
+        static_for<0, NA>([&](auto k1) { v[APW_CI(k1)] = row[g * NAP + APW_CI(k1)]; });
+        rdft<C, R, NA, -1>(v);
+        // z = g + NB m2 is the z of po[m2]
+        const __amdgpu_buffer_rsrc_t hr = make_rsrc(hpsi + band * ldh, (unsigned)((size_t)ldh * 16));
+        if (mode == GATHER_ADD) {
+            static_for<0, NA>([&](auto m2) {
+                double2 h = buf_load(hr, po[APW_CI(m2)], (double2*)nullptr);
+                h.x += scale * (double)v[APW_CI(m2)].x;
+                h.y += scale * (double)v[APW_CI(m2)].y;
+                buf_store(h, hr, po[APW_CI(m2)]);
+            });
+        } else if (mode == GATHER_SET_KIN) {
+            const __amdgpu_buffer_rsrc_t pr = make_rsrc(psi + band * ld, (unsigned)((size_t)ld * 16));
+            const __amdgpu_buffer_rsrc_t kr = make_rsrc(gk2, (unsigned)((size_t)ld * 8));
+            static_for<0, NA>([&](auto m2) {
+                const unsigned o = po[APW_CI(m2)];
+                const double2 p = buf_load(pr, o, (double2*)nullptr);
+                const double kin = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(kr, o == BUF_OOB ? o : o / 2u, 0, 0)) * tpiba2;
+                double2 h;
+                h.x = scale * (double)v[APW_CI(m2)].x + kin * p.x;
+                h.y = scale * (double)v[APW_CI(m2)].y + kin * p.y;
+                buf_store(h, hr, o);
+            });
+        } else {
+            static_for<0, NA>([&](auto m2) {
+                double2 h;
+                h.x = scale * (double)v[APW_CI(m2)].x;
+                h.y = scale * (double)v[APW_CI(m2)].y;
+                buf_store(h, hr, po[APW_CI(m2)]);
+            });
+        }
+    }
+}
+
 // ---- pass 4: y-forward, keeping the stick rows -----------------------------------------
 template <class R, int NF>
 __global__ __launch_bounds__(NT) void k_yfwd(const cplx<R>* __restrict__ X, int ns, int nz, int nxr,
@@ -599,6 +1099,18 @@ const cplx<R>* tw_ptr(const PwGrid& g, int off) {
         default: APW_DO_LAUNCH(0); break;    \
     }
 
+// x length for the register x pass; 0 (the run-time-plan kernel) when the x slots are more
+// than two runs or a band's slots exceed the 31-bit byte offsets of its buffer loads
+int xreg_len(const PwGrid& g) { return (g.xrun_ok && g.bytes_X(sizeof(double2)) < ((size_t)1 << 31)) ? g.nx : 0; }
+// y length for the register y passes (0: run-time-plan kernels): sticks of every slot in
+// at most two runs of y, a band's sticks within 31-bit byte offsets
+int yreg_len(const PwGrid& g) { return (g.yrun_ok && g.bytes_S(sizeof(double2)) < ((size_t)1 << 31)) ? g.ny : 0; }
+// z length for the register z passes: sphere offsets (16 B per plane wave) within 31 bits
+int zreg_len(const PwGrid& g, int ld, int ldh) {
+    const size_t m = (size_t)std::max(std::max(ld, ldh), g.npw) * 16;
+    return m < ((size_t)1 << 31) ? g.nz : 0;
+}
+
 template <class R>
 void pass_forward_half(hipStream_t s, const PwGrid& g, R* S, R* X, int nb, double2* hpsi, int ldh,
                        const double2* psi, int ld, const double* gk2, double tpiba2, GatherMode mode, double scale,
@@ -609,22 +1121,41 @@ void pass_forward_half(hipStream_t s, const PwGrid& g, R* S, R* X, int nb, doubl
     const double nSz = (double)g.ns * g.nz, nX = (double)g.nxr * g.ny * g.nz;
     {
         ProfScope ps(prof, s, "fft_y_fwd", nb * cs * (nX + nSz), nb * fft_flops(g.ny) * g.nxr * g.nz);
-#define APW_DO_LAUNCH(NF)                                                                                         \
-    hipLaunchKernelGGL((k_yfwd<R, NF>), dim3(g.nxr, (g.nz + Ly - 1) / Ly, nb), dim3(NT), lds_bytes<R>(g.ny, Ly), s, \
-                       reinterpret_cast<const C*>(X), g.ns, g.nz, g.nxr, g.slot_start.get(), g.slot_cnt.get(),     \
-                       g.stick_iy.get(), reinterpret_cast<C*>(S), g.py, tw_ptr<R>(g, g.twoff_y), Ly)
-        APW_DISPATCH(g.ny)
-#undef APW_DO_LAUNCH
+#define APW_YREG_LAUNCH(N)                                                                                    \
+    case N:                                                                                                  \
+        hipLaunchKernelGGL((k_yreg<R, N, -1>), dim3(g.nxr, (g.nz + XL - 1) / XL, nb), dim3(xreg_threads(N)), 0, \
+                           s, reinterpret_cast<const C*>(X), reinterpret_cast<C*>(S), g.ns, g.nz, g.nxr,         \
+                           g.slot_start.get(), g.yrun.get(), tw_ptr<R>(g, g.twoff_y));                       \
+        break;
+        switch (yreg_len(g)) {
+            APW_FFT_SIZES(APW_YREG_LAUNCH)
+            default:
+                hipLaunchKernelGGL((k_yfwd<R, 0>), dim3(g.nxr, (g.nz + Ly - 1) / Ly, nb), dim3(NT),
+                                   lds_bytes<R>(g.ny, Ly), s, reinterpret_cast<const C*>(X), g.ns, g.nz, g.nxr,
+                                   g.slot_start.get(), g.slot_cnt.get(), g.stick_iy.get(), reinterpret_cast<C*>(S),
+                                   g.py, tw_ptr<R>(g, g.twoff_y), Ly);
+                break;
+        }
+#undef APW_YREG_LAUNCH
         APW_LAUNCH_CHECK();
     }
     const double pwio = 16.0 * g.npw * ((mode == GATHER_SET) ? 1 : 2);
     ProfScope ps(prof, s, "fft_z_fwd_gather", nb * (cs * nSz + pwio), nb * (fft_flops(g.nz) * g.ns + 4.0 * g.npw));
-#define APW_DO_LAUNCH(NF)                                                                                           \
-    hipLaunchKernelGGL((k_zfwd<R, NF>), dim3((g.ns + Lz - 1) / Lz, nb), dim3(NT), lds_bytes<R>(g.nz, Lz), s,          \
-                       reinterpret_cast<const C*>(S), g.stick_ig.get(), g.ns, hpsi, ldh, psi, ld, gk2, tpiba2,       \
-                       (int)mode, scale, g.pz, tw_ptr<R>(g, g.twoff_z), Lz)
-    APW_DISPATCH(g.nz)
-#undef APW_DO_LAUNCH
+#define APW_ZREG_LAUNCH(N)                                                                                    \
+    case N:                                                                                                  \
+        hipLaunchKernelGGL((k_zreg<R, N, -1>), dim3((g.ns + ZL - 1) / ZL, nb), dim3(zreg_threads(N)), 0, s, psi, \
+                           ld, g.stick_ig.get(), g.ns, reinterpret_cast<C*>(S), hpsi, ldh, gk2, tpiba2, (int)mode, \
+                           scale, tw_ptr<R>(g, g.twoff_z));                                                  \
+        break;
+    switch (zreg_len(g, ld, ldh)) {
+        APW_FFT_SIZES(APW_ZREG_LAUNCH)
+        default:
+            hipLaunchKernelGGL((k_zfwd<R, 0>), dim3((g.ns + Lz - 1) / Lz, nb), dim3(NT), lds_bytes<R>(g.nz, Lz), s,
+                               reinterpret_cast<const C*>(S), g.stick_ig.get(), g.ns, hpsi, ldh, psi, ld, gk2, tpiba2,
+                               (int)mode, scale, g.pz, tw_ptr<R>(g, g.twoff_z), Lz);
+            break;
+    }
+#undef APW_ZREG_LAUNCH
     APW_LAUNCH_CHECK();
 }
 
@@ -636,20 +1167,40 @@ void pass_backward_half(hipStream_t s, const PwGrid& g, const double2* psi, int 
     const double nSz = (double)g.ns * g.nz, nX = (double)g.nxr * g.ny * g.nz;
     {
         ProfScope ps(prof, s, "fft_z_bwd_scatter", nb * (16.0 * g.npw + cs * nSz), nb * fft_flops(g.nz) * g.ns);
-#define APW_DO_LAUNCH(NF)                                                                                        \
-    hipLaunchKernelGGL((k_zbwd<R, NF>), dim3((g.ns + Lz - 1) / Lz, nb), dim3(NT), lds_bytes<R>(g.nz, Lz), s, psi, \
-                       ld, g.stick_ig.get(), g.ns, reinterpret_cast<C*>(S), g.pz, tw_ptr<R>(g, g.twoff_z), Lz)
-        APW_DISPATCH(g.nz)
-#undef APW_DO_LAUNCH
+#define APW_ZREG_LAUNCH(N)                                                                                    \
+    case N:                                                                                                  \
+        hipLaunchKernelGGL((k_zreg<R, N, 1>), dim3((g.ns + ZL - 1) / ZL, nb), dim3(zreg_threads(N)), 0, s, psi,  \
+                           ld, g.stick_ig.get(), g.ns, reinterpret_cast<C*>(S), (double2*)nullptr, 0,          \
+                           (const double*)nullptr, 0.0, 0, 0.0, tw_ptr<R>(g, g.twoff_z));                    \
+        break;
+        switch (zreg_len(g, ld, 0)) {
+            APW_FFT_SIZES(APW_ZREG_LAUNCH)
+            default:
+                hipLaunchKernelGGL((k_zbwd<R, 0>), dim3((g.ns + Lz - 1) / Lz, nb), dim3(NT), lds_bytes<R>(g.nz, Lz),
+                                   s, psi, ld, g.stick_ig.get(), g.ns, reinterpret_cast<C*>(S), g.pz,
+                                   tw_ptr<R>(g, g.twoff_z), Lz);
+                break;
+        }
+#undef APW_ZREG_LAUNCH
         APW_LAUNCH_CHECK();
     }
     ProfScope ps(prof, s, "fft_y_bwd", nb * cs * (nSz + nX), nb * fft_flops(g.ny) * g.nxr * g.nz);
-#define APW_DO_LAUNCH(NF)                                                                                         \
-    hipLaunchKernelGGL((k_ybwd<R, NF>), dim3(g.nxr, (g.nz + Ly - 1) / Ly, nb), dim3(NT), lds_bytes<R>(g.ny, Ly), s, \
-                       reinterpret_cast<const C*>(S), g.ns, g.nz, g.nxr, g.slot_start.get(), g.slot_cnt.get(),     \
-                       g.stick_iy.get(), reinterpret_cast<C*>(X), g.py, tw_ptr<R>(g, g.twoff_y), Ly)
-    APW_DISPATCH(g.ny)
-#undef APW_DO_LAUNCH
+#define APW_YREG_LAUNCH(N)                                                                                    \
+    case N:                                                                                                  \
+        hipLaunchKernelGGL((k_yreg<R, N, 1>), dim3(g.nxr, (g.nz + XL - 1) / XL, nb), dim3(xreg_threads(N)), 0,  \
+                           s, reinterpret_cast<const C*>(S), reinterpret_cast<C*>(X), g.ns, g.nz, g.nxr,         \
+                           g.slot_start.get(), g.yrun.get(), tw_ptr<R>(g, g.twoff_y));                       \
+        break;
+    switch (yreg_len(g)) {
+        APW_FFT_SIZES(APW_YREG_LAUNCH)
+        default:
+            hipLaunchKernelGGL((k_ybwd<R, 0>), dim3(g.nxr, (g.nz + Ly - 1) / Ly, nb), dim3(NT), lds_bytes<R>(g.ny, Ly),
+                               s, reinterpret_cast<const C*>(S), g.ns, g.nz, g.nxr, g.slot_start.get(),
+                               g.slot_cnt.get(), g.stick_iy.get(), reinterpret_cast<C*>(X), g.py,
+                               tw_ptr<R>(g, g.twoff_y), Ly);
+            break;
+    }
+#undef APW_YREG_LAUNCH
     APW_LAUNCH_CHECK();
 }
 
@@ -697,6 +1248,39 @@ void PwGrid::build(int nx_, int ny_, int nz_, int npw_, const int* ig2ixyz) {
         h_slot_cnt.back()++;
     }
     nxr = (int)h_slot_ix.size();
+    // the slots as runs of consecutive x (a sphere gives one or two)
+    std::vector<int> run_start, run_len;
+    for (int q = 0; q < nxr; ++q) {
+        if (q == 0 || h_slot_ix[q] != h_slot_ix[q - 1] + 1) {
+            run_start.push_back(h_slot_ix[q]);
+            run_len.push_back(0);
+        }
+        run_len.back()++;
+    }
+    xrun_ok = run_start.size() <= 2;
+    xrun = make_int4(run_start[0], run_len[0], run_start.size() > 1 ? run_start[1] : 0,
+                     run_start.size() > 1 ? run_len[1] : 0);
+    // the sticks of each slot as runs of consecutive y
+    std::vector<int4> h_yrun(nxr);
+    yrun_ok = true;
+    for (int q = 0; q < nxr; ++q) {
+        int nrun = 0;
+        int st[2] = {0, 0}, ln[2] = {0, 0};
+        for (int k = 0; k < h_slot_cnt[q]; ++k) {
+            const int iy = h_iy[h_slot_start[q] + k];
+            if (k == 0 || iy != h_iy[h_slot_start[q] + k - 1] + 1) {
+                if (nrun == 2) {
+                    yrun_ok = false;
+                    break;
+                }
+                st[nrun++] = iy;
+            }
+            ln[nrun - 1]++;
+        }
+        h_yrun[q] = make_int4(st[0], ln[0], st[1], ln[1]);
+    }
+    yrun.reserve(nxr);
+    APW_HIP(hipMemcpy(yrun.get(), h_yrun.data(), nxr * sizeof(int4), hipMemcpyHostToDevice));
     auto up = [](DevBuf<int>& d, const std::vector<int>& h) {
         d.reserve(h.size());
         APW_HIP(hipMemcpy(d.get(), h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice));
@@ -743,12 +1327,22 @@ void local_hpsi(hipStream_t s, const PwGrid& g, const double* psi, int ld, doubl
         const double nX = (double)g.nxr * g.ny * g.nz, nxyz = (double)g.nx * g.ny * g.nz;
         ProfScope ps(prof, s, "fft_x_veff_fused", nb * (2.0 * sizeof(C) * nX + sizeof(R) * nxyz),
                      nb * (2.0 * fft_flops(g.nx) * g.ny * g.nz + 2.0 * nxyz));
-#define APW_DO_LAUNCH(NF)                                                                                        \
-    hipLaunchKernelGGL((k_xpass<R, 0, NF>), dim3(g.ny, (g.nz + Lx - 1) / Lx, nb), dim3(NT), lds_bytes<R>(g.nx, Lx), \
-                       s, reinterpret_cast<C*>(X), g.nxr, g.ny, g.nz, g.slot_ix.get(), veff, (double2*)nullptr, 0,  \
-                       0.0, g.px, tw_ptr<R>(g, g.twoff_x), Lx)
-        APW_DISPATCH(g.nx)
-#undef APW_DO_LAUNCH
+#define APW_XREG_LAUNCH(N)                                                                                     \
+    case N:                                                                                                   \
+        hipLaunchKernelGGL((k_xreg<R, N, 0>), dim3(g.ny, (g.nz + XL - 1) / XL, nb), dim3(xreg_threads(N)), 0, s, \
+                           reinterpret_cast<C*>(X), g.nxr, g.ny, g.nz, g.xrun, veff, (double2*)nullptr, 0,  \
+                           0.0, tw_ptr<R>(g, g.twoff_x));                                                     \
+        break;
+        switch (xreg_len(g)) {
+            APW_FFT_SIZES(APW_XREG_LAUNCH)
+            default:
+                hipLaunchKernelGGL((k_xpass<R, 0, 0>), dim3(g.ny, (g.nz + Lx - 1) / Lx, nb), dim3(NT),
+                                   lds_bytes<R>(g.nx, Lx), s, reinterpret_cast<C*>(X), g.nxr, g.ny, g.nz,
+                                   g.slot_ix.get(), veff, (double2*)nullptr, 0, 0.0, g.px, tw_ptr<R>(g, g.twoff_x),
+                                   Lx);
+                break;
+        }
+#undef APW_XREG_LAUNCH
         APW_LAUNCH_CHECK();
     }
     pass_forward_half<R>(s, g, S, X, nb, reinterpret_cast<double2*>(hpsi), ldh, p2, ld, gk2, tpiba2, mode, scale, prof);
@@ -764,10 +1358,22 @@ void sphere_to_box(hipStream_t s, const PwGrid& g, const double* psi, int ld, do
     using C = cplx<double>;
     pass_backward_half<double>(s, g, reinterpret_cast<const double2*>(psi), ld, S, X, nb, nullptr);
     const int Lx = lines_for(g.nx);
-    hipLaunchKernelGGL((k_xpass<double, 1, 0>), dim3(g.ny, (g.nz + Lx - 1) / Lx, nb), dim3(NT),
-                       lds_bytes<double>(g.nx, Lx), s, reinterpret_cast<C*>(X), g.nxr, g.ny, g.nz, g.slot_ix.get(),
-                       (const double*)nullptr, reinterpret_cast<double2*>(box), add ? 1 : 0, factor, g.px,
-                       tw_ptr<double>(g, g.twoff_x), Lx);
+#define APW_XREG_LAUNCH(N)                                                                                         \
+    case N:                                                                                                       \
+        hipLaunchKernelGGL((k_xreg<double, N, 1>), dim3(g.ny, (g.nz + XL - 1) / XL, nb), dim3(xreg_threads(N)), 0, \
+                           s, reinterpret_cast<C*>(X), g.nxr, g.ny, g.nz, g.xrun, (const double*)nullptr,   \
+                           reinterpret_cast<double2*>(box), add ? 1 : 0, factor, tw_ptr<double>(g, g.twoff_x));    \
+        break;
+    switch (xreg_len(g)) {
+        APW_FFT_SIZES(APW_XREG_LAUNCH)
+        default:
+            hipLaunchKernelGGL((k_xpass<double, 1, 0>), dim3(g.ny, (g.nz + Lx - 1) / Lx, nb), dim3(NT),
+                               lds_bytes<double>(g.nx, Lx), s, reinterpret_cast<C*>(X), g.nxr, g.ny, g.nz,
+                               g.slot_ix.get(), (const double*)nullptr, reinterpret_cast<double2*>(box), add ? 1 : 0,
+                               factor, g.px, tw_ptr<double>(g, g.twoff_x), Lx);
+            break;
+    }
+#undef APW_XREG_LAUNCH
     APW_LAUNCH_CHECK();
 }
 
@@ -775,10 +1381,23 @@ void box_to_sphere(hipStream_t s, const PwGrid& g, const double* box, double* hp
                    GatherMode mode, double scale, double* S, double* X) {
     using C = cplx<double>;
     const int Lx = lines_for(g.nx);
-    hipLaunchKernelGGL((k_xpass<double, 2, 0>), dim3(g.ny, (g.nz + Lx - 1) / Lx, nb), dim3(NT),
-                       lds_bytes<double>(g.nx, Lx), s, reinterpret_cast<C*>(X), g.nxr, g.ny, g.nz, g.slot_ix.get(),
-                       (const double*)nullptr, const_cast<double2*>(reinterpret_cast<const double2*>(box)), 0, 0.0,
-                       g.px, tw_ptr<double>(g, g.twoff_x), Lx);
+    double2* bx = const_cast<double2*>(reinterpret_cast<const double2*>(box));
+#define APW_XREG_LAUNCH(N)                                                                                         \
+    case N:                                                                                                       \
+        hipLaunchKernelGGL((k_xreg<double, N, 2>), dim3(g.ny, (g.nz + XL - 1) / XL, nb), dim3(xreg_threads(N)), 0, \
+                           s, reinterpret_cast<C*>(X), g.nxr, g.ny, g.nz, g.xrun, (const double*)nullptr,   \
+                           bx, 0, 0.0, tw_ptr<double>(g, g.twoff_x));                                              \
+        break;
+    switch (xreg_len(g)) {
+        APW_FFT_SIZES(APW_XREG_LAUNCH)
+        default:
+            hipLaunchKernelGGL((k_xpass<double, 2, 0>), dim3(g.ny, (g.nz + Lx - 1) / Lx, nb), dim3(NT),
+                               lds_bytes<double>(g.nx, Lx), s, reinterpret_cast<C*>(X), g.nxr, g.ny, g.nz,
+                               g.slot_ix.get(), (const double*)nullptr, bx, 0, 0.0, g.px, tw_ptr<double>(g, g.twoff_x),
+                               Lx);
+            break;
+    }
+#undef APW_XREG_LAUNCH
     APW_LAUNCH_CHECK();
     pass_forward_half<double>(s, g, S, X, nb, reinterpret_cast<double2*>(hpsi), ldh, nullptr, 0, nullptr, 0.0, mode,
                               scale, nullptr);
@@ -795,14 +1414,23 @@ void psi_to_rho(hipStream_t s, const PwGrid& g, const double* psi, int ld, int n
     ProfScope ps(prof, s, "fft_x_rho", nb * sizeof(C) * nX + (add ? 16.0 : 8.0) * nxyz,
                  nb * (fft_flops(g.nx) * g.ny * g.nz + 3.0 * nxyz));
     const size_t lds = lds_bytes<double>(g.nx, Lx) + (size_t)g.nx * Lx * sizeof(double);
-#define APW_DO_LAUNCH(NF)                                                                                       \
-    APW_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_xrho<NF>),                                     \
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));                         \
-    hipLaunchKernelGGL((k_xrho<NF>), dim3(g.ny, (g.nz + Lx - 1) / Lx), dim3(NT), lds, s,                       \
-                       reinterpret_cast<const C*>(X), g.nxr, g.ny, g.nz, nb, g.slot_ix.get(), w, rho, add ? 1 : 0, \
-                       g.px, tw_ptr<double>(g, g.twoff_x), Lx)
-    APW_DISPATCH(g.nx)
-#undef APW_DO_LAUNCH
+#define APW_XREG_LAUNCH(N)                                                                                    \
+    case N:                                                                                                  \
+        hipLaunchKernelGGL((k_xrho_reg<N>), dim3(g.ny, (g.nz + XL - 1) / XL), dim3(xreg_threads(N)), 0, s,       \
+                           reinterpret_cast<const C*>(X), g.nxr, g.ny, g.nz, nb, g.xrun, w, rho, add ? 1 : 0, \
+                           tw_ptr<double>(g, g.twoff_x));                                                    \
+        break;
+    switch (xreg_len(g)) {
+        APW_FFT_SIZES(APW_XREG_LAUNCH)
+        default:
+            APW_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_xrho<0>),
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+            hipLaunchKernelGGL((k_xrho<0>), dim3(g.ny, (g.nz + Lx - 1) / Lx), dim3(NT), lds, s,
+                               reinterpret_cast<const C*>(X), g.nxr, g.ny, g.nz, nb, g.slot_ix.get(), w, rho,
+                               add ? 1 : 0, g.px, tw_ptr<double>(g, g.twoff_x), Lx);
+            break;
+    }
+#undef APW_XREG_LAUNCH
     APW_LAUNCH_CHECK();
 }
 

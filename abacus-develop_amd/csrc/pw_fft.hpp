@@ -33,6 +33,11 @@ struct PwGrid {
     DevBuf<int> slot_ix;      // nxr
     DevBuf<int> slot_start;   // nxr (first stick of the slot; sticks sorted by ix, iy)
     DevBuf<int> slot_cnt;     // nxr
+    int4 xrun{};              // slots as runs of x: slot = x - xrun.x on [xrun.x, +xrun.y),
+                              // xrun.y + x - xrun.z on [xrun.z, +xrun.w)
+    bool xrun_ok = false;     // the slots are at most two runs
+    DevBuf<int4> yrun;        // nxr: each slot's sticks as runs of y (same convention)
+    bool yrun_ok = false;     // every slot's sticks are at most two runs
     DevBuf<double> tw64;      // twiddles exp(-2 pi i k / n) for x, y, z (c128), offsets below
     DevBuf<float> tw32;
     int twoff_x = 0, twoff_y = 0, twoff_z = 0;

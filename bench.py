@@ -323,7 +323,7 @@ def main():
     fb = flops_per_band(s)
     it_m, eig_m, st_m = mixed
     roof, _ = roofline_from_profile(prof)
-    top = sorted(prof.items(), key=lambda kv: -kv[1]["ms"])[:8]
+    top = sorted(prof.items(), key=lambda kv: -kv[1]["ms"])
     out = {
         "metric": METRIC,
         "value": round(sum(times) / len(times), 4),
@@ -352,7 +352,7 @@ def main():
         "max_abs_eig_diff_mixed_vs_fp64_Ry": float(np.abs(eig_m - eig64).max()),
         "warmup_s": [round(t, 4) for t in warm],
         "roofline": roof,
-        "rooflines": [_roofline(k, v) for k, v in sorted(prof.items(), key=lambda kv: -kv[1]["ms"])[:8]],
+        "rooflines": [_roofline(k, v) for k, v in top[:8]],
         "kernels": {k: {"launches": v["launches"], "ms": round(v["ms"], 3),
                         "avg_us": round(1e3 * v["ms"] / max(1, v["launches"]), 2),
                         "GB_s": round(v["bytes"] / max(v["ms"], 1e-9) / 1e6, 1),
