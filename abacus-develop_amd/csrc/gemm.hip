@@ -15,6 +15,8 @@
 // 256-byte coalesced runs.  Accumulator register r of lane l holds D row drow(l, r) and
 // column l % 16; drow is (l / 16) + 4 r for the f64 instruction and 4 (l / 16) + r for the
 // f32 one (checked on the device by the layout probe below and its unit test).
+#include <cstdio>
+#include <cstdlib>
 #include <type_traits>
 #include <utility>
 
@@ -134,6 +136,15 @@ __global__ __launch_bounds__(NT) void zgemm_kernel(int M, int N, int K, int kchu
             okb |= (unsigned)ok << i;
         }
     };
+    // LDS store of the prefetched elements: range mask and conjugation as bit operations
+    // (no branches), so the stores can be scheduled among the MFMAs of the k step
+    using bits_t = typename std::conditional<sizeof(R) == 8, unsigned long long, unsigned>::type;
+    constexpr bits_t SIGN = (bits_t)1 << (8 * sizeof(R) - 1);
+    auto masked = [](R v, bool ok, bool neg) {
+        bits_t b = __builtin_bit_cast(bits_t, v);
+        if (neg) b ^= SIGN;
+        return __builtin_bit_cast(R, b & ((bits_t)0 - (bits_t)ok));
+    };
     auto stage = [&](int st) {
 #pragma unroll
         for (int i = 0; i < NA; ++i) {
@@ -141,8 +152,8 @@ __global__ __launch_bounds__(NT) void zgemm_kernel(int M, int N, int K, int kchu
             int mm, kk;
             if (TA == 0) { mm = e % BM; kk = e / BM; } else { kk = e % BK; mm = e / BK; }
             const bool ok = (oka >> i) & 1u;
-            As[st][0][kk][mm] = ok ? ra[i].x : R(0);
-            As[st][1][kk][mm] = ok ? (TA == 2 ? -ra[i].y : ra[i].y) : R(0);
+            As[st][0][kk][mm] = masked(ra[i].x, ok, false);
+            As[st][1][kk][mm] = masked(ra[i].y, ok, TA == 2);
         }
 #pragma unroll
         for (int i = 0; i < NB; ++i) {
@@ -150,15 +161,15 @@ __global__ __launch_bounds__(NT) void zgemm_kernel(int M, int N, int K, int kchu
             int nn, kk;
             if (TB == 0) { kk = e % BK; nn = e / BK; } else { nn = e % BN; kk = e / BN; }
             const bool ok = (okb >> i) & 1u;
-            Bs[st][0][kk][nn] = ok ? rb[i].x : R(0);
-            Bs[st][1][kk][nn] = ok ? (TB == 2 ? -rb[i].y : rb[i].y) : R(0);
+            Bs[st][0][kk][nn] = masked(rb[i].x, ok, false);
+            Bs[st][1][kk][nn] = masked(rb[i].y, ok, TB == 2);
         }
     };
 
     if (kbeg < kend) {
         fetch(kbeg);
         stage(0);
-        if (kbeg + BK < kend) fetch(kbeg + BK);
+        fetch(kbeg + BK);
     }
     __syncthreads();
     int st = 0;
@@ -189,10 +200,10 @@ __global__ __launch_bounds__(NT) void zgemm_kernel(int M, int N, int K, int kchu
                     acc_im[tn][tm] = Mfma<R>::mma(xi[tn], yr[tm], acc_im[tn][tm]);
                 }
         }
-        if (k0 + BK < kend) {
-            stage(st ^ 1);
-            if (k0 + 2 * BK < kend) fetch(k0 + 2 * BK);
-        }
+        // unconditional (one basic block with the MFMAs): past the end the masks are zero and
+        // the loads read element 0
+        stage(st ^ 1);
+        fetch(k0 + 2 * BK);
         __syncthreads();
         st ^= 1;
     }
@@ -332,25 +343,30 @@ void gemm(hipStream_t s, GemmWorkspace& ws, int num_cu, char ta, char tb, int m,
     // C, else 64 x 64.  Updates (no split): 128 x 64 or 128 x 32 for tall C (m >> n; the
     // 32-wide tile unless n is a multiple of 64: less waste in the last tile column, more
     // workgroups) in the XCD-aware order, 64 x 128 for wide, else 64 x 64.
-    const int target = 2 * std::max(num_cu, 64);
-    auto split_count = [&](int bm, int bn) {
-        const long tiles = (long)((m + bm - 1) / bm) * ((n + bn - 1) / bn);
-        int sp = 1;
-        if (tiles < target) {
-            sp = (int)((target + tiles - 1) / tiles);
-            sp = std::min(sp, std::max(1, k / 512));
-            sp = std::min(sp, 256);
-        }
-        return sp;
+    // Split count: every workgroup of a split launch is resident at once (floor of the
+    // resident slots over the tiles; a launch with a partial last round of workgroups ran
+    // up to 1.5x longer), with at least 64 of k per split.
+    static const int occ_env = [] {
+        const char* e = std::getenv("APW_GEMM_WG_PER_CU");   // tuning override
+        return e ? std::atoi(e) : 0;
+    }();
+    const int cus = std::max(num_cu, 64);
+    auto tiles_of = [&](int bm, int bn) { return (long)((m + bm - 1) / bm) * ((n + bn - 1) / bn); };
+    auto split_count = [&](int bm, int bn, int occ) {
+        const long tiles = tiles_of(bm, bn), slots = (long)(occ_env > 0 ? occ_env : occ) * cus;
+        int sp = tiles < slots ? (int)(slots / tiles) : 1;
+        sp = std::min(sp, std::max(1, k / 64));
+        return std::max(1, std::min(sp, 256));
     };
     const bool tall = m >= 4 * n && m >= 256, wide = n >= 4 * m && n >= 256;
-    const bool inner = split_count(64, 64) > 1;
+    const bool inner = tiles_of(64, 64) < 2 * cus && k / 64 > 1;
     int variant;   // 0: 2x2 map0, 1: 2x1 map0, 2: 4x2 map1, 3: 4x2 map2, 4: 4x1 map2, 5: 2x4 bk16 map0
     if (inner) variant = n <= 32 ? 1 : (tall ? 2 : 0);
     else if (tall) variant = n % 64 == 0 ? 3 : 4;
     else variant = wide ? 5 : 0;
     static const int VBM[6] = {64, 64, 128, 128, 128, 64}, VBN[6] = {64, 32, 64, 64, 32, 128};
-    int splits = split_count(VBM[variant], VBN[variant]);
+    static const int VOCC[6] = {4, 6, 3, 3, 3, 2};   // workgroups per CU (LDS and registers)
+    int splits = inner ? split_count(VBM[variant], VBN[variant], VOCC[variant]) : 1;
     int kchunk = (k + splits - 1) / splits;
     kchunk = (kchunk + BKMAX - 1) / BKMAX * BKMAX;
     splits = (k + kchunk - 1) / kchunk;
@@ -360,6 +376,10 @@ void gemm(hipStream_t s, GemmWorkspace& ws, int num_cu, char ta, char tb, int m,
         ws.part.reserve(need);
         part = reinterpret_cast<R*>(ws.part.get());
     }
+    static const bool trace = std::getenv("APW_GEMM_TRACE") != nullptr;   // tuning aid: one line per call
+    if (trace)
+        std::fprintf(stderr, "apw_gemm %s %c%c m=%d n=%d k=%d variant=%d splits=%d\n",
+                     std::is_same<R, double>::value ? "f64" : "f32", ta, tb, m, n, k, variant, splits);
     const double esz = 2.0 * sizeof(R);
     const bool beta_nz = (beta_re != 0.0 || beta_im != 0.0);
     const char* pname = std::is_same<R, double>::value ? (splits > 1 ? "zgemm_f64_splitk" : "zgemm_f64_tiled")

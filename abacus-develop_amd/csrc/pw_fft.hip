@@ -11,10 +11,15 @@
 //   4. y-forward, keeping the stick rows;
 //   5. z-forward on the sticks, scaled by 1/nxyz and gathered into hpsi, with the kinetic
 //      term gk2*tpiba2*psi fused in.
-// Each pass is a batch of 1D transforms done in LDS by a mixed-radix (4, 2, 3, 5)
-// Stockham FFT (self-sorting, so no bit reversal); lines are interleaved in LDS so the
-// global loads/stores are runs of consecutive z (16 x 16 B) or whole sticks.
-// Twiddles exp(-2 pi i k / n) are computed on the host in long double.
+// Each pass is a batch of 1D transforms.  For the lengths compiled in (APW_FFT_SIZES) a
+// length N = NA * NB transform runs as a four-step in registers: NB DFTs of length NA per
+// line in the threads' registers, twiddles, one exchange through LDS, NA DFTs of length NB,
+// every index and inner constant fixed at compile time; so one line crosses LDS once per
+// direction, and the fused x pass twice in all.  Global accesses are runs of consecutive z
+// (16 lines x 16 B) or of one stick, and the sphere's holes are read as zeros by raw
+// buffer loads (no branches).  Other 2-3-5 lengths run a mixed-radix (4, 2, 3, 5) Stockham
+// FFT in LDS with the plan read at run time.  Twiddles exp(-2 pi i k / n) are computed on
+// the host in long double.
 #include "pw_fft.hpp"
 
 #include <algorithm>
@@ -167,29 +172,11 @@ __device__ inline cplx<R> to_c(double2 v) {
 
 constexpr int NT = 256;
 
-// ---- compile-time plans ----------------------------------------------------------------
-// Lengths with a dedicated instantiation (every FFT length of the named workloads); other
-// 2-3-5 lengths run the same kernels with the plan read at run time.
+// ---- compile-time lengths ---------------------------------------------------------------
+// Lengths with register-resident kernels (every FFT length of the named workloads); other
+// 2-3-5 lengths run the LDS Stockham kernels with the plan read at run time.
 #define APW_FFT_SIZES(X) X(12) X(16) X(18) X(20) X(24) X(25) X(27) X(30) X(32) X(36) X(40) X(45) \
     X(48) X(50) X(54) X(60) X(64) X(72) X(80) X(90) X(96) X(100) X(120) X(128) X(144) X(160)
-
-constexpr int ct_nstages(int n) {
-    int c = 0;
-    while (n % 4 == 0) { n /= 4; ++c; }
-    while (n % 2 == 0) { n /= 2; ++c; }
-    while (n % 3 == 0) { n /= 3; ++c; }
-    while (n % 5 == 0) { n /= 5; ++c; }
-    return c;
-}
-
-constexpr int ct_radix(int n, int st) {
-    int c = 0;
-    while (n % 4 == 0) { n /= 4; if (c++ == st) return 4; }
-    while (n % 2 == 0) { n /= 2; if (c++ == st) return 2; }
-    while (n % 3 == 0) { n /= 3; if (c++ == st) return 3; }
-    while (n % 5 == 0) { n /= 5; if (c++ == st) return 5; }
-    return 1;
-}
 
 // lines per workgroup: two LDS buffers of n * (L+1) complex doubles within ~64 KB
 constexpr int lines_for(int n) {
@@ -198,55 +185,6 @@ constexpr int lines_for(int n) {
     return L;
 }
 
-// Stockham stages with the length, radices and direction known at compile time: every
-// index division and twiddle stride is a constant.  Returns the buffer holding the result.
-template <class R, int N, int ST, int NS, int SG>
-__device__ inline cplx<R>* ct_stages(cplx<R>* x, cplx<R>* y, const cplx<R>* tw, int ld) {
-    using C = cplx<R>;
-    if constexpr (ST == ct_nstages(N)) {
-        return x;
-    } else {
-        constexpr int RAD = ct_radix(N, ST);
-        constexpr int NB = N / RAD;
-        constexpr int STRIDE = N / (NS * RAD);
-        constexpr int L = lines_for(N);
-        for (int b = threadIdx.x; b < NB * L; b += NT) {
-            const int l = b % L;
-            const int j = b / L;
-            const int jm = j % NS;
-            C v[RAD];
-#pragma unroll
-            for (int r = 0; r < RAD; ++r) v[r] = x[(j + r * NB) * ld + l];
-            if constexpr (NS > 1) {
-#pragma unroll
-                for (int r = 1; r < RAD; ++r) {
-                    C t = tw[r * jm * STRIDE];
-                    if (SG > 0) t.y = -t.y;
-                    v[r] = cmul(v[r], t);
-                }
-            }
-            dft_radix<C, R>(v, RAD, SG);
-            const int base = (j / NS) * NS * RAD + jm;
-#pragma unroll
-            for (int r = 0; r < RAD; ++r) y[(base + r * NS) * ld + l] = v[r];
-        }
-        __syncthreads();
-        return ct_stages<R, N, ST + 1, NS * RAD, SG>(y, x, tw, ld);
-    }
-}
-
-// FFT of the L lines in LDS: compile-time path when NF != 0, run-time plan otherwise
-template <class R, int NF, int SG>
-__device__ inline cplx<R>* fft_lines(cplx<R>* x, cplx<R>* y, const cplx<R>* tw, const FftPlan& p, int L, int ld) {
-    if constexpr (NF != 0) {
-        return ct_stages<R, NF, 0, 1, SG>(x, y, tw, ld);
-    } else {
-        return lds_fft<R>(x, y, tw, p, L, ld, SG);
-    }
-}
-
-#define APW_LEN(NF, p) ((NF) != 0 ? (NF) : (p).n)
-#define APW_LINES(NF, Lrt) ((NF) != 0 ? lines_for(NF) : (Lrt))
 
 // ---- register-resident transforms ------------------------------------------------------
 // A length-P DFT held in one thread's registers, with every index and constant fixed at
@@ -394,14 +332,14 @@ APW_FFT_SIZES(APW_XREG_CHECK)
 
 // ---- pass 1: z-backward on sticks, gathered from the sphere --------------------------
 // grid (ceil(ns/L), nb); LDS: 2 * nz * (L+1) + nz complex
-template <class R, int NF>
+template <class R>
 __global__ __launch_bounds__(NT) void k_zbwd(const double2* __restrict__ psi, int ld, const int* __restrict__ stick_ig,
                                              int ns, cplx<R>* __restrict__ S, FftPlan pz,
                                              const cplx<R>* __restrict__ tw, int Lrt) {
     using C = cplx<R>;
     extern __shared__ __align__(16) unsigned char smem[];
-    const int nz = APW_LEN(NF, pz);
-    const int L = APW_LINES(NF, Lrt);
+    const int nz = pz.n;
+    const int L = Lrt;
     const int ldl = L + 1;
     C* x = reinterpret_cast<C*>(smem);
     C* y = x + nz * ldl;
@@ -423,7 +361,7 @@ __global__ __launch_bounds__(NT) void k_zbwd(const double2* __restrict__ psi, in
         x[iz * ldl + l] = v;
     }
     __syncthreads();
-    C* res = fft_lines<R, NF, +1>(x, y, tws, pz, L, ldl);
+    C* res = lds_fft<R>(x, y, tws, pz, L, ldl, +1);
     C* Sb = S + (size_t)band * ns * nz;
     for (int t = threadIdx.x; t < L * nz; t += NT) {
         const int l = t / nz, iz = t - (t / nz) * nz;
@@ -434,15 +372,15 @@ __global__ __launch_bounds__(NT) void k_zbwd(const double2* __restrict__ psi, in
 
 // ---- pass 2: y-backward on the x slots -----------------------------------------------
 // grid (nxr, ceil(nz/L), nb); LDS 2 * ny * (L+1) + ny
-template <class R, int NF>
+template <class R>
 __global__ __launch_bounds__(NT) void k_ybwd(const cplx<R>* __restrict__ S, int ns, int nz, int nxr,
                                              const int* __restrict__ slot_start, const int* __restrict__ slot_cnt,
                                              const int* __restrict__ stick_iy, cplx<R>* __restrict__ X, FftPlan py,
                                              const cplx<R>* __restrict__ tw, int Lrt) {
     using C = cplx<R>;
     extern __shared__ __align__(16) unsigned char smem[];
-    const int ny = APW_LEN(NF, py);
-    const int L = APW_LINES(NF, Lrt);
+    const int ny = py.n;
+    const int L = Lrt;
     const int ldl = L + 1;
     C* x = reinterpret_cast<C*>(smem);
     C* y = x + ny * ldl;
@@ -467,7 +405,7 @@ __global__ __launch_bounds__(NT) void k_ybwd(const cplx<R>* __restrict__ S, int 
         if (iz < nz) x[stick_iy[s] * ldl + dz] = Sb[(size_t)s * nz + iz];
     }
     __syncthreads();
-    C* res = fft_lines<R, NF, +1>(x, y, tws, py, L, ldl);
+    C* res = lds_fft<R>(x, y, tws, py, L, ldl, +1);
     C* Xb = X + ((size_t)band * nxr + slot) * ny * nz;
     for (int t = threadIdx.x; t < ny * L; t += NT) {
         const int iy = t / L, dz = t - (t / L) * L;
@@ -478,15 +416,15 @@ __global__ __launch_bounds__(NT) void k_ybwd(const cplx<R>* __restrict__ S, int 
 
 // ---- pass 3: x pass.  MODE 0: bwd, *veff, fwd (X -> X); 1: bwd, write full box; 2: read full box, fwd
 // grid (ny, ceil(nz/L), nb); LDS 2 * nx * (L+1) + nx
-template <class R, int MODE, int NF>
+template <class R, int MODE>
 __global__ __launch_bounds__(NT) void k_xpass(cplx<R>* __restrict__ X, int nxr, int ny, int nz,
                                               const int* __restrict__ slot_ix, const R* __restrict__ veff,
                                               double2* __restrict__ box, int add, double factor, FftPlan px,
                                               const cplx<R>* __restrict__ tw, int Lrt) {
     using C = cplx<R>;
     extern __shared__ __align__(16) unsigned char smem[];
-    const int nx = APW_LEN(NF, px);
-    const int L = APW_LINES(NF, Lrt);
+    const int nx = px.n;
+    const int L = Lrt;
     const int ldl = L + 1;
     C* x = reinterpret_cast<C*>(smem);
     C* y = x + nx * ldl;
@@ -524,7 +462,7 @@ __global__ __launch_bounds__(NT) void k_xpass(cplx<R>* __restrict__ X, int nxr, 
     }
     __syncthreads();
     C* res = x;
-    if (MODE != 2) res = fft_lines<R, NF, +1>(x, y, tws, px, L, ldl);
+    if (MODE != 2) res = lds_fft<R>(x, y, tws, px, L, ldl, +1);
     if (MODE == 1) {
         for (int t = threadIdx.x; t < nx * L; t += NT) {
             const int ix = t / L, dz = t - (t / L) * L;
@@ -561,7 +499,7 @@ __global__ __launch_bounds__(NT) void k_xpass(cplx<R>* __restrict__ X, int nxr, 
         __syncthreads();
     }
     C* other = (res == x) ? y : x;
-    C* out = fft_lines<R, NF, -1>(res, other, tws, px, L, ldl);
+    C* out = lds_fft<R>(res, other, tws, px, L, ldl, -1);
     for (int t = threadIdx.x; t < nxr * L; t += NT) {
         const int q = t / L, dz = t - (t / L) * L;
         const int iz = iz0 + dz;
@@ -573,15 +511,14 @@ __global__ __launch_bounds__(NT) void k_xpass(cplx<R>* __restrict__ X, int nxr, 
 // grid (ny, ceil(nz/L)); each workgroup owns lines (iy, iz0 .. iz0+L) for all nb bands and
 // accumulates in LDS in band order (deterministic, no atomics), then adds to rho once.
 // LDS 2 * nx * (L+1) + nx complex + nx * L doubles
-template <int NF>
 __global__ __launch_bounds__(NT) void k_xrho(const cplx<double>* __restrict__ X, int nxr, int ny, int nz, int nb,
                                              const int* __restrict__ slot_ix, const double* __restrict__ w,
                                              double* __restrict__ rho, int add, FftPlan px,
                                              const cplx<double>* __restrict__ tw, int Lrt) {
     using C = cplx<double>;
     extern __shared__ __align__(16) unsigned char smem[];
-    const int nx = APW_LEN(NF, px);
-    const int L = APW_LINES(NF, Lrt);
+    const int nx = px.n;
+    const int L = Lrt;
     const int ldl = L + 1;
     C* x = reinterpret_cast<C*>(smem);
     C* y = x + nx * ldl;
@@ -607,7 +544,7 @@ __global__ __launch_bounds__(NT) void k_xrho(const cplx<double>* __restrict__ X,
             if (iz < nz) x[slot_ix[q] * ldl + dz] = Xb[((size_t)q * ny + iy) * nz + iz];
         }
         __syncthreads();
-        const C* res = fft_lines<double, NF, +1>(x, y, tws, px, L, ldl);
+        const C* res = lds_fft<double>(x, y, tws, px, L, ldl, +1);
         const double wb = w[b];
         for (int t = threadIdx.x; t < nx * L; t += NT) {
             const int ix = t / L, dz = t - (t / L) * L;
@@ -793,10 +730,17 @@ __global__ __launch_bounds__(xreg_threads(N)) void k_xrho_reg(const cplx<double>
     });
     double acc[NB];
     static_for<0, NB>([&](auto k2) { acc[APW_CI(k2)] = 0.0; });
-    C v[NA], u[NB];
-    for (int b = 0; b < nb; ++b) {
-        const __amdgpu_buffer_rsrc_t xr = make_rsrc(X + (size_t)b * nxr * plane, (unsigned)(nxr * plane * sizeof(C)));
+    C v[NA], vn[NA], u[NB];
+    const unsigned band_bytes = (unsigned)(nxr * plane * sizeof(C));
+    if (nb > 0) {
+        const __amdgpu_buffer_rsrc_t xr = make_rsrc(X, band_bytes);
         static_for<0, NA>([&](auto n1) { v[APW_CI(n1)] = buf_load(xr, qo[APW_CI(n1)], (C*)nullptr); });
+    }
+    for (int b = 0; b < nb; ++b) {
+        if (b + 1 < nb) {   // the next band's loads fly while this band is transformed
+            const __amdgpu_buffer_rsrc_t xr = make_rsrc(X + (size_t)(b + 1) * nxr * plane, band_bytes);
+            static_for<0, NA>([&](auto n1) { vn[APW_CI(n1)] = buf_load(xr, qo[APW_CI(n1)], (C*)nullptr); });
+        }
         rdft<C, double, NA, +1>(v);
         static_for<1, NA>([&](auto k1) { v[APW_CI(k1)] = cmul(v[APW_CI(k1)], twa[APW_CI(k1)]); });
         __syncthreads();   // the previous band's reads of buf are done
@@ -811,6 +755,7 @@ __global__ __launch_bounds__(xreg_threads(N)) void k_xrho_reg(const cplx<double>
                 acc[APW_CI(k2)] += wb * (r.x * r.x + r.y * r.y);
             });
         }
+        static_for<0, NA>([&](auto n1) { v[APW_CI(n1)] = vn[APW_CI(n1)]; });
     }
     if (mid && zin) {
         static_for<0, NB>([&](auto k2) {
@@ -918,11 +863,26 @@ __global__ __launch_bounds__(zreg_threads(N)) void k_zreg(const double2* __restr
         const int ig = sig[NB * APW_CI(n1) + g];
         po[APW_CI(n1)] = (sin && ig >= 0) ? (unsigned)ig * 16u : BUF_OOB;
     });
+    double2 add[NA];   // DIR -1: what the result is added to (kinetic term or old hpsi)
+    double kin[NA];
+    const __amdgpu_buffer_rsrc_t hr = make_rsrc(hpsi + band * ldh, (unsigned)((size_t)ldh * 16));
     if constexpr (DIR > 0) {
         const __amdgpu_buffer_rsrc_t pr = make_rsrc(psi + band * ld, (unsigned)((size_t)ld * 16));
         static_for<0, NA>([&](auto n1) { v[APW_CI(n1)] = to_c<R>(buf_load(pr, po[APW_CI(n1)], (double2*)nullptr)); });
     } else {
         if (mid) static_for<0, NB>([&](auto k2) { u[APW_CI(k2)] = Sl[g + NA * APW_CI(k2)]; });
+        if (mode == GATHER_ADD) {
+            static_for<0, NA>([&](auto m2) { add[APW_CI(m2)] = buf_load(hr, po[APW_CI(m2)], (double2*)nullptr); });
+        } else if (mode == GATHER_SET_KIN) {
+            const __amdgpu_buffer_rsrc_t pr = make_rsrc(psi + band * ld, (unsigned)((size_t)ld * 16));
+            const __amdgpu_buffer_rsrc_t kr = make_rsrc(gk2, (unsigned)((size_t)ld * 8));
+            static_for<0, NA>([&](auto m2) {
+                const unsigned o = po[APW_CI(m2)];
+                add[APW_CI(m2)] = buf_load(pr, o, (double2*)nullptr);
+                kin[APW_CI(m2)] = __builtin_bit_cast(
+                    double, __builtin_amdgcn_raw_buffer_load_b64(kr, o == BUF_OOB ? o : o / 2u, 0, 0));
+            });
+        }
     }
     for (int t = threadIdx.x; t < N; t += zreg_threads(N)) tws[t] = tw[t];
     __syncthreads();
@@ -951,25 +911,21 @@ __global__ __launch_bounds__(zreg_threads(N)) void k_zreg(const double2* __restr
         static_for<0, NA>([&](auto k1) { v[APW_CI(k1)] = row[g * NAP + APW_CI(k1)]; });
         rdft<C, R, NA, -1>(v);
         // z = g + NB m2 is the z of po[m2]
-        const __amdgpu_buffer_rsrc_t hr = make_rsrc(hpsi + band * ldh, (unsigned)((size_t)ldh * 16));
         if (mode == GATHER_ADD) {
             static_for<0, NA>([&](auto m2) {
-                double2 h = buf_load(hr, po[APW_CI(m2)], (double2*)nullptr);
+                double2 h = add[APW_CI(m2)];
                 h.x += scale * (double)v[APW_CI(m2)].x;
                 h.y += scale * (double)v[APW_CI(m2)].y;
                 buf_store(h, hr, po[APW_CI(m2)]);
             });
         } else if (mode == GATHER_SET_KIN) {
-            const __amdgpu_buffer_rsrc_t pr = make_rsrc(psi + band * ld, (unsigned)((size_t)ld * 16));
-            const __amdgpu_buffer_rsrc_t kr = make_rsrc(gk2, (unsigned)((size_t)ld * 8));
             static_for<0, NA>([&](auto m2) {
-                const unsigned o = po[APW_CI(m2)];
-                const double2 p = buf_load(pr, o, (double2*)nullptr);
-                const double kin = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(kr, o == BUF_OOB ? o : o / 2u, 0, 0)) * tpiba2;
+                const double k = kin[APW_CI(m2)] * tpiba2;
+                const double2 p = add[APW_CI(m2)];
                 double2 h;
-                h.x = scale * (double)v[APW_CI(m2)].x + kin * p.x;
-                h.y = scale * (double)v[APW_CI(m2)].y + kin * p.y;
-                buf_store(h, hr, o);
+                h.x = scale * (double)v[APW_CI(m2)].x + k * p.x;
+                h.y = scale * (double)v[APW_CI(m2)].y + k * p.y;
+                buf_store(h, hr, po[APW_CI(m2)]);
             });
         } else {
             static_for<0, NA>([&](auto m2) {
@@ -983,15 +939,15 @@ __global__ __launch_bounds__(zreg_threads(N)) void k_zreg(const double2* __restr
 }
 
 // ---- pass 4: y-forward, keeping the stick rows -----------------------------------------
-template <class R, int NF>
+template <class R>
 __global__ __launch_bounds__(NT) void k_yfwd(const cplx<R>* __restrict__ X, int ns, int nz, int nxr,
                                              const int* __restrict__ slot_start, const int* __restrict__ slot_cnt,
                                              const int* __restrict__ stick_iy, cplx<R>* __restrict__ S, FftPlan py,
                                              const cplx<R>* __restrict__ tw, int Lrt) {
     using C = cplx<R>;
     extern __shared__ __align__(16) unsigned char smem[];
-    const int ny = APW_LEN(NF, py);
-    const int L = APW_LINES(NF, Lrt);
+    const int ny = py.n;
+    const int L = Lrt;
     const int ldl = L + 1;
     C* x = reinterpret_cast<C*>(smem);
     C* y = x + ny * ldl;
@@ -1011,7 +967,7 @@ __global__ __launch_bounds__(NT) void k_yfwd(const cplx<R>* __restrict__ X, int 
         x[iy * ldl + dz] = v;
     }
     __syncthreads();
-    C* res = fft_lines<R, NF, -1>(x, y, tws, py, L, ldl);
+    C* res = lds_fft<R>(x, y, tws, py, L, ldl, -1);
     const int st0 = slot_start[slot], cnt = slot_cnt[slot];
     C* Sb = S + (size_t)band * ns * nz;
     for (int t = threadIdx.x; t < cnt * L; t += NT) {
@@ -1023,15 +979,15 @@ __global__ __launch_bounds__(NT) void k_yfwd(const cplx<R>* __restrict__ X, int 
 }
 
 // ---- pass 5: z-forward on sticks, gathered into hpsi ----------------------------------
-template <class R, int NF>
+template <class R>
 __global__ __launch_bounds__(NT) void k_zfwd(const cplx<R>* __restrict__ S, const int* __restrict__ stick_ig, int ns,
                                              double2* __restrict__ hpsi, int ldh, const double2* __restrict__ psi,
                                              int ld, const double* __restrict__ gk2, double tpiba2, int mode,
                                              double scale, FftPlan pz, const cplx<R>* __restrict__ tw, int Lrt) {
     using C = cplx<R>;
     extern __shared__ __align__(16) unsigned char smem[];
-    const int nz = APW_LEN(NF, pz);
-    const int L = APW_LINES(NF, Lrt);
+    const int nz = pz.n;
+    const int L = Lrt;
     const int ldl = L + 1;
     C* x = reinterpret_cast<C*>(smem);
     C* y = x + nz * ldl;
@@ -1050,7 +1006,7 @@ __global__ __launch_bounds__(NT) void k_zfwd(const cplx<R>* __restrict__ S, cons
         x[iz * ldl + l] = v;
     }
     __syncthreads();
-    C* res = fft_lines<R, NF, -1>(x, y, tws, pz, L, ldl);
+    C* res = lds_fft<R>(x, y, tws, pz, L, ldl, -1);
     double2* hb = hpsi + (size_t)band * ldh;
     const double2* pb = psi ? psi + (size_t)band * ld : nullptr;
     for (int t = threadIdx.x; t < L * nz; t += NT) {
@@ -1090,15 +1046,6 @@ const cplx<R>* tw_ptr(const PwGrid& g, int off) {
     return reinterpret_cast<const cplx<R>*>(g.tw32.get() + 2 * off);
 }
 
-// launch KERNEL<..., NF> with NF = n when n has a dedicated instantiation, else NF = 0
-#define APW_CASE_LAUNCH(N) \
-    case N: APW_DO_LAUNCH(N); break;
-#define APW_DISPATCH(n)                      \
-    switch (n) {                             \
-        APW_FFT_SIZES(APW_CASE_LAUNCH)       \
-        default: APW_DO_LAUNCH(0); break;    \
-    }
-
 // x length for the register x pass; 0 (the run-time-plan kernel) when the x slots are more
 // than two runs or a band's slots exceed the 31-bit byte offsets of its buffer loads
 int xreg_len(const PwGrid& g) { return (g.xrun_ok && g.bytes_X(sizeof(double2)) < ((size_t)1 << 31)) ? g.nx : 0; }
@@ -1130,7 +1077,7 @@ void pass_forward_half(hipStream_t s, const PwGrid& g, R* S, R* X, int nb, doubl
         switch (yreg_len(g)) {
             APW_FFT_SIZES(APW_YREG_LAUNCH)
             default:
-                hipLaunchKernelGGL((k_yfwd<R, 0>), dim3(g.nxr, (g.nz + Ly - 1) / Ly, nb), dim3(NT),
+                hipLaunchKernelGGL((k_yfwd<R>), dim3(g.nxr, (g.nz + Ly - 1) / Ly, nb), dim3(NT),
                                    lds_bytes<R>(g.ny, Ly), s, reinterpret_cast<const C*>(X), g.ns, g.nz, g.nxr,
                                    g.slot_start.get(), g.slot_cnt.get(), g.stick_iy.get(), reinterpret_cast<C*>(S),
                                    g.py, tw_ptr<R>(g, g.twoff_y), Ly);
@@ -1150,7 +1097,7 @@ void pass_forward_half(hipStream_t s, const PwGrid& g, R* S, R* X, int nb, doubl
     switch (zreg_len(g, ld, ldh)) {
         APW_FFT_SIZES(APW_ZREG_LAUNCH)
         default:
-            hipLaunchKernelGGL((k_zfwd<R, 0>), dim3((g.ns + Lz - 1) / Lz, nb), dim3(NT), lds_bytes<R>(g.nz, Lz), s,
+            hipLaunchKernelGGL((k_zfwd<R>), dim3((g.ns + Lz - 1) / Lz, nb), dim3(NT), lds_bytes<R>(g.nz, Lz), s,
                                reinterpret_cast<const C*>(S), g.stick_ig.get(), g.ns, hpsi, ldh, psi, ld, gk2, tpiba2,
                                (int)mode, scale, g.pz, tw_ptr<R>(g, g.twoff_z), Lz);
             break;
@@ -1176,7 +1123,7 @@ void pass_backward_half(hipStream_t s, const PwGrid& g, const double2* psi, int 
         switch (zreg_len(g, ld, 0)) {
             APW_FFT_SIZES(APW_ZREG_LAUNCH)
             default:
-                hipLaunchKernelGGL((k_zbwd<R, 0>), dim3((g.ns + Lz - 1) / Lz, nb), dim3(NT), lds_bytes<R>(g.nz, Lz),
+                hipLaunchKernelGGL((k_zbwd<R>), dim3((g.ns + Lz - 1) / Lz, nb), dim3(NT), lds_bytes<R>(g.nz, Lz),
                                    s, psi, ld, g.stick_ig.get(), g.ns, reinterpret_cast<C*>(S), g.pz,
                                    tw_ptr<R>(g, g.twoff_z), Lz);
                 break;
@@ -1194,7 +1141,7 @@ void pass_backward_half(hipStream_t s, const PwGrid& g, const double2* psi, int 
     switch (yreg_len(g)) {
         APW_FFT_SIZES(APW_YREG_LAUNCH)
         default:
-            hipLaunchKernelGGL((k_ybwd<R, 0>), dim3(g.nxr, (g.nz + Ly - 1) / Ly, nb), dim3(NT), lds_bytes<R>(g.ny, Ly),
+            hipLaunchKernelGGL((k_ybwd<R>), dim3(g.nxr, (g.nz + Ly - 1) / Ly, nb), dim3(NT), lds_bytes<R>(g.ny, Ly),
                                s, reinterpret_cast<const C*>(S), g.ns, g.nz, g.nxr, g.slot_start.get(),
                                g.slot_cnt.get(), g.stick_iy.get(), reinterpret_cast<C*>(X), g.py,
                                tw_ptr<R>(g, g.twoff_y), Ly);
@@ -1336,7 +1283,7 @@ void local_hpsi(hipStream_t s, const PwGrid& g, const double* psi, int ld, doubl
         switch (xreg_len(g)) {
             APW_FFT_SIZES(APW_XREG_LAUNCH)
             default:
-                hipLaunchKernelGGL((k_xpass<R, 0, 0>), dim3(g.ny, (g.nz + Lx - 1) / Lx, nb), dim3(NT),
+                hipLaunchKernelGGL((k_xpass<R, 0>), dim3(g.ny, (g.nz + Lx - 1) / Lx, nb), dim3(NT),
                                    lds_bytes<R>(g.nx, Lx), s, reinterpret_cast<C*>(X), g.nxr, g.ny, g.nz,
                                    g.slot_ix.get(), veff, (double2*)nullptr, 0, 0.0, g.px, tw_ptr<R>(g, g.twoff_x),
                                    Lx);
@@ -1367,7 +1314,7 @@ void sphere_to_box(hipStream_t s, const PwGrid& g, const double* psi, int ld, do
     switch (xreg_len(g)) {
         APW_FFT_SIZES(APW_XREG_LAUNCH)
         default:
-            hipLaunchKernelGGL((k_xpass<double, 1, 0>), dim3(g.ny, (g.nz + Lx - 1) / Lx, nb), dim3(NT),
+            hipLaunchKernelGGL((k_xpass<double, 1>), dim3(g.ny, (g.nz + Lx - 1) / Lx, nb), dim3(NT),
                                lds_bytes<double>(g.nx, Lx), s, reinterpret_cast<C*>(X), g.nxr, g.ny, g.nz,
                                g.slot_ix.get(), (const double*)nullptr, reinterpret_cast<double2*>(box), add ? 1 : 0,
                                factor, g.px, tw_ptr<double>(g, g.twoff_x), Lx);
@@ -1391,7 +1338,7 @@ void box_to_sphere(hipStream_t s, const PwGrid& g, const double* box, double* hp
     switch (xreg_len(g)) {
         APW_FFT_SIZES(APW_XREG_LAUNCH)
         default:
-            hipLaunchKernelGGL((k_xpass<double, 2, 0>), dim3(g.ny, (g.nz + Lx - 1) / Lx, nb), dim3(NT),
+            hipLaunchKernelGGL((k_xpass<double, 2>), dim3(g.ny, (g.nz + Lx - 1) / Lx, nb), dim3(NT),
                                lds_bytes<double>(g.nx, Lx), s, reinterpret_cast<C*>(X), g.nxr, g.ny, g.nz,
                                g.slot_ix.get(), (const double*)nullptr, bx, 0, 0.0, g.px, tw_ptr<double>(g, g.twoff_x),
                                Lx);
@@ -1423,9 +1370,9 @@ void psi_to_rho(hipStream_t s, const PwGrid& g, const double* psi, int ld, int n
     switch (xreg_len(g)) {
         APW_FFT_SIZES(APW_XREG_LAUNCH)
         default:
-            APW_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_xrho<0>),
+            APW_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_xrho),
                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-            hipLaunchKernelGGL((k_xrho<0>), dim3(g.ny, (g.nz + Lx - 1) / Lx), dim3(NT), lds, s,
+            hipLaunchKernelGGL(k_xrho, dim3(g.ny, (g.nz + Lx - 1) / Lx), dim3(NT), lds, s,
                                reinterpret_cast<const C*>(X), g.nxr, g.ny, g.nz, nb, g.slot_ix.get(), w, rho,
                                add ? 1 : 0, g.px, tw_ptr<double>(g, g.twoff_x), Lx);
             break;
