@@ -365,7 +365,7 @@ void gemm(hipStream_t s, GemmWorkspace& ws, int num_cu, char ta, char tb, int m,
     else if (tall) variant = n % 64 == 0 ? 3 : 4;
     else variant = wide ? 5 : 0;
     static const int VBM[6] = {64, 64, 128, 128, 128, 64}, VBN[6] = {64, 32, 64, 64, 32, 128};
-    static const int VOCC[6] = {4, 6, 3, 3, 3, 2};   // workgroups per CU (LDS and registers)
+    static const int VOCC[6] = {3, 6, 3, 3, 3, 2};   // resident workgroups per CU (measured best)
     int splits = inner ? split_count(VBM[variant], VBN[variant], VOCC[variant]) : 1;
     int kchunk = (k + splits - 1) / splits;
     kchunk = (kchunk + BKMAX - 1) / BKMAX * BKMAX;

@@ -13,6 +13,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import re
 import math
 import os
 import sys
@@ -52,14 +53,16 @@ def run_solve(pw, ctx, ham, s, psi0_dev, precision):
 
 
 # library profiler name -> kernel-name prefixes in the rocprofv3 traffic summary
-_PMC_FAMILY = {
-    "eig_tridiag": ("apw::k_tridiag_reg", "apw::k_tridiag_lds"),
-    "zgemm_f64_tiled": ("apw::zgemm_kernel<double",), "zgemm_f64_splitk": ("apw::zgemm_kernel<double",),
-    "cgemm_f32_tiled": ("apw::zgemm_kernel<float",), "cgemm_f32_splitk": ("apw::zgemm_kernel<float",),
-    "fft_x_veff_fused": ("apw::k_xpass",), "fft_y_bwd": ("apw::k_ybwd",), "fft_y_fwd": ("apw::k_yfwd",),
-    "fft_z_bwd_scatter": ("apw::k_zbwd",), "fft_z_fwd_gather": ("apw::k_zfwd",),
-    "eig_bisect": ("apw::k_bisect",), "eig_invit": ("apw::k_invit",),
-    "eig_backtransform": ("apw::k_backtransform",), "chol_inv": ("apw::k_chol_inv",),
+_PMC_FAMILY = {   # regular expressions over the kernel names of the committed traffic summary
+    "eig_tridiag": r"apw::k_tridiag_(reg|lds)",
+    "zgemm_f64_tiled": r"apw::zgemm_kernel<double", "zgemm_f64_splitk": r"apw::zgemm_kernel<double",
+    "cgemm_f32_tiled": r"apw::zgemm_kernel<float", "cgemm_f32_splitk": r"apw::zgemm_kernel<float",
+    "fft_x_veff_fused": r"apw::k_xreg<\w+, \d+, 0>|apw::k_xpass<\w+, 0>",
+    "fft_y_bwd": r"apw::k_yreg<\w+, \d+, 1>|apw::k_ybwd", "fft_y_fwd": r"apw::k_yreg<\w+, \d+, -1>|apw::k_yfwd",
+    "fft_z_bwd_scatter": r"apw::k_zreg<\w+, \d+, 1>|apw::k_zbwd",
+    "fft_z_fwd_gather": r"apw::k_zreg<\w+, \d+, -1>|apw::k_zfwd",
+    "eig_bisect": r"apw::k_bisect", "eig_invit": r"apw::k_invit",
+    "eig_backtransform": r"apw::k_backtransform", "chol_inv": r"apw::k_chol_inv",
 }
 # kernels whose time is set by dependent latency chains, not by a byte or flop roof
 _LATENCY_BOUND = {
@@ -79,7 +82,7 @@ def _pmc_traffic(name):
         ks = json.load(f)["kernels"]
     tot = cnt = 0.0
     for k, v in ks.items():
-        if k.startswith(_PMC_FAMILY[name]):
+        if re.match(_PMC_FAMILY[name], k):
             tot += (v["fetch_bytes"] + v["write_bytes"]) * v["dispatches"]
             cnt += v["dispatches"]
     return round(tot / cnt) if cnt else None
