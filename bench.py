@@ -55,8 +55,9 @@ def run_solve(pw, ctx, ham, s, psi0_dev, precision):
 # library profiler name -> kernel-name prefixes in the rocprofv3 traffic summary
 _PMC_FAMILY = {   # regular expressions over the kernel names of the committed traffic summary
     "eig_tridiag": r"apw::k_tridiag_(reg|lds)",
-    "zgemm_f64_tiled": r"apw::zgemm_kernel<double", "zgemm_f64_splitk": r"apw::zgemm_kernel<double",
-    "cgemm_f32_tiled": r"apw::zgemm_kernel<float", "cgemm_f32_splitk": r"apw::zgemm_kernel<float",
+    # GEMMs: the path's updates are op(A) = N (tiled), its inner products op(A) = C (split-K)
+    "zgemm_f64_tiled": r"apw::zgemm_kernel<double, 0, ", "zgemm_f64_splitk": r"apw::zgemm_kernel<double, [12], ",
+    "cgemm_f32_tiled": r"apw::zgemm_kernel<float, 0, ", "cgemm_f32_splitk": r"apw::zgemm_kernel<float, [12], ",
     "fft_x_veff_fused": r"apw::k_xreg<\w+, \d+, 0>|apw::k_xpass<\w+, 0>",
     "fft_y_bwd": r"apw::k_yreg<\w+, \d+, 1>|apw::k_ybwd", "fft_y_fwd": r"apw::k_yreg<\w+, \d+, -1>|apw::k_yfwd",
     "fft_z_bwd_scatter": r"apw::k_zreg<\w+, \d+, 1>|apw::k_zbwd",
