@@ -131,6 +131,57 @@ def test_fft3d_vs_numpy(torch, ctx, shape):
     assert np.abs(host(ctx, box) - ref2).max() < 1e-13 * np.abs(ref2).max()
 
 
+# every length with a register-resident kernel, in each axis (x and y passes: box <-> X;
+# z passes: sticks), against numpy's FFT
+_REG_LENGTHS = [12, 16, 18, 20, 24, 25, 27, 30, 32, 36, 40, 45, 48, 50, 54, 60, 64, 72, 80, 90, 96, 100, 120, 128,
+                144, 160]
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_fft3d_register_lengths(torch, ctx, axis):
+    from apw import pw
+    for n in _REG_LENGTHS:
+        shape = [8, 12, 10]
+        shape[axis] = n
+        rng = np.random.default_rng(1000 * axis + n)
+        x = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+        box = dev(torch, x.reshape(-1))
+        pw.fft3d(ctx, *shape, box, +1)
+        ref = np.fft.ifftn(x) * x.size
+        got = host(ctx, box).reshape(shape)
+        assert np.abs(got - ref).max() < 1e-13 * np.abs(ref).max(), (axis, n)
+        pw.fft3d(ctx, *shape, box, -1)
+        assert np.abs(host(ctx, box).reshape(shape) - np.fft.fftn(ref)).max() < 1e-13 * np.abs(ref).max() * x.size
+
+
+@pytest.mark.parametrize("even_x", [False, True])
+def test_hpsi_scattered_sphere(torch, ctx, even_x):
+    """H|psi> on an irregular set of plane waves: the y columns of an x slot are not one or
+    two runs (and with even x only, neither are the x slots), so the run-time-plan
+    kernels take those passes; GPU vs oracle."""
+    import copy
+    import dataclasses
+    from apw import pw
+    s0 = configs.make("si8")
+    rng = np.random.default_rng(11 + even_x)
+    nx, ny, nz = s0.nx, s0.ny, s0.nz
+    ix = rng.integers(0, nx // 2, size=4 * s0.npw) * 2 if even_x else rng.integers(0, nx, size=4 * s0.npw)
+    idx = (ix * ny + rng.integers(0, ny, size=ix.size)) * nz + rng.integers(0, nz, size=ix.size)
+    idx = np.unique(idx)
+    idx = rng.permutation(idx)[: s0.npw].astype(np.int32)
+    b = copy.copy(s0.basis)
+    b.ig2ixyz_k = s0.basis.ig2ixyz_k.copy()
+    b.ig2ixyz_k[s0.ik, : s0.npw] = idx
+    s = dataclasses.replace(s0, basis=b)
+    ham = pw.HamiltPW(ctx, s)
+    nb = 4
+    X = _rand_psi(s, nb, 5)
+    got = host(ctx, ham.hPsi(dev(torch, X), nb))[:, : s.npw]
+    ref = O.HamiltOracle(s).hpsi(X[:, : s.npw].reshape(-1), nb, s.npw).reshape(nb, s.npw)
+    assert np.abs(got - ref).max() < 1e-12 * np.abs(ref).max()
+    ham.close()
+
+
 def test_fft_chain_reference_fixture(torch, ctx):
     """pw_op_test box -> backward FFT reproduces veff_op_test's input (12^3 grid)."""
     from apw import pw
