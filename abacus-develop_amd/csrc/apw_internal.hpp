@@ -102,7 +102,8 @@ struct Prof {
     std::deque<ProfEntry> entries;   // deque: references stay valid while entries are added
     std::vector<hipEvent_t> pool;
     ProfEntry& entry(const char* name);
-    hipEvent_t event();
+    hipEvent_t event();                 // from the pool (created only when it is empty)
+    void prefill(size_t n);             // create events up front, so timed code creates none
     void resolve();
     void reset();
     ~Prof();
@@ -173,6 +174,9 @@ template <class R>
 void gemm(hipStream_t s, GemmWorkspace& ws, int num_cu, char ta, char tb, int m, int n, int k,
           double alpha_re, double alpha_im, const R* A, int lda, const R* B, int ldb,
           double beta_re, double beta_im, R* C, int ldc, Prof* prof = nullptr);
+// Sizes the split-K partials for every call on this GPU (num_cu CUs), so gemm allocates
+// nothing afterwards.
+void gemm_reserve(GemmWorkspace& ws, int num_cu);
 
 // ---------------------------------------------------------------- eigensolver
 // Lowest nband eigenpairs of Hermitian hcc (n x n, lower triangle, ld ldh), n <= HEEVX_MAX.
@@ -180,6 +184,9 @@ constexpr int HEEVX_MAX = 2000;
 // eig: device double[nband]; vcc: device c128 (ld ldv), columns 0..nband-1 rows 0..n-1.
 void heevx_lowest(hipStream_t s, EigWorkspace& ws, int num_cu, int n, int ldh, const double* hcc,
                   int nband, double* eig, double* vcc, int ldv, Prof* prof = nullptr);
+// Sizes ws for every n <= nmax, so heevx_lowest allocates nothing (a growing workspace
+// frees and allocates, and hipFree waits for the device).
+void heevx_reserve(EigWorkspace& ws, int nmax, int nband);
 
 // ---------------------------------------------------------------- small dense helpers
 // Cholesky of Hermitian positive definite G (n x n, ld ldg, upper triangle read) and inverse

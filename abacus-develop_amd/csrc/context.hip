@@ -33,6 +33,14 @@ hipEvent_t Prof::event() {
     return e;
 }
 
+void Prof::prefill(size_t n) {
+    while (pool.size() < n) {
+        hipEvent_t e;
+        APW_HIP(hipEventCreate(&e));
+        pool.push_back(e);
+    }
+}
+
 void Prof::resolve() {
     for (auto& e : entries) {
         for (auto& pr : e.pending) {
@@ -148,6 +156,8 @@ int apw_ctx_set_profiling(apw_ctx* ctx, int on) {
         apw::DeviceGuard g(ctx->device);
         ctx->prof.reset();
         ctx->prof.on = on != 0;
+        // event creation costs tens of microseconds of host time: none inside the profiled code
+        if (ctx->prof.on) ctx->prof.prefill(4096);
         return (int)APW_OK;
     });
 }

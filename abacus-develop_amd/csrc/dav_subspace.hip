@@ -173,7 +173,7 @@ struct DavSub {
     DevBuf<double> pre, psi_iter, hphi, hcc, scc, vcc, eig, ec, vc, tmp, n2, hs, ss, vs;
     DevBuf<int> unconv_d, status;
     EigWorkspace ews;
-    GemmWorkspace gws;
+    GemmWorkspace& gws() { return ctx->gws; }   // shared by the context: sized once
     GenWorkspace gen;
 
     // per call
@@ -219,6 +219,8 @@ struct DavSub {
         hs.reserve((size_t)2 * nband * nband);
         ss.reserve((size_t)2 * nband * nband);
         vs.reserve((size_t)2 * nband * nband);
+        heevx_reserve(ews, nbx, nbx);   // the pencil path asks for up to nbx pairs
+        gemm_reserve(ctx->gws, ctx->num_cu);
         unconv_d.reserve(nband);
         status.reserve(4);
         hipLaunchKernelGGL(k_ds_zero, dim3(1), dim3(64), 0, st(), (size_t)2, reinterpret_cast<double*>(status.get()));
@@ -231,15 +233,13 @@ struct DavSub {
 
     void gemm(char ta, char tb, int m, int n, int k, double ar, const double* A, int lda, const double* B, int ldb,
               double br, double* C, int ldc) {
-        apw::gemm<double>(st(), gws, ctx->num_cu, ta, tb, m, n, k, ar, 0.0, A, lda, B, ldb, br, 0.0, C, ldc, &ctx->prof);
+        apw::gemm<double>(st(), gws(), ctx->num_cu, ta, tb, m, n, k, ar, 0.0, A, lda, B, ldb, br, 0.0, C, ldc, &ctx->prof);
     }
 
     // HX = H X[:, i0..i1] (HPsiFunc semantics; X is the head of the block, ld = dim)
     void apply_h(double* X, int nvec, double* HX, int i0, int i1) {
         const int nb = i1 - i0 + 1;
-        hipEvent_t e0, e1;
-        APW_HIP(hipEventCreate(&e0));
-        APW_HIP(hipEventCreate(&e1));
+        hipEvent_t e0 = ctx->prof.event(), e1 = ctx->prof.event();   // pooled: returned after the solve
         APW_HIP(hipEventRecord(e0, st()));
         if (ham) {
             ham->hpsi(Bd(X, i0), dim, HX, dim, nb, APW_FP64);
@@ -289,7 +289,7 @@ struct DavSub {
         gemm('C', 'N', nband, nband, dim, 1.0, psi_iter.get(), dim, tmp.get(), dim, 0.0, hs.get(), nband);
         gemm('C', 'N', nband, nband, dim, 1.0, psi_iter.get(), dim, psi_iter.get(), dim, 0.0, ss.get(), nband);
         zero_status();
-        hegvx_lowest(st(), gws, ews, gen, ctx->num_cu, nband, nband, hs.get(), ss.get(), nband, eig.get(), vs.get(),
+        hegvx_lowest(st(), gws(), ews, gen, ctx->num_cu, nband, nband, hs.get(), ss.get(), nband, eig.get(), vs.get(),
                      nband, status.get(), &ctx->prof);
         check_status("diagH_subspace: overlap matrix not positive definite");
         gemm('N', 'N', dim, nband, nband, 1.0, psi_iter.get(), dim, vs.get(), nband, 0.0, tmp.get(), dim);
@@ -323,7 +323,7 @@ struct DavSub {
             heevx_lowest(st(), ews, ctx->num_cu, nbase, nbx, hcc.get(), nband, eig.get(), vcc.get(), nbx, &ctx->prof);
         } else {
             zero_status();
-            hegvx_lowest(st(), gws, ews, gen, ctx->num_cu, nbase, nbx, hcc.get(), scc.get(), nband, eig.get(),
+            hegvx_lowest(st(), gws(), ews, gen, ctx->num_cu, nbase, nbx, hcc.get(), scc.get(), nband, eig.get(),
                          vcc.get(), nbx, status.get(), &ctx->prof);
         }
         check_status("overlap matrix not positive definite");
@@ -491,8 +491,8 @@ int apw_dav_subspace_diag(apw_dav_subspace* dav, apw_ham* ham, apw_hpsi_cb hpsi_
             if (rc == APW_OK && hipEventSynchronize(pr.second) == hipSuccess &&
                 hipEventElapsedTime(&ms, pr.first, pr.second) == hipSuccess)
                 th += ms * 1e-3;
-            (void)hipEventDestroy(pr.first);
-            (void)hipEventDestroy(pr.second);
+            D.ctx->prof.pool.push_back(pr.first);
+            D.ctx->prof.pool.push_back(pr.second);
         }
         D.hev.clear();
         if (rc != APW_OK) return rc;

@@ -90,26 +90,42 @@ __global__ void k_precond(int dim, int nc, double2* __restrict__ x, const double
     }
 }
 
-// out[m] = || x[:, m] - lam[m] * y[:, m] ||^2  (one workgroup per column; y may be null)
-__global__ void k_colres2(int rows, const double2* __restrict__ x, int ldx, const double2* __restrict__ y, int ldy,
-                          const double* __restrict__ lam, double* __restrict__ out) {
-    __shared__ double red[4];
+// out[m] = || x[:, m] - lam[m] * y[:, m] ||^2  (one workgroup of 1024 per column; y may be
+// null).  Four elements per thread per step, so four loads are in flight per lane.
+constexpr int CR_NT = 1024;
+__global__ __launch_bounds__(CR_NT) void k_colres2(int rows, const double2* __restrict__ x, int ldx,
+                                                   const double2* __restrict__ y, int ldy,
+                                                   const double* __restrict__ lam, double* __restrict__ out) {
+    __shared__ double red[CR_NT / 64];
     const int m = blockIdx.x;
     const double l = (y && lam) ? lam[m] : 0.0;
-    double s = 0;
-    for (int i = threadIdx.x; i < rows; i += blockDim.x) {
-        double2 v = x[(size_t)m * ldx + i];
-        if (y) {
-            const double2 w = y[(size_t)m * ldy + i];
-            v.x -= l * w.x;
-            v.y -= l * w.y;
+    const double2* xm = x + (size_t)m * ldx;
+    const double2* ym = y ? y + (size_t)m * ldy : nullptr;
+    double s[4] = {0, 0, 0, 0};
+    for (int i0 = threadIdx.x; i0 < rows; i0 += 4 * CR_NT) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int i = i0 + u * CR_NT;
+            if (i < rows) {
+                double2 v = xm[i];
+                if (ym) {
+                    const double2 w = ym[i];
+                    v.x -= l * w.x;
+                    v.y -= l * w.y;
+                }
+                s[u] += v.x * v.x + v.y * v.y;
+            }
         }
-        s += v.x * v.x + v.y * v.y;
     }
-    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    double t = (s[0] + s[1]) + (s[2] + s[3]);
+    for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = t;
     __syncthreads();
-    if (threadIdx.x == 0) out[m] = red[0] + red[1] + red[2] + red[3];
+    if (threadIdx.x == 0) {
+        double r = 0;
+        for (int w = 0; w < CR_NT / 64; ++w) r += red[w];
+        out[m] = r;
+    }
 }
 
 // refresh(): hcc = diag(eig[0..nband)), vcc[i][i] = 1 (off-diagonal entries of vcc kept)
@@ -129,7 +145,7 @@ struct David {
     DevBuf<double> pre, basis, hp, sp, hcc, vcc, eig, tmp, vc, lmat, resid;
     DevBuf<int> unconv_d, status;
     EigWorkspace ews;
-    GemmWorkspace gws;
+    GemmWorkspace& gws() { return ctx->gws; }   // shared by the context: sized once
 
     // per call
     Ham* ham = nullptr;
@@ -141,6 +157,17 @@ struct David {
     long long hpsi_bands = 0;
     double last_resid = -1.0;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> hev;
+    // pinned host block: [status 2 int][barrier words 2 uint][eig nband][resid nband][unconv nband int]
+    unsigned char* hbuf = nullptr;
+    size_t hbuf_bytes() const { return 16 + (size_t)2 * nband * sizeof(double) + (size_t)nband * sizeof(int); }
+    int* h_status() { return reinterpret_cast<int*>(hbuf); }
+    unsigned* h_bar() { return reinterpret_cast<unsigned*>(hbuf + 8); }
+    double* h_eig() { return reinterpret_cast<double*>(hbuf + 16); }
+    double* h_res() { return h_eig() + nband; }
+    int* h_unconv() { return reinterpret_cast<int*>(h_res() + nband); }
+    ~David() {
+        if (hbuf) (void)hipHostFree(hbuf);
+    }
 
     hipStream_t st() const { return ctx->stream; }
     double2* B(double* p, int col) const { return reinterpret_cast<double2*>(p) + (size_t)col * dim; }
@@ -171,6 +198,10 @@ struct David {
         resid.reserve(nbx);
         unconv_d.reserve(nband);
         status.reserve(4);
+        heevx_reserve(ews, nbx, nband);
+        gemm_reserve(ctx->gws, ctx->num_cu);
+        // pinned host words of the per-iteration reads (pageable copies go through staging)
+        APW_HIP(hipHostMalloc(reinterpret_cast<void**>(&hbuf), hbuf_bytes(), hipHostMallocDefault));
         hipLaunchKernelGGL(k_zero, dim3(gsz((size_t)2 * nbx * nbx)), dim3(256), 0, st(), (size_t)2 * nbx * nbx, vcc.get());
         hipLaunchKernelGGL(k_zero, dim3(gsz((size_t)2 * nbx * nbx)), dim3(256), 0, st(), (size_t)2 * nbx * nbx, hcc.get());
         APW_LAUNCH_CHECK();
@@ -181,15 +212,13 @@ struct David {
 
     void gemm(char ta, char tb, int m, int n, int k, double ar, const double* A, int lda, const double* Bm, int ldb,
               double br, double* C, int ldc) {
-        apw::gemm<double>(st(), gws, ctx->num_cu, ta, tb, m, n, k, ar, 0.0, A, lda, Bm, ldb, br, 0.0, C, ldc, &ctx->prof);
+        apw::gemm<double>(st(), gws(), ctx->num_cu, ta, tb, m, n, k, ar, 0.0, A, lda, Bm, ldb, br, 0.0, C, ldc, &ctx->prof);
     }
 
     // HX = H X[:, i0..i1]  (hpsi_func semantics: X = head of basis, HX = exact destination)
     void apply_h(double* HX, int i0, int i1, int p) {
         const int nb = i1 - i0 + 1;
-        hipEvent_t e0, e1;
-        APW_HIP(hipEventCreate(&e0));
-        APW_HIP(hipEventCreate(&e1));
+        hipEvent_t e0 = ctx->prof.event(), e1 = ctx->prof.event();   // pooled: returned after the solve
         APW_HIP(hipEventRecord(e0, st()));
         if (ham) {
             ham->hpsi(Bd(basis.get(), i0), dim, HX, dim, nb, p);
@@ -252,12 +281,23 @@ struct David {
         apply_s(c0, nc);
     }
 
-    void check_status() {
-        int h[2] = {0, 0};
-        unsigned b[2] = {0, 0};
-        APW_HIP(hipMemcpyAsync(h, status.get(), sizeof(h), hipMemcpyDeviceToHost, st()));
-        if (ews.bar.get()) APW_HIP(hipMemcpyAsync(b, ews.bar.get(), sizeof(b), hipMemcpyDeviceToHost, st()));
+    // the breakdown and time-out words, the eigenvalues and (nres > 0) residual norms in one
+    // host round trip; throws on a breakdown or a time-out
+    void fetch_iteration(int nres) {
+        APW_HIP(hipMemcpyAsync(h_status(), status.get(), 2 * sizeof(int), hipMemcpyDeviceToHost, st()));
+        h_bar()[0] = h_bar()[1] = 0;
+        if (ews.bar.get()) APW_HIP(hipMemcpyAsync(h_bar(), ews.bar.get(), 2 * sizeof(unsigned), hipMemcpyDeviceToHost, st()));
+        APW_HIP(hipMemcpyAsync(h_eig(), eig.get(), nband * sizeof(double), hipMemcpyDeviceToHost, st()));
+        if (nres > 0) APW_HIP(hipMemcpyAsync(h_res(), resid.get(), nres * sizeof(double), hipMemcpyDeviceToHost, st()));
         APW_HIP(hipStreamSynchronize(st()));
+        check_words(h_status(), h_bar());
+    }
+
+    void check_status() {
+        fetch_iteration(0);
+    }
+
+    void check_words(const int* h, const unsigned* b) {
         if (h[0] != 0) {
             set_error("DiagoDavid::SchmidtOrth: psi_norm < 1.0e-12 (vector %d of the block)", h[0] - 1);
             throw Error{APW_EBREAKDOWN};
@@ -294,15 +334,16 @@ struct David {
         cal_elem(0, nband);
         int nbase = nband;
         solve_subspace(nbase);
-        check_status();
-        APW_HIP(hipMemcpyAsync(eig_host, eig.get(), nband * sizeof(double), hipMemcpyDeviceToHost, st()));
-        APW_HIP(hipStreamSynchronize(st()));
-        std::vector<double> ehost(nband), res(nband);
+        fetch_iteration(0);
+        std::copy(h_eig(), h_eig() + nband, eig_host);
         int it = 0;
         while (true) {
             ++it;
             // ---- cal_grad (:359-611) ----
-            APW_HIP(hipMemcpyAsync(unconv_d.get(), unconv.data(), notconv * sizeof(int), hipMemcpyHostToDevice, st()));
+            // (the pinned unconv block is not rewritten before the stream has consumed it: the
+            // next write follows the synchronisation of fetch_iteration below)
+            std::copy(unconv.begin(), unconv.begin() + notconv, h_unconv());
+            APW_HIP(hipMemcpyAsync(unconv_d.get(), h_unconv(), notconv * sizeof(int), hipMemcpyHostToDevice, st()));
             double2* vcp = reinterpret_cast<double2*>(vc.get());
             hipLaunchKernelGGL(k_gather_vc, dim3(gsz((size_t)nbase * notconv)), dim3(256), 0, st(), nbase, notconv,
                                reinterpret_cast<const double2*>(vcc.get()), nbx, unconv_d.get(), vcp);
@@ -312,7 +353,7 @@ struct David {
                                eig.get(), unconv_d.get());
             gemm('N', 'N', dim, notconv, nbase, 1.0, S(), dim, vc.get(), nbase, 1.0, NB, dim);
             if (switch_thr > 0) {
-                hipLaunchKernelGGL(k_colres2, dim3(notconv), dim3(256), 0, st(), dim,
+                hipLaunchKernelGGL(k_colres2, dim3(notconv), dim3(CR_NT), 0, st(), dim,
                                    reinterpret_cast<const double2*>(NB), dim, (const double2*)nullptr, 0,
                                    (const double*)nullptr, resid.get());
             }
@@ -326,14 +367,10 @@ struct David {
             const int nc_grad = notconv;
             nbase += notconv;
             solve_subspace(nbase);
-            check_status();
-            APW_HIP(hipMemcpyAsync(ehost.data(), eig.get(), nband * sizeof(double), hipMemcpyDeviceToHost, st()));
+            fetch_iteration(switch_thr > 0 ? nc_grad : 0);
+            const double* ehost = h_eig();
             double rmax = 0;
-            if (switch_thr > 0) {
-                APW_HIP(hipMemcpyAsync(res.data(), resid.get(), nc_grad * sizeof(double), hipMemcpyDeviceToHost, st()));
-            }
-            APW_HIP(hipStreamSynchronize(st()));
-            for (int m = 0; m < nc_grad && switch_thr > 0; ++m) rmax = std::max(rmax, std::sqrt(res[m]));
+            for (int m = 0; m < nc_grad && switch_thr > 0; ++m) rmax = std::max(rmax, std::sqrt(h_res()[m]));
             // ---- convergence (:270-280) ----
             notconv = 0;
             for (int m = 0; m < nband; ++m) {
@@ -354,7 +391,7 @@ struct David {
                     // residual norms ||H psi_m - e_m psi_m|| of the returned vectors, from the
                     // subspace (hpsi vcc - e psi): one GEMM, no extra H application
                     gemm('N', 'N', dim, nband, nbase, 1.0, hp.get(), dim, vcc.get(), nbx, 0.0, tmp.get(), dim);
-                    hipLaunchKernelGGL(k_colres2, dim3(nband), dim3(256), 0, st(), dim,
+                    hipLaunchKernelGGL(k_colres2, dim3(nband), dim3(CR_NT), 0, st(), dim,
                                        reinterpret_cast<const double2*>(tmp.get()), dim,
                                        reinterpret_cast<const double2*>(psi), ldpsi, eig.get(), resid.get());
                     APW_LAUNCH_CHECK();
@@ -473,8 +510,8 @@ int apw_david_diag(apw_david* dav, apw_ham* ham, apw_hpsi_cb hpsi_cb, apw_spsi_c
         for (auto& pr : D.hev) {
             float ms = 0;
             if (rc == APW_OK && hipEventElapsedTime(&ms, pr.first, pr.second) == hipSuccess) th += ms * 1e-3;
-            (void)hipEventDestroy(pr.first);
-            (void)hipEventDestroy(pr.second);
+            D.ctx->prof.pool.push_back(pr.first);
+            D.ctx->prof.pool.push_back(pr.second);
         }
         D.hev.clear();
         if (rc != APW_OK) return rc;

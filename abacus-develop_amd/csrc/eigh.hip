@@ -1097,6 +1097,18 @@ __global__ __launch_bounds__(1024) void k_chol_inv(int n, const double2* __restr
 
 }  // namespace
 
+void heevx_reserve(EigWorkspace& ws, int nmax, int nband) {
+    const size_t n = (size_t)std::max(nmax, 1);
+    ws.a.reserve(2 * n * n);
+    ws.v.reserve(2 * n * n);
+    ws.tau.reserve(2 * n);
+    ws.d.reserve(n);
+    ws.e.reserve(n);
+    ws.z.reserve(n * (size_t)std::max(nband, 1));
+    ws.bar.reserve(64 + 256);
+    ws.ring.reserve(2 * (2 * n * n));   // the larger of the two exchange rings below
+}
+
 void heevx_lowest(hipStream_t s, EigWorkspace& ws, int num_cu, int n, int ldh, const double* hcc, int nband,
                   double* eig, double* vcc, int ldv, Prof* prof) {
     APW_CHECK(n >= 1 && nband >= 1 && nband <= n && ldh >= n && ldv >= n, "heevx: bad sizes n=%d nband=%d", n, nband);

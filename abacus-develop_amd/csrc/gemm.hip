@@ -364,6 +364,11 @@ void gemm(hipStream_t s, GemmWorkspace& ws, int num_cu, char ta, char tb, int m,
     if (inner) variant = n <= 32 ? 1 : (tall ? 2 : 0);
     else if (tall) variant = n % 64 == 0 ? 3 : 4;
     else variant = wide ? 5 : 0;
+    static const int variant_env = [] {
+        const char* e = std::getenv("APW_GEMM_VARIANT");   // tuning override of the tile choice
+        return e ? std::atoi(e) : -1;
+    }();
+    if (variant_env >= 0 && variant_env <= 5) variant = variant_env;
     static const int VBM[6] = {64, 64, 128, 128, 128, 64}, VBN[6] = {64, 32, 64, 64, 32, 128};
     static const int VOCC[6] = {3, 6, 3, 3, 3, 2};   // resident workgroups per CU (measured best)
     int splits = inner ? split_count(VBM[variant], VBN[variant], VOCC[variant]) : 1;
@@ -419,6 +424,12 @@ void gemm(hipStream_t s, GemmWorkspace& ws, int num_cu, char ta, char tb, int m,
                            alpha_im, beta_re, beta_im);
         APW_LAUNCH_CHECK();
     }
+}
+
+void gemm_reserve(GemmWorkspace& ws, int num_cu) {
+    // splits x tiles <= resident slots, so splits m n <= slots x BM x BN; the largest product
+    // of the split variants is 3 x 128 x 64 (c128: two doubles an element)
+    ws.part.reserve((size_t)3 * std::max(num_cu, 64) * 128 * 64 * 2 + 2);
 }
 
 template void gemm<double>(hipStream_t, GemmWorkspace&, int, char, char, int, int, int, double, double,
