@@ -260,13 +260,17 @@ void Ham::create(apw_ctx* c, const apw_ham_desc* d) {
     // size of the Infinity Cache so the five passes mostly hit it
     const size_t per_band = grid.bytes_S(16) + grid.bytes_X(16);
     chunk = (int)std::max<size_t>(1, std::min<size_t>(256, (size_t)(192u << 20) / per_band));
+    S.reserve(2 * grid.bytes_S(1) * chunk);   // sized here: no allocation inside H|psi>
+    X.reserve(2 * grid.bytes_X(1) * chunk);
     APW_HIP(hipStreamSynchronize(ctx->stream));
 }
 
 void Ham::local(const double* psi, int ld, double* hpsi, int ldh, int nb, GatherMode mode, int prec) {
     const double scale = 1.0 / (double)((size_t)grid.nx * grid.ny * grid.nz);
-    for (int b0 = 0; b0 < nb; b0 += chunk) {
-        const int n = std::min(chunk, nb - b0);
+    // FP32 intermediates take half the bytes: twice the bands per chunk, half the launches
+    const int cb = prec == APW_FP64 ? chunk : 2 * chunk;
+    for (int b0 = 0; b0 < nb; b0 += cb) {
+        const int n = std::min(cb, nb - b0);
         if (prec == APW_FP64) {
             S.reserve(2 * grid.bytes_S(1) * n);
             X.reserve(2 * grid.bytes_X(1) * n);
