@@ -339,8 +339,8 @@ void gemm(hipStream_t s, GemmWorkspace& ws, int num_cu, char ta, char tb, int m,
         return;
     }
     // Tile choice (measured on the Davidson shapes).  Inner products (k >> m n, split over k
-    // with a fixed-order reduction): 64 x 32 tiles for n <= 32, 128 x 64 walking n for tall
-    // C, else 64 x 64.  Updates (no split): 128 x 64 or 128 x 32 for tall C (m >> n; the
+    // with a fixed-order reduction): 64 x 32 or 64 x 64 tiles (below).  Updates (no split):
+    // 128 x 64 or 128 x 32 for tall C (m >> n; the
     // 32-wide tile unless n is a multiple of 64: less waste in the last tile column, more
     // workgroups) in the XCD-aware order, 64 x 128 for wide, else 64 x 64.
     // Split count: every workgroup of a split launch is resident at once (floor of the
@@ -361,7 +361,9 @@ void gemm(hipStream_t s, GemmWorkspace& ws, int num_cu, char ta, char tb, int m,
     const bool tall = m >= 4 * n && m >= 256, wide = n >= 4 * m && n >= 256;
     const bool inner = tiles_of(64, 64) < 2 * cus && k / 64 > 1;
     int variant;   // 0: 2x2 map0, 1: 2x1 map0, 2: 4x2 map1, 3: 4x2 map2, 4: 4x1 map2, 5: 2x4 bk16 map0
-    if (inner) variant = n <= 32 ? 1 : (tall ? 2 : 0);
+    // inner products: 64 x 32 tiles when they cover n with fewer columns than 64 x 64 ones
+    // (n <= 32, 65..96), else 64 x 64 (measured; 128 x 64 lost on every si64 shape)
+    if (inner) variant = (n <= 32 || (n > 64 && n <= 96)) ? 1 : 0;
     else if (tall) variant = n % 64 == 0 ? 3 : 4;
     else variant = wide ? 5 : 0;
     static const int variant_env = [] {
