@@ -1,6 +1,7 @@
 // Internal declarations of the apw library (HIP, gfx950).
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <cstdarg>
@@ -109,14 +110,29 @@ struct Prof {
     ~Prof();
 };
 
+// attached = false: events recorded on the stream around whatever the scope launches.
+// attached = true: the scope's one kernel is launched by APW_PLAUNCH, which hands the two
+// events to the dispatch itself (no marker packets, no extra host work per launch).
 struct ProfScope {
     Prof* p = nullptr;
     hipStream_t s = nullptr;
     ProfEntry* e = nullptr;
     hipEvent_t a = nullptr, b = nullptr;
-    ProfScope(Prof* prof, hipStream_t st, const char* name, double bytes, double flops);
+    bool attached = false, used = false;
+    double bytes_ = 0, flops_ = 0;
+    ProfScope(Prof* prof, hipStream_t st, const char* name, double bytes, double flops, bool attached_ = false);
     ~ProfScope();
 };
+
+#define APW_PLAUNCH(ps, K, G, B, L, S, ...)                                                            \
+    do {                                                                                               \
+        if ((ps).attached && (ps).e) {                                                                 \
+            hipExtLaunchKernelGGL(K, G, B, (std::uint32_t)(L), S, (ps).a, (ps).b, 0u, __VA_ARGS__);   \
+            (ps).used = true;                                                                          \
+        } else {                                                                                       \
+            hipLaunchKernelGGL(K, G, B, L, S, __VA_ARGS__);                                            \
+        }                                                                                              \
+    } while (0)
 
 struct EigWorkspace {
     DevBuf<double> a;      // n x n c128 working matrix

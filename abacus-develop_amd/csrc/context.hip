@@ -69,21 +69,38 @@ Prof::~Prof() {
     for (auto ev : pool) (void)hipEventDestroy(ev);
 }
 
-ProfScope::ProfScope(Prof* prof, hipStream_t st, const char* name, double bytes, double flops) {
+ProfScope::ProfScope(Prof* prof, hipStream_t st, const char* name, double bytes, double flops, bool attached_) {
     if (!prof || !prof->on) return;
     p = prof;
     s = st;
+    attached = attached_;
     e = &prof->entry(name);
-    e->launches += 1;
-    e->bytes += bytes;
-    e->flops += flops;
+    bytes_ = bytes;
+    flops_ = flops;
+    if (!attached) {
+        e->launches += 1;
+        e->bytes += bytes;
+        e->flops += flops;
+    }
     a = prof->event();
     b = prof->event();
-    APW_HIP(hipEventRecord(a, s));
+    if (!attached) APW_HIP(hipEventRecord(a, s));
 }
 
 ProfScope::~ProfScope() {
     if (!p) return;
+    if (attached) {
+        if (used) {
+            e->launches += 1;
+            e->bytes += bytes_;
+            e->flops += flops_;
+            e->pending.emplace_back(a, b);
+        } else {   // nothing was launched: the events go back to the pool
+            p->pool.push_back(a);
+            p->pool.push_back(b);
+        }
+        return;
+    }
     if (hipEventRecord(b, s) == hipSuccess) e->pending.emplace_back(a, b);
 }
 

@@ -305,13 +305,13 @@ int opcode(char t) {
 }
 
 template <class R, int TA, int TB, int TM, int TN, int BK, int MAP>
-void launch(hipStream_t s, int splits, int M, int N, int K, int kchunk, const R* A, int lda, const R* B, int ldb,
-            R* C, int ldc, double ar, double ai, double br, double bi, R* part) {
+void launch(ProfScope& ps, hipStream_t s, int splits, int M, int N, int K, int kchunk, const R* A, int lda,
+            const R* B, int ldb, R* C, int ldc, double ar, double ai, double br, double bi, R* part) {
     using c_t = typename Mfma<R>::c_t;
     const int tm = (M + 32 * TM - 1) / (32 * TM), tn = (N + 32 * TN - 1) / (32 * TN);
     const dim3 grid = MAP == 2 ? dim3((unsigned)(((tm + 7) / 8) * 8 * tn), 1, splits)
                                : (MAP == 1 ? dim3(tn, tm, splits) : dim3(tm, tn, splits));
-    hipLaunchKernelGGL((zgemm_kernel<R, TA, TB, TM, TN, BK, MAP>), grid, dim3(NT), 0, s, M, N, K, kchunk,
+    APW_PLAUNCH(ps, (zgemm_kernel<R, TA, TB, TM, TN, BK, MAP>), grid, dim3(NT), 0, s, M, N, K, kchunk,
                        reinterpret_cast<const c_t*>(A), lda, reinterpret_cast<const c_t*>(B), ldb,
                        reinterpret_cast<c_t*>(C), ldc, ar, ai, br, bi, reinterpret_cast<c_t*>(part));
 }
@@ -393,21 +393,21 @@ void gemm(hipStream_t s, GemmWorkspace& ws, int num_cu, char ta, char tb, int m,
                                                         : (splits > 1 ? "cgemm_f32_splitk" : "cgemm_f32_tiled");
     ProfScope ps(prof, s, pname,
                  esz * ((double)m * k + (double)k * n + (double)m * n * (splits > 1 ? splits : (beta_nz ? 2 : 1))),
-                 8.0 * m * n * (double)k);
+                 8.0 * m * n * (double)k, true);
 #define APW_GEMM_CASE(X, Y)                                                                                  \
     if (oa == X && ob == Y) {                                                                               \
         switch (variant) {                                                                                  \
-            case 1: launch<R, X, Y, 2, 1, 8, 0>(s, splits, m, n, k, kchunk, A, lda, B, ldb, C, ldc, alpha_re,    \
+            case 1: launch<R, X, Y, 2, 1, 8, 0>(ps, s, splits, m, n, k, kchunk, A, lda, B, ldb, C, ldc, alpha_re,    \
                                                alpha_im, beta_re, beta_im, part); break;                    \
-            case 2: launch<R, X, Y, 4, 2, 8, 1>(s, splits, m, n, k, kchunk, A, lda, B, ldb, C, ldc, alpha_re,    \
+            case 2: launch<R, X, Y, 4, 2, 8, 1>(ps, s, splits, m, n, k, kchunk, A, lda, B, ldb, C, ldc, alpha_re,    \
                                                alpha_im, beta_re, beta_im, part); break;                    \
-            case 3: launch<R, X, Y, 4, 2, 8, 2>(s, splits, m, n, k, kchunk, A, lda, B, ldb, C, ldc, alpha_re,    \
+            case 3: launch<R, X, Y, 4, 2, 8, 2>(ps, s, splits, m, n, k, kchunk, A, lda, B, ldb, C, ldc, alpha_re,    \
                                                alpha_im, beta_re, beta_im, part); break;                    \
-            case 4: launch<R, X, Y, 4, 1, 8, 2>(s, splits, m, n, k, kchunk, A, lda, B, ldb, C, ldc, alpha_re,    \
+            case 4: launch<R, X, Y, 4, 1, 8, 2>(ps, s, splits, m, n, k, kchunk, A, lda, B, ldb, C, ldc, alpha_re,    \
                                                alpha_im, beta_re, beta_im, part); break;                    \
-            case 5: launch<R, X, Y, 2, 4, 16, 0>(s, splits, m, n, k, kchunk, A, lda, B, ldb, C, ldc, alpha_re,   \
+            case 5: launch<R, X, Y, 2, 4, 16, 0>(ps, s, splits, m, n, k, kchunk, A, lda, B, ldb, C, ldc, alpha_re,   \
                                                 alpha_im, beta_re, beta_im, part); break;                   \
-            default: launch<R, X, Y, 2, 2, 8, 0>(s, splits, m, n, k, kchunk, A, lda, B, ldb, C, ldc, alpha_re,   \
+            default: launch<R, X, Y, 2, 2, 8, 0>(ps, s, splits, m, n, k, kchunk, A, lda, B, ldb, C, ldc, alpha_re,   \
                                                 alpha_im, beta_re, beta_im, part); break;                   \
         }                                                                                                   \
     }
@@ -418,10 +418,10 @@ void gemm(hipStream_t s, GemmWorkspace& ws, int num_cu, char ta, char tb, int m,
     APW_LAUNCH_CHECK();
     if (splits > 1) {
         ProfScope pr(prof, s, "gemm_splitk_reduce", esz * (double)m * n * (splits + (beta_nz ? 2 : 1)),
-                     2.0 * m * n * (double)splits);
+                     2.0 * m * n * (double)splits, true);
         const size_t total = (size_t)m * n;
         const int blocks = (int)std::min<size_t>((total + 255) / 256, 65536);
-        hipLaunchKernelGGL(splitk_reduce<R>, dim3(blocks), dim3(256), 0, s, m, n, splits,
+        APW_PLAUNCH(pr, splitk_reduce<R>, dim3(blocks), dim3(256), 0, s, m, n, splits,
                            reinterpret_cast<const c_t*>(part), reinterpret_cast<c_t*>(C), ldc, alpha_re,
                            alpha_im, beta_re, beta_im);
         APW_LAUNCH_CHECK();

@@ -1067,17 +1067,17 @@ void pass_forward_half(hipStream_t s, const PwGrid& g, R* S, R* X, int nb, doubl
     const double cs = sizeof(C);
     const double nSz = (double)g.ns * g.nz, nX = (double)g.nxr * g.ny * g.nz;
     {
-        ProfScope ps(prof, s, "fft_y_fwd", nb * cs * (nX + nSz), nb * fft_flops(g.ny) * g.nxr * g.nz);
+        ProfScope ps(prof, s, "fft_y_fwd", nb * cs * (nX + nSz), nb * fft_flops(g.ny) * g.nxr * g.nz, true);
 #define APW_YREG_LAUNCH(N)                                                                                    \
     case N:                                                                                                  \
-        hipLaunchKernelGGL((k_yreg<R, N, -1>), dim3(g.nxr, (g.nz + XL - 1) / XL, nb), dim3(xreg_threads(N)), 0, \
+        APW_PLAUNCH(ps, (k_yreg<R, N, -1>), dim3(g.nxr, (g.nz + XL - 1) / XL, nb), dim3(xreg_threads(N)), 0, \
                            s, reinterpret_cast<const C*>(X), reinterpret_cast<C*>(S), g.ns, g.nz, g.nxr,         \
                            g.slot_start.get(), g.yrun.get(), tw_ptr<R>(g, g.twoff_y));                       \
         break;
         switch (yreg_len(g)) {
             APW_FFT_SIZES(APW_YREG_LAUNCH)
             default:
-                hipLaunchKernelGGL((k_yfwd<R>), dim3(g.nxr, (g.nz + Ly - 1) / Ly, nb), dim3(NT),
+                APW_PLAUNCH(ps, (k_yfwd<R>), dim3(g.nxr, (g.nz + Ly - 1) / Ly, nb), dim3(NT),
                                    lds_bytes<R>(g.ny, Ly), s, reinterpret_cast<const C*>(X), g.ns, g.nz, g.nxr,
                                    g.slot_start.get(), g.slot_cnt.get(), g.stick_iy.get(), reinterpret_cast<C*>(S),
                                    g.py, tw_ptr<R>(g, g.twoff_y), Ly);
@@ -1087,17 +1087,17 @@ void pass_forward_half(hipStream_t s, const PwGrid& g, R* S, R* X, int nb, doubl
         APW_LAUNCH_CHECK();
     }
     const double pwio = 16.0 * g.npw * ((mode == GATHER_SET) ? 1 : 2);
-    ProfScope ps(prof, s, "fft_z_fwd_gather", nb * (cs * nSz + pwio), nb * (fft_flops(g.nz) * g.ns + 4.0 * g.npw));
+    ProfScope ps(prof, s, "fft_z_fwd_gather", nb * (cs * nSz + pwio), nb * (fft_flops(g.nz) * g.ns + 4.0 * g.npw), true);
 #define APW_ZREG_LAUNCH(N)                                                                                    \
     case N:                                                                                                  \
-        hipLaunchKernelGGL((k_zreg<R, N, -1>), dim3((g.ns + ZL - 1) / ZL, nb), dim3(zreg_threads(N)), 0, s, psi, \
+        APW_PLAUNCH(ps, (k_zreg<R, N, -1>), dim3((g.ns + ZL - 1) / ZL, nb), dim3(zreg_threads(N)), 0, s, psi, \
                            ld, g.stick_ig.get(), g.ns, reinterpret_cast<C*>(S), hpsi, ldh, gk2, tpiba2, (int)mode, \
                            scale, tw_ptr<R>(g, g.twoff_z));                                                  \
         break;
     switch (zreg_len(g, ld, ldh)) {
         APW_FFT_SIZES(APW_ZREG_LAUNCH)
         default:
-            hipLaunchKernelGGL((k_zfwd<R>), dim3((g.ns + Lz - 1) / Lz, nb), dim3(NT), lds_bytes<R>(g.nz, Lz), s,
+            APW_PLAUNCH(ps, (k_zfwd<R>), dim3((g.ns + Lz - 1) / Lz, nb), dim3(NT), lds_bytes<R>(g.nz, Lz), s,
                                reinterpret_cast<const C*>(S), g.stick_ig.get(), g.ns, hpsi, ldh, psi, ld, gk2, tpiba2,
                                (int)mode, scale, g.pz, tw_ptr<R>(g, g.twoff_z), Lz);
             break;
@@ -1113,17 +1113,17 @@ void pass_backward_half(hipStream_t s, const PwGrid& g, const double2* psi, int 
     const double cs = sizeof(C);
     const double nSz = (double)g.ns * g.nz, nX = (double)g.nxr * g.ny * g.nz;
     {
-        ProfScope ps(prof, s, "fft_z_bwd_scatter", nb * (16.0 * g.npw + cs * nSz), nb * fft_flops(g.nz) * g.ns);
+        ProfScope ps(prof, s, "fft_z_bwd_scatter", nb * (16.0 * g.npw + cs * nSz), nb * fft_flops(g.nz) * g.ns, true);
 #define APW_ZREG_LAUNCH(N)                                                                                    \
     case N:                                                                                                  \
-        hipLaunchKernelGGL((k_zreg<R, N, 1>), dim3((g.ns + ZL - 1) / ZL, nb), dim3(zreg_threads(N)), 0, s, psi,  \
+        APW_PLAUNCH(ps, (k_zreg<R, N, 1>), dim3((g.ns + ZL - 1) / ZL, nb), dim3(zreg_threads(N)), 0, s, psi,  \
                            ld, g.stick_ig.get(), g.ns, reinterpret_cast<C*>(S), (double2*)nullptr, 0,          \
                            (const double*)nullptr, 0.0, 0, 0.0, tw_ptr<R>(g, g.twoff_z));                    \
         break;
         switch (zreg_len(g, ld, 0)) {
             APW_FFT_SIZES(APW_ZREG_LAUNCH)
             default:
-                hipLaunchKernelGGL((k_zbwd<R>), dim3((g.ns + Lz - 1) / Lz, nb), dim3(NT), lds_bytes<R>(g.nz, Lz),
+                APW_PLAUNCH(ps, (k_zbwd<R>), dim3((g.ns + Lz - 1) / Lz, nb), dim3(NT), lds_bytes<R>(g.nz, Lz),
                                    s, psi, ld, g.stick_ig.get(), g.ns, reinterpret_cast<C*>(S), g.pz,
                                    tw_ptr<R>(g, g.twoff_z), Lz);
                 break;
@@ -1131,17 +1131,17 @@ void pass_backward_half(hipStream_t s, const PwGrid& g, const double2* psi, int 
 #undef APW_ZREG_LAUNCH
         APW_LAUNCH_CHECK();
     }
-    ProfScope ps(prof, s, "fft_y_bwd", nb * cs * (nSz + nX), nb * fft_flops(g.ny) * g.nxr * g.nz);
+    ProfScope ps(prof, s, "fft_y_bwd", nb * cs * (nSz + nX), nb * fft_flops(g.ny) * g.nxr * g.nz, true);
 #define APW_YREG_LAUNCH(N)                                                                                    \
     case N:                                                                                                  \
-        hipLaunchKernelGGL((k_yreg<R, N, 1>), dim3(g.nxr, (g.nz + XL - 1) / XL, nb), dim3(xreg_threads(N)), 0,  \
+        APW_PLAUNCH(ps, (k_yreg<R, N, 1>), dim3(g.nxr, (g.nz + XL - 1) / XL, nb), dim3(xreg_threads(N)), 0,  \
                            s, reinterpret_cast<const C*>(S), reinterpret_cast<C*>(X), g.ns, g.nz, g.nxr,         \
                            g.slot_start.get(), g.yrun.get(), tw_ptr<R>(g, g.twoff_y));                       \
         break;
     switch (yreg_len(g)) {
         APW_FFT_SIZES(APW_YREG_LAUNCH)
         default:
-            hipLaunchKernelGGL((k_ybwd<R>), dim3(g.nxr, (g.nz + Ly - 1) / Ly, nb), dim3(NT), lds_bytes<R>(g.ny, Ly),
+            APW_PLAUNCH(ps, (k_ybwd<R>), dim3(g.nxr, (g.nz + Ly - 1) / Ly, nb), dim3(NT), lds_bytes<R>(g.ny, Ly),
                                s, reinterpret_cast<const C*>(S), g.ns, g.nz, g.nxr, g.slot_start.get(),
                                g.slot_cnt.get(), g.stick_iy.get(), reinterpret_cast<C*>(X), g.py,
                                tw_ptr<R>(g, g.twoff_y), Ly);
@@ -1273,17 +1273,17 @@ void local_hpsi(hipStream_t s, const PwGrid& g, const double* psi, int ld, doubl
     {
         const double nX = (double)g.nxr * g.ny * g.nz, nxyz = (double)g.nx * g.ny * g.nz;
         ProfScope ps(prof, s, "fft_x_veff_fused", nb * (2.0 * sizeof(C) * nX + sizeof(R) * nxyz),
-                     nb * (2.0 * fft_flops(g.nx) * g.ny * g.nz + 2.0 * nxyz));
+                     nb * (2.0 * fft_flops(g.nx) * g.ny * g.nz + 2.0 * nxyz), true);
 #define APW_XREG_LAUNCH(N)                                                                                     \
     case N:                                                                                                   \
-        hipLaunchKernelGGL((k_xreg<R, N, 0>), dim3(g.ny, (g.nz + XL - 1) / XL, nb), dim3(xreg_threads(N)), 0, s, \
+        APW_PLAUNCH(ps, (k_xreg<R, N, 0>), dim3(g.ny, (g.nz + XL - 1) / XL, nb), dim3(xreg_threads(N)), 0, s, \
                            reinterpret_cast<C*>(X), g.nxr, g.ny, g.nz, g.xrun, veff, (double2*)nullptr, 0,  \
                            0.0, tw_ptr<R>(g, g.twoff_x));                                                     \
         break;
         switch (xreg_len(g)) {
             APW_FFT_SIZES(APW_XREG_LAUNCH)
             default:
-                hipLaunchKernelGGL((k_xpass<R, 0>), dim3(g.ny, (g.nz + Lx - 1) / Lx, nb), dim3(NT),
+                APW_PLAUNCH(ps, (k_xpass<R, 0>), dim3(g.ny, (g.nz + Lx - 1) / Lx, nb), dim3(NT),
                                    lds_bytes<R>(g.nx, Lx), s, reinterpret_cast<C*>(X), g.nxr, g.ny, g.nz,
                                    g.slot_ix.get(), veff, (double2*)nullptr, 0, 0.0, g.px, tw_ptr<R>(g, g.twoff_x),
                                    Lx);
@@ -1359,11 +1359,11 @@ void psi_to_rho(hipStream_t s, const PwGrid& g, const double* psi, int ld, int n
     const double nX = (double)g.nxr * g.ny * g.nz, nxyz = (double)g.nx * g.ny * g.nz;
     // algorithmic: the x slots of every band in, rho read (when adding) and written once
     ProfScope ps(prof, s, "fft_x_rho", nb * sizeof(C) * nX + (add ? 16.0 : 8.0) * nxyz,
-                 nb * (fft_flops(g.nx) * g.ny * g.nz + 3.0 * nxyz));
+                 nb * (fft_flops(g.nx) * g.ny * g.nz + 3.0 * nxyz), true);
     const size_t lds = lds_bytes<double>(g.nx, Lx) + (size_t)g.nx * Lx * sizeof(double);
 #define APW_XREG_LAUNCH(N)                                                                                    \
     case N:                                                                                                  \
-        hipLaunchKernelGGL((k_xrho_reg<N>), dim3(g.ny, (g.nz + XL - 1) / XL), dim3(xreg_threads(N)), 0, s,       \
+        APW_PLAUNCH(ps, (k_xrho_reg<N>), dim3(g.ny, (g.nz + XL - 1) / XL), dim3(xreg_threads(N)), 0, s,       \
                            reinterpret_cast<const C*>(X), g.nxr, g.ny, g.nz, nb, g.xrun, w, rho, add ? 1 : 0, \
                            tw_ptr<double>(g, g.twoff_x));                                                    \
         break;
@@ -1372,7 +1372,7 @@ void psi_to_rho(hipStream_t s, const PwGrid& g, const double* psi, int ld, int n
         default:
             APW_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_xrho),
                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-            hipLaunchKernelGGL(k_xrho, dim3(g.ny, (g.nz + Lx - 1) / Lx), dim3(NT), lds, s,
+            APW_PLAUNCH(ps, k_xrho, dim3(g.ny, (g.nz + Lx - 1) / Lx), dim3(NT), lds, s,
                                reinterpret_cast<const C*>(X), g.nxr, g.ny, g.nz, nb, g.slot_ix.get(), w, rho,
                                add ? 1 : 0, g.px, tw_ptr<double>(g, g.twoff_x), Lx);
             break;
