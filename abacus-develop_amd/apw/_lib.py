@@ -69,6 +69,8 @@ PROTOTYPES = {
     "apw_david_destroy": (_i, [_vp]),
     "apw_david_diag": (_i, [_vp, _vp, HPSI_CB, SPSI_CB, _vp, _i, _vp, C.POINTER(_d), _d, _i, _i, _i, _i,
                             C.POINTER(DiagStats)]),
+    "apw_david_diag_batched": (_i, [_i, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_i), C.POINTER(_vp),
+                                    C.POINTER(_vp), _d, _i, _i, _i, _i, C.POINTER(DiagStats), C.POINTER(_i)]),
     "apw_ekinetic_pw_op": (_i, [_vp, _i, _i, _i, _d, _vp, _vp, _vp]),
     "apw_veff_pw_op": (_i, [_vp, _i, _vp, _vp]),
     "apw_nonlocal_pw_op": (_i, [_vp, _i, _i, _i, C.POINTER(_i), C.POINTER(_i), _i, _i, _i, _i, _i, _vp, _vp, _vp]),

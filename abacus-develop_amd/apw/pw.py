@@ -424,6 +424,36 @@ def gemm_op(ctx, transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc):
     check(fn(ctx.handle, transa.encode(), transb.encode(), m, n, k, al, _ptr(a), lda, _ptr(b), ldb, be, _ptr(c), ldc))
 
 
+def david_diag_batched(units, david_diag_thr, david_maxiter, ntry_max=5, notconv_max=0, precision="fp64"):
+    """Batched DiagoDavid::diag over independent units.  ``units``: list of
+    (DiagoDavid, HamiltPW, psi, ldPsi, eigenvalue) with psi CUDA complex128 (nband x ldPsi)
+    and eigenvalue a host float64 array filled in place.  Units whose solver and
+    Hamiltonian were built on different contexts run concurrently.  Returns the per-unit
+    iteration counts; each DiagoDavid's ``stats`` is set."""
+    import torch
+    n = len(units)
+    if n == 0:
+        return []
+    lib = units[0][0].lib
+    dav = (C.c_void_p * n)(*[u[0].handle for u in units])
+    ham = (C.c_void_p * n)(*[u[1].handle for u in units])
+    ld = (C.c_int * n)(*[int(u[3]) for u in units])
+    psi = (C.c_void_p * n)(*[_ptr(u[2]) for u in units])
+    eigs = [np.zeros(u[0].nband, dtype=np.float64) for u in units]
+    eig = (C.c_void_p * n)(*[e.ctypes.data for e in eigs])
+    stats = (_lib.DiagStats * n)()
+    status = (C.c_int * n)()
+    for u in units:
+        _dev_c128(u[2])
+    torch.cuda.synchronize()
+    check(lib.apw_david_diag_batched(n, dav, ham, ld, psi, eig, float(david_diag_thr), int(david_maxiter),
+                                     int(ntry_max), int(notconv_max), _PREC[precision], stats, status))
+    for k, u in enumerate(units):
+        u[4][: u[0].nband] = eigs[k]
+        u[0].stats = stats[k].as_dict()
+    return [int(status[k]) for k in range(n)]
+
+
 def dnevx_op(ctx, nstart, ldh, hcc, nbands, eigenvalue, vcc):
     ctx.pre()
     check(ctx.lib.apw_dnevx_op(ctx.handle, nstart, ldh, _ptr(hcc), nbands, _ptr(eigenvalue), _ptr(vcc)))

@@ -22,6 +22,9 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <map>
+#include <string>
+#include <thread>
 #include <vector>
 
 #include "apw_ham.hpp"
@@ -527,6 +530,44 @@ int apw_david_diag(apw_david* dav, apw_ham* ham, apw_hpsi_cb hpsi_cb, apw_spsi_c
             stats->t_total_s = std::chrono::duration<double>(t1 - t0).count();
         }
         return total;
+    });
+}
+
+int apw_david_diag_batched(int nunits, apw_david* const* dav, apw_ham* const* ham, const int* ldpsi,
+                           double* const* psi, double* const* eig, double david_diag_thr, int david_maxiter,
+                           int ntry_max, int notconv_max, int precision, apw_diag_stats* stats, int* status) {
+    return guarded([&] {
+        APW_CHECK(nunits >= 0, "apw_david_diag_batched: nunits=%d", nunits);
+        if (nunits == 0) return (int)APW_OK;
+        APW_CHECK(dav && ham && ldpsi && psi && eig && status, "apw_david_diag_batched: null argument");
+        // units of one context share its stream and workspaces: they run in turn on one host
+        // thread; units of different contexts run concurrently, one host thread each
+        std::map<apw_ctx*, std::vector<int>> groups;
+        for (int u = 0; u < nunits; ++u) {
+            APW_CHECK(dav[u] && ham[u], "apw_david_diag_batched: unit %d has no solver or Hamiltonian", u);
+            APW_CHECK(ham[u]->h.ctx == dav[u]->d.ctx,
+                      "apw_david_diag_batched: unit %d: solver and Hamiltonian on different contexts", u);
+            groups[dav[u]->d.ctx].push_back(u);
+            status[u] = APW_OK;
+        }
+        std::vector<std::string> err(nunits);
+        auto run = [&](const std::vector<int>& units) {
+            for (int u : units) {
+                status[u] = apw_david_diag(dav[u], ham[u], nullptr, nullptr, nullptr, ldpsi[u], psi[u], eig[u],
+                                           david_diag_thr, david_maxiter, ntry_max, notconv_max, precision,
+                                           stats ? &stats[u] : nullptr);
+                if (status[u] < 0) err[u] = apw_last_error();
+            }
+        };
+        std::vector<std::thread> threads;
+        for (auto& g : groups) threads.emplace_back(run, std::cref(g.second));
+        for (auto& t : threads) t.join();
+        for (int u = 0; u < nunits; ++u)
+            if (status[u] < 0) {
+                apw::set_error("apw_david_diag_batched: unit %d: %s", u, err[u].c_str());
+                return status[u];
+            }
+        return (int)APW_OK;
     });
 }
 

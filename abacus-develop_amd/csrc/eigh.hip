@@ -16,6 +16,7 @@
 #include <cmath>
 
 #include <cstdlib>
+#include <mutex>
 
 #include "apw_internal.hpp"
 
@@ -1097,6 +1098,31 @@ __global__ __launch_bounds__(1024) void k_chol_inv(int n, const double2* __restr
 
 }  // namespace
 
+// Cooperative launches of the reduction never overlap one another, whatever stream they are
+// on: each waits for the previous one.  Two partially resident cooperative grids could each
+// spin on workgroups that cannot start; one cooperative grid beside ordinary kernels always
+// gets its workgroups once those finish.
+namespace {
+struct CoopChain {
+    std::mutex m;
+    hipEvent_t ev = nullptr;
+    bool recorded = false;
+};
+CoopChain g_coop[64];
+
+void coop_launch(const void* fn, dim3 grid, dim3 block, void** args, unsigned lds, hipStream_t s) {
+    int dev = 0;
+    APW_HIP(hipGetDevice(&dev));
+    CoopChain& c = g_coop[dev & 63];
+    std::lock_guard<std::mutex> lock(c.m);
+    if (!c.ev) APW_HIP(hipEventCreateWithFlags(&c.ev, hipEventDisableTiming));
+    if (c.recorded) APW_HIP(hipStreamWaitEvent(s, c.ev, 0));
+    APW_HIP(hipLaunchCooperativeKernel(fn, grid, block, args, lds, s));
+    APW_HIP(hipEventRecord(c.ev, s));
+    c.recorded = true;
+}
+}  // namespace
+
 void heevx_reserve(EigWorkspace& ws, int nmax, int nband) {
     const size_t n = (size_t)std::max(nmax, 1);
     ws.a.reserve(2 * n * n);
@@ -1195,7 +1221,7 @@ void heevx_lowest(hipStream_t s, EigWorkspace& ws, int num_cu, int n, int ldh, c
             unsigned* abortp = barp + 1;
             int ncol_r = nc;
             void* args[] = {&Ac, &n, &ncol_r, &Vp, &taup, &dp, &ep, &ring, &abortp};
-            APW_HIP(hipLaunchCooperativeKernel(fn, dim3(w), dim3(ntr), args, (unsigned)lds, s));
+            coop_launch(fn, dim3(w), dim3(ntr), args, (unsigned)lds, s);
         } else {
             // columns in LDS when the share fits, else in A itself (one column per
             // workgroup slot, L2/Infinity-Cache resident); the ring as above
@@ -1217,7 +1243,7 @@ void heevx_lowest(hipStream_t s, EigWorkspace& ws, int num_cu, int n, int ldh, c
             APW_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
             unsigned* abortp = barp + 1;
             void* args[] = {&A, &n, &ncol, &Vp, &taup, &dp, &ep, &ring, &abortp};
-            APW_HIP(hipLaunchCooperativeKernel(fn, dim3(nwg), dim3(NT), args, (unsigned)lds, s));
+            coop_launch(fn, dim3(nwg), dim3(NT), args, (unsigned)lds, s);
         }
     } else {
         // n == 1: T = [Re a00]; the real part is the first double of A

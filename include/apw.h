@@ -225,6 +225,19 @@ int apw_david_diag(apw_david* dav, apw_ham* ham, apw_hpsi_cb hpsi_cb, apw_spsi_c
                    int david_maxiter, int ntry_max, int notconv_max, int precision,
                    apw_diag_stats* stats);
 
+/* Batched form: nunits independent DiagoDavid::diag calls (k-points or cells, uneven sizes),
+ * each unit with its own solver, Hamiltonian (library H|psi>), psi (device, ld ldpsi[u]),
+ * eig (host) and stats (optional array).  Units created on different contexts run
+ * concurrently (one host thread and one stream each); units sharing a context run in
+ * turn.  status[u]: the unit's iteration count or negative status.  Returns APW_OK, or the
+ * first failing unit's status with its message in apw_last_error().  No single reference
+ * function corresponds: HSolverPW::solve loops diag over k-points (hsolver_pw.cpp:472-545);
+ * this is the proposed apw_david_diag_batched of the C-ABI boundary. */
+int apw_david_diag_batched(int nunits, apw_david* const* dav, apw_ham* const* ham,
+                           const int* ldpsi, double* const* psi, double* const* eig,
+                           double david_diag_thr, int david_maxiter, int ntry_max,
+                           int notconv_max, int precision, apw_diag_stats* stats, int* status);
+
 /* dngvx_op<complex<double>, DEVICE_GPU> (dngvd_op.h:67-81): lowest m eigenpairs of the
  * Hermitian-definite pencil A x = lambda B x (nstart x nstart, ld ldh, both triangles;
  * the CPU op calls zhegvx).  eigenvalue device double[m]; vcc device c128 (ld ldh),

@@ -501,3 +501,54 @@ def test_david_callback_dense(torch, ctx, n, nband, ndim):
     # eigenvalues is a convergence property, bounded relative to their scale
     assert np.abs(eig - w[:nband]).max() < max(1e-8, 1e-9 * np.abs(w[:nband]).max())
     dav.close()
+
+
+# ------------------------------------------------------------------ batched form
+def test_david_batched_kpoints_match_sequential(torch):
+    """Four k-points of the 4x4x4 mesh (uneven plane-wave counts), one context (stream) each,
+    solved concurrently by apw_david_diag_batched: the same iterations and eigenvalues as
+    the same solves one after another on one context."""
+    from apw import pw
+    systems = [configs.make("kpts16", ik=k) for k in (0, 5, 21, 37)]
+    c0 = pw.Context(0)
+    seq = []
+    for s in systems:
+        ham = pw.HamiltPW(c0, s)
+        dav = pw.DiagoDavid(c0, s.precondition, s.nband, s.npw, s.david_ndim)
+        psi = dev(torch, s.psi0)
+        eig = np.zeros(s.nband)
+        it = dav.diag(ham, None, s.npwx, psi, eig, 1e-9, 500, 5, 0)
+        seq.append((it, eig.copy(), host(c0, psi)))
+        dav.close()
+        ham.close()
+    ctxs = [pw.Context(0) for _ in systems]
+    units = []
+    for c, s in zip(ctxs, systems):
+        ham = pw.HamiltPW(c, s)
+        dav = pw.DiagoDavid(c, s.precondition, s.nband, s.npw, s.david_ndim)
+        units.append((dav, ham, dev(torch, s.psi0), s.npwx, np.zeros(s.nband)))
+    its = pw.david_diag_batched(units, 1e-9, 500)
+    for k, (dav, ham, psi, ld, eig) in enumerate(units):
+        assert its[k] == seq[k][0]
+        assert np.abs(eig - seq[k][1]).max() < 1e-12
+        P = host(ctxs[k], psi)[:, : systems[k].npw]
+        assert np.abs(np.abs(P) - np.abs(seq[k][2][:, : systems[k].npw])).max() < 1e-8
+        dav.close()
+        ham.close()
+    for c in ctxs + [c0]:
+        c.close()
+
+
+def test_david_batched_rejects_mixed_contexts(torch):
+    from apw import pw
+    from apw._lib import ApwError
+    s = configs.make("tiny")
+    ca, cb = pw.Context(0), pw.Context(0)
+    ham = pw.HamiltPW(ca, s)
+    dav = pw.DiagoDavid(cb, s.precondition, s.nband, s.npw, s.david_ndim)
+    with pytest.raises(ApwError, match="different contexts"):
+        pw.david_diag_batched([(dav, ham, dev(torch, s.psi0), s.npwx, np.zeros(s.nband))], 1e-9, 500)
+    dav.close()
+    ham.close()
+    ca.close()
+    cb.close()
