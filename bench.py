@@ -253,7 +253,45 @@ def measure_next(pw, ctx, ham, s, psi0_dev, eig_david, reps=5):
     mb = 16.0 * s.npw * s.nband / 1e6
     out["wfc_io"] = {"MB": round(mb, 1), "write_MB_s": round(mb / tw, 1), "read_MB_s": round(mb / tr, 1),
                      "round_trip_identical": same, "filesystem": "tempfile default (host)"}
+    out["batched_kpoints"] = measure_batched(pw)
     return out
+
+
+def measure_batched(pw, nk=16, ncontexts=(1, 2)):
+    """The batched form on k-points of the 16-atom 4x4x4 mesh (every fourth of the 64,
+    uneven plane-wave counts): apw_david_diag_batched with the units spread over 1 context
+    (one after another) and over 2 (two streams, two host threads), FP64 to 1e-9 Ry."""
+    import numpy as np
+    import torch
+    from apw import configs
+    systems = [configs.make("kpts16", ik=k) for k in range(0, 64, 64 // nk)]
+    res, eig_ref = {}, None
+    for nc in ncontexts:
+        ctxs = [pw.Context(0) for _ in range(nc)]
+        units = []
+        for i, s in enumerate(systems):
+            c = ctxs[i % nc]
+            units.append((pw.DiagoDavid(c, s.precondition, s.nband, s.npw, s.david_ndim), pw.HamiltPW(c, s),
+                          torch.from_numpy(np.ascontiguousarray(s.psi0)).cuda(), s.npwx, np.zeros(s.nband)))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        its = pw.david_diag_batched(units, 1e-9, 500)
+        for c in ctxs:
+            c.synchronize()
+        dt = time.perf_counter() - t0
+        eig = np.concatenate([u[4] for u in units])
+        if eig_ref is None:
+            eig_ref = eig
+        res[f"contexts_{nc}"] = {"time_s": round(dt, 4), "iterations": int(sum(its)),
+                                 "max_abs_eig_diff_vs_1_context_Ry": float(np.abs(eig - eig_ref).max())}
+        for u in units:
+            u[0].close()
+            u[1].close()
+        for c in ctxs:
+            c.close()
+    res["kpoints"] = len(systems)
+    res["npw_range"] = [int(min(s.npw for s in systems)), int(max(s.npw for s in systems))]
+    return res
 
 
 def main():
