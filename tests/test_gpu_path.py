@@ -552,3 +552,34 @@ def test_david_batched_rejects_mixed_contexts(torch):
     ham.close()
     ca.close()
     cb.close()
+
+
+def test_david_batched_uneven_cells(torch):
+    """Cells of the uneven batch (20^3 to 48^3 grids, 437 to 6619 plane waves, 6 to 54
+    bands) on two contexts, two units each: the same results as one after another."""
+    from apw import pw
+    systems = [configs.make_cell(i) for i in (1, 3, 4, 7)]
+    c0 = pw.Context(0)
+    seq = []
+    for s in systems:
+        ham = pw.HamiltPW(c0, s)
+        dav = pw.DiagoDavid(c0, s.precondition, s.nband, s.npw, s.david_ndim)
+        eig = np.zeros(s.nband)
+        it = dav.diag(ham, None, s.npwx, dev(torch, s.psi0), eig, 1e-9, 500, 5, 0)
+        seq.append((it, eig.copy()))
+        dav.close()
+        ham.close()
+    ctxs = [pw.Context(0), pw.Context(0)]
+    units = []
+    for i, s in enumerate(systems):
+        c = ctxs[i % 2]
+        units.append((pw.DiagoDavid(c, s.precondition, s.nband, s.npw, s.david_ndim), pw.HamiltPW(c, s),
+                      dev(torch, s.psi0), s.npwx, np.zeros(s.nband)))
+    its = pw.david_diag_batched(units, 1e-9, 500)
+    for k, u in enumerate(units):
+        assert its[k] == seq[k][0]
+        assert np.abs(u[4] - seq[k][1]).max() < 1e-12
+        u[0].close()
+        u[1].close()
+    for c in ctxs + [c0]:
+        c.close()
