@@ -74,18 +74,25 @@ def make_cell(i: int, nband: int | None = None):
     nat = int(rng.integers(2, 33))
     a = float(rng.uniform(5.13, 15.4))
     ecut = float(rng.uniform(30.0, 50.0))
-    # random positions with a minimum separation of 1.6 bohr (periodic images included)
-    pos = []
-    while len(pos) < nat:
-        p = rng.uniform(0.0, 1.0, 3)
-        ok = True
-        for q in pos:
-            d = p - q
-            d -= np.round(d)
-            if np.linalg.norm(d * a) < 1.6:
-                ok = False
-                break
-        if ok:
-            pos.append(p)
+    # random positions with a minimum separation of 1.6 bohr (periodic images included); a
+    # cell too dense for random placement (cell 15: 31 atoms at a = 5.46) is enlarged by 5 %
+    # and placed again, so every cell terminates
+    while True:
+        pos, tries = [], 0
+        while len(pos) < nat and tries < 200000:
+            tries += 1
+            p = rng.uniform(0.0, 1.0, 3)
+            ok = True
+            for q in pos:
+                d = p - q
+                d -= np.round(d)
+                if np.linalg.norm(d * a) < 1.6:
+                    ok = False
+                    break
+            if ok:
+                pos.append(p)
+        if len(pos) == nat:
+            break
+        a *= 1.05
     lat = Lattice(a, np.eye(3))
     return build_system(f"cell{i}", lat, np.array(pos), ecut, nband or 3 * nat, 4, 5000 + i)

@@ -254,7 +254,50 @@ def measure_next(pw, ctx, ham, s, psi0_dev, eig_david, reps=5):
     out["wfc_io"] = {"MB": round(mb, 1), "write_MB_s": round(mb / tw, 1), "read_MB_s": round(mb / tr, 1),
                      "round_trip_identical": same, "filesystem": "tempfile default (host)"}
     out["batched_kpoints"] = measure_batched(pw)
+    out["batched_cells"] = measure_cells(pw)
     return out
+
+
+def measure_cells(pw, ncells=32):
+    """The uneven-cell batch (cells 0..ncells-1 of the 128-cell set: 2-32 atoms, 18^3-64^3
+    grids, nbands = 3 atoms): the plain FP64 per-cell loop on one context vs the FP32-mixed
+    batched solve on two contexts, with the per-cell eigenvalue difference between them."""
+    import numpy as np
+    import torch
+    from apw import configs
+    systems = [configs.make_cell(i) for i in range(ncells)]
+    res, eig64 = {}, None
+    for label, nc, prec in (("fp64_loop_1_context", 1, "fp64"), ("fp32_mixed_batched_2_contexts", 2, "fp32_mixed")):
+        ctxs = [pw.Context(0) for _ in range(nc)]
+        units = []
+        for i, s in enumerate(systems):
+            c = ctxs[i % nc]
+            units.append((pw.DiagoDavid(c, s.precondition, s.nband, s.npw, s.david_ndim), pw.HamiltPW(c, s),
+                          torch.from_numpy(np.ascontiguousarray(s.psi0)).cuda(), s.npwx, np.zeros(s.nband)))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        its = pw.david_diag_batched(units, 1e-9, 500, precision=prec)
+        for c in ctxs:
+            c.synchronize()
+        dt = time.perf_counter() - t0
+        eigs = [u[4].copy() for u in units]
+        entry = {"time_s": round(dt, 4), "iterations": int(sum(its))}
+        if eig64 is None:
+            eig64 = eigs
+        else:
+            d = [float(np.abs(a - b).max()) for a, b in zip(eigs, eig64)]
+            entry["max_abs_eig_diff_vs_fp64_Ry"] = max(d)
+            entry["cells_within_1e-9_Ry"] = int(sum(x < 1e-9 for x in d))
+        res[label] = entry
+        for u in units:
+            u[0].close()
+            u[1].close()
+        for c in ctxs:
+            c.close()
+    res["cells"] = ncells
+    res["npw_range"] = [int(min(s.npw for s in systems)), int(max(s.npw for s in systems))]
+    res["grids"] = sorted({f"{s.nx}^3" for s in systems}, key=lambda g: int(g.split("^")[0]))
+    return res
 
 
 def measure_batched(pw, nk=16, ncontexts=(1, 2)):
