@@ -160,6 +160,24 @@ __device__ inline bool ring_get(const double2* slot, double2& v) {
            (unsigned long long)__double_as_longlong(v.y) != RING_EMPTY;
 }
 
+// 16-byte forms: one vector buffer load / store per value with the device-scope cache
+// policy (sc1, the bits the 8-byte device-scope atomics above carry).  Each 8-byte half
+// still validates itself, so a load that sees one half written and the other not simply
+// polls again.  r spans one step's slot (2n values); i is the value's index in it.
+typedef unsigned ring_u4 __attribute__((ext_vector_type(4)));
+constexpr int RING_SC1 = 16;
+__device__ inline __amdgpu_buffer_rsrc_t ring_rsrc(const double2* slot, int count) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<double2*>(slot), 0, count * (int)sizeof(double2), 0x00020000);
+}
+__device__ inline void ring_put16(__amdgpu_buffer_rsrc_t r, int i, double2 v) {
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(ring_u4, v), r, (unsigned)i * 16u, 0, RING_SC1);
+}
+__device__ inline bool ring_get16(__amdgpu_buffer_rsrc_t r, int i, double2& v) {
+    v = __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, (unsigned)i * 16u, 0, RING_SC1));
+    return (unsigned long long)__double_as_longlong(v.x) != RING_EMPTY &&
+           (unsigned long long)__double_as_longlong(v.y) != RING_EMPTY;
+}
+
 // Householder tridiagonalisation (lower) with the trailing matrix resident in LDS.
 // Column c belongs to workgroup c % nwg (slot c / nwg) for the whole reduction and lives
 // in that workgroup's LDS, so the hemv reads and rank-2 writes never touch global memory.
@@ -386,7 +404,7 @@ __global__ __launch_bounds__(NTR) void k_tridiag_reg(const double2* __restrict__
     // the first reflector), optionally after the rank-2 update with (v, y) of step k;
     // published into the ring slot of step k + 1 with the owner's copy of column k + 2.
     auto update_and_project = [&](int k, bool update) {
-        double2* slot = ring + (size_t)(k + 1) * 2 * n;
+        const __amdgpu_buffer_rsrc_t rs = ring_rsrc(ring + (size_t)(k + 1) * 2 * n, 2 * n);
         const int lc1 = (k + 2 - bid + nwg - 1) / nwg;
         for (int lc = lc1 + wave; lc < ncol; lc += NW) {
             const int c = bid + lc * nwg;
@@ -417,13 +435,13 @@ __global__ __launch_bounds__(NTR) void k_tridiag_reg(const double2* __restrict__
             }
             pr = wave_sum(pr);
             pi = wave_sum(pi);
-            if (k + 2 < n && lane == 0) ring_put(slot + (c - (k + 2)), make_double2(pr, pi));
+            if (k + 2 < n && lane == 0) ring_put16(rs, c - (k + 2), make_double2(pr, pi));
             if (c == k + 2) {
                 // owner: publish the updated column k + 2 (rows k+2 .. n-1)
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
                     const int i = lane + 64 * r;
-                    if (i >= k + 2 && i < n) ring_put(slot + n + (i - (k + 2)), col[i]);
+                    if (i >= k + 2 && i < n) ring_put16(rs, n + (i - (k + 2)), col[i]);
                 }
             }
         }
@@ -455,18 +473,26 @@ __global__ __launch_bounds__(NTR) void k_tridiag_reg(const double2* __restrict__
     for (int r = 0; r < R; ++r) v[r] = w[r];
     for (int k = 0; k < n - 1; ++k) {
         const int m = n - k - 1;
-        const double2* slot = ring + (size_t)k * 2 * n;
+        const __amdgpu_buffer_rsrc_t rs = ring_rsrc(ring + (size_t)k * 2 * n, 2 * n);
         double2* g = gs + (size_t)(k & 1) * 2 * n;
         // ---- gather p_k and column k+1 ----
+        // (a thread's q-th value, once valid, is not polled again: bit q of got; at most
+        // ceil(2n / NTR) <= 8 values a thread for the sizes this form takes)
         {
-            unsigned spins = 0;
+            unsigned spins = 0, got = 0;
             while (true) {
                 bool mine = true;
-                for (int i = threadIdx.x; i < 2 * m; i += NTR) {
+                int q = 0;
+                for (int i = threadIdx.x; i < 2 * m; i += NTR, ++q) {
+                    if ((got >> q) & 1u) continue;
                     const int ii = i < m ? i : n + (i - m);
                     double2 val;
-                    if (ring_get(slot + ii, val)) g[ii] = val;
-                    else mine = false;
+                    if (ring_get16(rs, ii, val)) {
+                        g[ii] = val;
+                        got |= 1u << q;
+                    } else {
+                        mine = false;
+                    }
                 }
                 if (__syncthreads_and(mine)) break;
                 // the abort word and the spin bound every 32 failed polls: its load is a
