@@ -95,6 +95,7 @@ PROTOTYPES = {
     "apw_psi_to_rho": (_i, [_vp, _vp, _i, _i, C.POINTER(_d), _d, _vp, _i]),
     "apw_elecstate_pw_op": (_i, [_vp, _i, _i, _d, C.POINTER(_vp), _vp]),
     "apw_elecstate_pw_op_nc": (_i, [_vp, _i, _i, _i, _d, C.POINTER(_vp), _vp, _vp]),
+    "apw_zgemm_x32": (_i, [_vp, _ch, _ch, _i, _i, _i, C.POINTER(_d), _vp, _i, _vp, _i, C.POINTER(_d), _vp, _i]),
     "apw_cgemm": (_i, [_vp, _ch, _ch, _i, _i, _i, C.POINTER(C.c_float), _vp, _i, _vp, _i, C.POINTER(C.c_float), _vp, _i]),
     "apw_dnevx_op": (_i, [_vp, _i, _i, _vp, _i, _vp, _vp]),
     "apw_probe_mfma_layout": (_i, [_vp, _vp]),

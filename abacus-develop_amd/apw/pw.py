@@ -414,10 +414,14 @@ def fft3d(ctx, nx, ny, nz, box, direction):
     check(ctx.lib.apw_fft3d(ctx.handle, nx, ny, nz, _ptr(box), int(direction)))
 
 
-def gemm_op(ctx, transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc):
-    """gemm_op<complex<double>|complex<float>, DEVICE_GPU>: column-major, dtype from ``c``."""
+def gemm_op(ctx, transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, x32=False):
+    """gemm_op<complex<double>|complex<float>, DEVICE_GPU>: column-major, dtype from ``c``.
+    ``x32``: a complex128 product on the FP32 matrix cores (apw_zgemm_x32)."""
     single = str(getattr(c, "dtype", "")) == "torch.complex64"
-    ct, fn = (C.c_float, ctx.lib.apw_cgemm) if single else (C.c_double, ctx.lib.apw_zgemm)
+    if x32 and single:
+        raise ValueError("gemm_op: x32 takes complex128 data")
+    ct, fn = (C.c_float, ctx.lib.apw_cgemm) if single else (
+        C.c_double, ctx.lib.apw_zgemm_x32 if x32 else ctx.lib.apw_zgemm)
     al = (ct * 2)(complex(alpha).real, complex(alpha).imag)
     be = (ct * 2)(complex(beta).real, complex(beta).imag)
     ctx.pre()

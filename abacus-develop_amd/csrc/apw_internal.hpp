@@ -190,6 +190,12 @@ template <class R>
 void gemm(hipStream_t s, GemmWorkspace& ws, int num_cu, char ta, char tb, int m, int n, int k,
           double alpha_re, double alpha_im, const R* A, int lda, const R* B, int ldb,
           double beta_re, double beta_im, R* C, int ldc, Prof* prof = nullptr);
+// Same product of c128 data on the FP32 matrix cores: operands rounded to FP32 on their way
+// into LDS, FP32 sums folded into FP64 ones every 32-64 k values, C written in FP64.  The
+// error is that of the FP32 operands (relative ~1e-7 of sum |a||b|), not of a long FP32 sum.
+void gemm_x32(hipStream_t s, GemmWorkspace& ws, int num_cu, char ta, char tb, int m, int n, int k,
+              double alpha_re, double alpha_im, const double* A, int lda, const double* B, int ldb,
+              double beta_re, double beta_im, double* C, int ldc, Prof* prof = nullptr);
 // Sizes the split-K partials for every call on this GPU (num_cu CUs), so gemm allocates
 // nothing afterwards.
 void gemm_reserve(GemmWorkspace& ws, int num_cu);

@@ -216,6 +216,19 @@ int apw_zgemm(apw_ctx* ctx, char transa, char transb, int m, int n, int k, const
     });
 }
 
+int apw_zgemm_x32(apw_ctx* ctx, char transa, char transb, int m, int n, int k, const double* alpha,
+                  const double* a, int lda, const double* b, int ldb, const double* beta, double* c, int ldc) {
+    return guarded([&] {
+        APW_CHECK(ctx && alpha && beta && c, "apw_zgemm_x32: null argument");
+        APW_CHECK(k == 0 || (a && b), "apw_zgemm_x32: null A/B");
+        apw::DeviceGuard g(ctx->device);
+        apw::gemm_x32(ctx->stream, ctx->gws, ctx->num_cu, transa, transb, m, n, k, alpha[0], alpha[1], a, lda, b, ldb,
+                      beta[0], beta[1], c, ldc, &ctx->prof);
+        APW_HIP(hipStreamSynchronize(ctx->stream));
+        return (int)APW_OK;
+    });
+}
+
 int apw_cgemm(apw_ctx* ctx, char transa, char transb, int m, int n, int k, const float* alpha, const float* a,
               int lda, const float* b, int ldb, const float* beta, float* c, int ldc) {
     return guarded([&] {

@@ -57,15 +57,23 @@ constexpr int BKMAX = 16, NT = 256;
 // after the next is loaded into registers and the next one is written to stage s^1, so a
 // k step costs one workgroup barrier.  LDS rows are padded by one element so that the
 // k-fastest operand stores (16 lanes down a column) fall in distinct banks.
-template <class R, int TA, int TB, int TM, int TN, int BK, int MAP>
+// S: element type of A, B, C and the split-K partials.  S = R is the plain form.  R = float
+// with S = double is the FP32-MFMA form of an FP64 product: operands rounded to FP32 as
+// they are staged in LDS, FP32 accumulators folded into FP64 ones every XFOLD k steps
+// (32 or 64 k values), so the rounding is that of the FP32 operands, not of a long FP32
+// sum; C and the partials stay FP64.
+constexpr int XFOLD = 4;
+template <class R, class S, int TA, int TB, int TM, int TN, int BK, int MAP>
 __global__ __launch_bounds__(NT) void zgemm_kernel(int M, int N, int K, int kchunk,
-                                                   const typename Mfma<R>::c_t* __restrict__ A, int lda,
-                                                   const typename Mfma<R>::c_t* __restrict__ B, int ldb,
-                                                   typename Mfma<R>::c_t* __restrict__ C, int ldc,
+                                                   const typename Mfma<S>::c_t* __restrict__ A, int lda,
+                                                   const typename Mfma<S>::c_t* __restrict__ B, int ldb,
+                                                   typename Mfma<S>::c_t* __restrict__ C, int ldc,
                                                    double ar, double ai, double br, double bi,
-                                                   typename Mfma<R>::c_t* __restrict__ part) {
-    using c_t = typename Mfma<R>::c_t;
+                                                   typename Mfma<S>::c_t* __restrict__ part) {
+    using c_t = typename Mfma<S>::c_t;
     using acc_t = typename Mfma<R>::acc_t;
+    constexpr bool X32 = !std::is_same<R, S>::value;
+    static_assert(!X32 || (std::is_same<R, float>::value && std::is_same<S, double>::value), "x32: f32 on f64");
     constexpr int BM = 32 * TM, BN = 32 * TN;
     constexpr int NA = BM * BK / NT, NB = BN * BK / NT;
     constexpr int LA = BM + 1, LB = BN + 1;
@@ -97,13 +105,34 @@ __global__ __launch_bounds__(NT) void zgemm_kernel(int M, int N, int K, int kchu
     const int kend = min(K, kbeg + kchunk);
 
     acc_t acc_re[TN][TM], acc_im[TN][TM];
+    d4_t wide_re[X32 ? TN : 1][X32 ? TM : 1], wide_im[X32 ? TN : 1][X32 ? TM : 1];   // X32: FP64 sums
 #pragma unroll
     for (int a = 0; a < TN; ++a)
 #pragma unroll
         for (int b = 0; b < TM; ++b) {
             acc_re[a][b] = acc_t{0, 0, 0, 0};
             acc_im[a][b] = acc_t{0, 0, 0, 0};
+            if constexpr (X32) {
+                wide_re[a][b] = d4_t{0, 0, 0, 0};
+                wide_im[a][b] = d4_t{0, 0, 0, 0};
+            }
         }
+    auto fold = [&]() {
+        if constexpr (X32) {
+#pragma unroll
+            for (int a = 0; a < TN; ++a)
+#pragma unroll
+                for (int b = 0; b < TM; ++b) {
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        wide_re[a][b][r] += (double)acc_re[a][b][r];
+                        wide_im[a][b][r] += (double)acc_im[a][b][r];
+                    }
+                    acc_re[a][b] = acc_t{0, 0, 0, 0};
+                    acc_im[a][b] = acc_t{0, 0, 0, 0};
+                }
+        }
+    };
 
     // Prefetch registers.  The loads are unconditional (an out-of-range element reads
     // element 0 instead, which is valid for k > 0) so the compiler never has to wait on
@@ -152,8 +181,8 @@ __global__ __launch_bounds__(NT) void zgemm_kernel(int M, int N, int K, int kchu
             int mm, kk;
             if (TA == 0) { mm = e % BM; kk = e / BM; } else { kk = e % BK; mm = e / BK; }
             const bool ok = (oka >> i) & 1u;
-            As[st][0][kk][mm] = masked(ra[i].x, ok, false);
-            As[st][1][kk][mm] = masked(ra[i].y, ok, TA == 2);
+            As[st][0][kk][mm] = masked((R)ra[i].x, ok, false);
+            As[st][1][kk][mm] = masked((R)ra[i].y, ok, TA == 2);
         }
 #pragma unroll
         for (int i = 0; i < NB; ++i) {
@@ -161,8 +190,8 @@ __global__ __launch_bounds__(NT) void zgemm_kernel(int M, int N, int K, int kchu
             int nn, kk;
             if (TB == 0) { kk = e % BK; nn = e / BK; } else { nn = e % BN; kk = e / BN; }
             const bool ok = (okb >> i) & 1u;
-            Bs[st][0][kk][nn] = masked(rb[i].x, ok, false);
-            Bs[st][1][kk][nn] = masked(rb[i].y, ok, TB == 2);
+            Bs[st][0][kk][nn] = masked((R)rb[i].x, ok, false);
+            Bs[st][1][kk][nn] = masked((R)rb[i].y, ok, TB == 2);
         }
     };
 
@@ -172,7 +201,7 @@ __global__ __launch_bounds__(NT) void zgemm_kernel(int M, int N, int K, int kchu
         fetch(kbeg + BK);
     }
     __syncthreads();
-    int st = 0;
+    int st = 0, nfold = 0;
     for (int k0 = kbeg; k0 < kend; k0 += BK) {
 #pragma unroll
         for (int ks = 0; ks < BK / 4; ++ks) {
@@ -204,9 +233,16 @@ __global__ __launch_bounds__(NT) void zgemm_kernel(int M, int N, int K, int kchu
         // the loads read element 0
         stage(st ^ 1);
         fetch(k0 + 2 * BK);
+        if constexpr (X32) {
+            if (++nfold == XFOLD) {
+                fold();
+                nfold = 0;
+            }
+        }
         __syncthreads();
         st ^= 1;
     }
+    fold();
 
     // epilogue: lane holds D[n = drow(lane, r)][m = lane%16] of each sub-tile
 #pragma unroll
@@ -218,12 +254,18 @@ __global__ __launch_bounds__(NT) void zgemm_kernel(int M, int N, int K, int kchu
                 const int gm = m0 + wm * 16 * TM + tm * 16 + (lane & 15);
                 const int gn = n0 + wn * 16 * TN + tn * 16 + Mfma<R>::drow(lane, r);
                 if (gm >= M || gn >= N) continue;
-                const double xr = (double)acc_re[tn][tm][r];
-                const double xi = (double)acc_im[tn][tm][r];
+                double xr, xi;
+                if constexpr (X32) {
+                    xr = wide_re[tn][tm][r];
+                    xi = wide_im[tn][tm][r];
+                } else {
+                    xr = (double)acc_re[tn][tm][r];
+                    xi = (double)acc_im[tn][tm][r];
+                }
                 if (part) {
                     c_t v;
-                    v.x = (R)xr;
-                    v.y = (R)xi;
+                    v.x = (S)xr;
+                    v.y = (S)xi;
                     part[(size_t)blockIdx.z * M * N + gm + (size_t)gn * M] = v;
                 } else {
                     double cr = ar * xr - ai * xi;
@@ -235,8 +277,8 @@ __global__ __launch_bounds__(NT) void zgemm_kernel(int M, int N, int K, int kchu
                         ci += br * (double)c.y + bi * (double)c.x;
                     }
                     c_t v;
-                    v.x = (R)cr;
-                    v.y = (R)ci;
+                    v.x = (S)cr;
+                    v.y = (S)ci;
                     *cp = v;
                 }
             }
@@ -304,25 +346,25 @@ int opcode(char t) {
     }
 }
 
-template <class R, int TA, int TB, int TM, int TN, int BK, int MAP>
-void launch(ProfScope& ps, hipStream_t s, int splits, int M, int N, int K, int kchunk, const R* A, int lda,
-            const R* B, int ldb, R* C, int ldc, double ar, double ai, double br, double bi, R* part) {
-    using c_t = typename Mfma<R>::c_t;
+template <class R, class S, int TA, int TB, int TM, int TN, int BK, int MAP>
+void launch(ProfScope& ps, hipStream_t s, int splits, int M, int N, int K, int kchunk, const S* A, int lda,
+            const S* B, int ldb, S* C, int ldc, double ar, double ai, double br, double bi, S* part) {
+    using c_t = typename Mfma<S>::c_t;
     const int tm = (M + 32 * TM - 1) / (32 * TM), tn = (N + 32 * TN - 1) / (32 * TN);
     const dim3 grid = MAP == 2 ? dim3((unsigned)(((tm + 7) / 8) * 8 * tn), 1, splits)
                                : (MAP == 1 ? dim3(tn, tm, splits) : dim3(tm, tn, splits));
-    APW_PLAUNCH(ps, (zgemm_kernel<R, TA, TB, TM, TN, BK, MAP>), grid, dim3(NT), 0, s, M, N, K, kchunk,
+    APW_PLAUNCH(ps, (zgemm_kernel<R, S, TA, TB, TM, TN, BK, MAP>), grid, dim3(NT), 0, s, M, N, K, kchunk,
                        reinterpret_cast<const c_t*>(A), lda, reinterpret_cast<const c_t*>(B), ldb,
                        reinterpret_cast<c_t*>(C), ldc, ar, ai, br, bi, reinterpret_cast<c_t*>(part));
 }
 
-}  // namespace
-
-template <class R>
-void gemm(hipStream_t s, GemmWorkspace& ws, int num_cu, char ta, char tb, int m, int n, int k,
-          double alpha_re, double alpha_im, const R* A, int lda, const R* B, int ldb,
-          double beta_re, double beta_im, R* C, int ldc, Prof* prof) {
-    using c_t = typename Mfma<R>::c_t;
+// R: MFMA type; S: storage type of A, B, C (S = R, or R = float on S = double, see zgemm_kernel)
+template <class R, class S>
+void gemm_impl(hipStream_t s, GemmWorkspace& ws, int num_cu, char ta, char tb, int m, int n, int k,
+               double alpha_re, double alpha_im, const S* A, int lda, const S* B, int ldb,
+               double beta_re, double beta_im, S* C, int ldc, Prof* prof) {
+    using c_t = typename Mfma<S>::c_t;
+    constexpr bool X32 = !std::is_same<R, S>::value;
     const int oa = opcode(ta), ob = opcode(tb);
     APW_CHECK(oa >= 0 && ob >= 0, "gemm: bad trans '%c' '%c'", ta, tb);
     APW_CHECK(m >= 0 && n >= 0 && k >= 0, "gemm: negative size");
@@ -333,7 +375,7 @@ void gemm(hipStream_t s, GemmWorkspace& ws, int num_cu, char ta, char tb, int m,
     if (k == 0 || (alpha_re == 0.0 && alpha_im == 0.0)) {
         const size_t total = (size_t)m * n;
         const int blocks = (int)std::min<size_t>((total + 255) / 256, 65536);
-        hipLaunchKernelGGL(scale_c<R>, dim3(blocks), dim3(256), 0, s, m, n, reinterpret_cast<c_t*>(C), ldc,
+        hipLaunchKernelGGL(scale_c<S>, dim3(blocks), dim3(256), 0, s, m, n, reinterpret_cast<c_t*>(C), ldc,
                            beta_re, beta_im);
         APW_LAUNCH_CHECK();
         return;
@@ -371,43 +413,47 @@ void gemm(hipStream_t s, GemmWorkspace& ws, int num_cu, char ta, char tb, int m,
         return e ? std::atoi(e) : -1;
     }();
     if (variant_env >= 0 && variant_env <= 5) variant = variant_env;
+    // FP32-MFMA on FP64 storage: the 128 x 64 tiles would hold 2 x 128 accumulator registers
+    // (FP32 and FP64 sums); the 128 x 32 tile in the same XCD-aware order takes their place
+    if (X32 && (variant == 2 || variant == 3)) variant = 4;
     static const int VBM[6] = {64, 64, 128, 128, 128, 64}, VBN[6] = {64, 32, 64, 64, 32, 128};
     static const int VOCC[6] = {3, 6, 3, 3, 3, 2};   // resident workgroups per CU (measured best)
     int splits = inner ? split_count(VBM[variant], VBN[variant], VOCC[variant]) : 1;
     int kchunk = (k + splits - 1) / splits;
     kchunk = (kchunk + BKMAX - 1) / BKMAX * BKMAX;
     splits = (k + kchunk - 1) / kchunk;
-    R* part = nullptr;
+    S* part = nullptr;
     if (splits > 1) {
-        const size_t need = (size_t)splits * m * n * 2 * sizeof(R) / sizeof(double) + 2;
+        const size_t need = (size_t)splits * m * n * 2 * sizeof(S) / sizeof(double) + 2;
         ws.part.reserve(need);
-        part = reinterpret_cast<R*>(ws.part.get());
+        part = reinterpret_cast<S*>(ws.part.get());
     }
     static const bool trace = std::getenv("APW_GEMM_TRACE") != nullptr;   // tuning aid: one line per call
     if (trace)
         std::fprintf(stderr, "apw_gemm %s %c%c m=%d n=%d k=%d variant=%d splits=%d\n",
-                     std::is_same<R, double>::value ? "f64" : "f32", ta, tb, m, n, k, variant, splits);
-    const double esz = 2.0 * sizeof(R);
+                     X32 ? "x32" : (std::is_same<R, double>::value ? "f64" : "f32"), ta, tb, m, n, k, variant, splits);
+    const double esz = 2.0 * sizeof(S);
     const bool beta_nz = (beta_re != 0.0 || beta_im != 0.0);
-    const char* pname = std::is_same<R, double>::value ? (splits > 1 ? "zgemm_f64_splitk" : "zgemm_f64_tiled")
-                                                        : (splits > 1 ? "cgemm_f32_splitk" : "cgemm_f32_tiled");
+    const char* pname = X32 ? (splits > 1 ? "zgemm_x32_splitk" : "zgemm_x32_tiled")
+                            : std::is_same<R, double>::value ? (splits > 1 ? "zgemm_f64_splitk" : "zgemm_f64_tiled")
+                                                             : (splits > 1 ? "cgemm_f32_splitk" : "cgemm_f32_tiled");
     ProfScope ps(prof, s, pname,
                  esz * ((double)m * k + (double)k * n + (double)m * n * (splits > 1 ? splits : (beta_nz ? 2 : 1))),
                  8.0 * m * n * (double)k, true);
 #define APW_GEMM_CASE(X, Y)                                                                                  \
     if (oa == X && ob == Y) {                                                                               \
         switch (variant) {                                                                                  \
-            case 1: launch<R, X, Y, 2, 1, 8, 0>(ps, s, splits, m, n, k, kchunk, A, lda, B, ldb, C, ldc, alpha_re,    \
+            case 1: launch<R, S, X, Y, 2, 1, 8, 0>(ps, s, splits, m, n, k, kchunk, A, lda, B, ldb, C, ldc, alpha_re,    \
                                                alpha_im, beta_re, beta_im, part); break;                    \
-            case 2: launch<R, X, Y, 4, 2, 8, 1>(ps, s, splits, m, n, k, kchunk, A, lda, B, ldb, C, ldc, alpha_re,    \
+            case 2: launch<R, S, X, Y, 4, 2, 8, 1>(ps, s, splits, m, n, k, kchunk, A, lda, B, ldb, C, ldc, alpha_re,    \
                                                alpha_im, beta_re, beta_im, part); break;                    \
-            case 3: launch<R, X, Y, 4, 2, 8, 2>(ps, s, splits, m, n, k, kchunk, A, lda, B, ldb, C, ldc, alpha_re,    \
+            case 3: launch<R, S, X, Y, 4, 2, 8, 2>(ps, s, splits, m, n, k, kchunk, A, lda, B, ldb, C, ldc, alpha_re,    \
                                                alpha_im, beta_re, beta_im, part); break;                    \
-            case 4: launch<R, X, Y, 4, 1, 8, 2>(ps, s, splits, m, n, k, kchunk, A, lda, B, ldb, C, ldc, alpha_re,    \
+            case 4: launch<R, S, X, Y, 4, 1, 8, 2>(ps, s, splits, m, n, k, kchunk, A, lda, B, ldb, C, ldc, alpha_re,    \
                                                alpha_im, beta_re, beta_im, part); break;                    \
-            case 5: launch<R, X, Y, 2, 4, 16, 0>(ps, s, splits, m, n, k, kchunk, A, lda, B, ldb, C, ldc, alpha_re,   \
+            case 5: launch<R, S, X, Y, 2, 4, 16, 0>(ps, s, splits, m, n, k, kchunk, A, lda, B, ldb, C, ldc, alpha_re,   \
                                                 alpha_im, beta_re, beta_im, part); break;                   \
-            default: launch<R, X, Y, 2, 2, 8, 0>(ps, s, splits, m, n, k, kchunk, A, lda, B, ldb, C, ldc, alpha_re,   \
+            default: launch<R, S, X, Y, 2, 2, 8, 0>(ps, s, splits, m, n, k, kchunk, A, lda, B, ldb, C, ldc, alpha_re,   \
                                                 alpha_im, beta_re, beta_im, part); break;                   \
         }                                                                                                   \
     }
@@ -421,17 +467,34 @@ void gemm(hipStream_t s, GemmWorkspace& ws, int num_cu, char ta, char tb, int m,
                      2.0 * m * n * (double)splits, true);
         const size_t total = (size_t)m * n;
         const int blocks = (int)std::min<size_t>((total + 255) / 256, 65536);
-        APW_PLAUNCH(pr, splitk_reduce<R>, dim3(blocks), dim3(256), 0, s, m, n, splits,
+        APW_PLAUNCH(pr, splitk_reduce<S>, dim3(blocks), dim3(256), 0, s, m, n, splits,
                            reinterpret_cast<const c_t*>(part), reinterpret_cast<c_t*>(C), ldc, alpha_re,
                            alpha_im, beta_re, beta_im);
         APW_LAUNCH_CHECK();
     }
 }
 
+}  // namespace
+
 void gemm_reserve(GemmWorkspace& ws, int num_cu) {
     // splits x tiles <= resident slots, so splits m n <= slots x BM x BN; the largest product
     // of the split variants is 3 x 128 x 64 (c128: two doubles an element)
     ws.part.reserve((size_t)3 * std::max(num_cu, 64) * 128 * 64 * 2 + 2);
+}
+
+template <class R>
+void gemm(hipStream_t s, GemmWorkspace& ws, int num_cu, char ta, char tb, int m, int n, int k,
+          double alpha_re, double alpha_im, const R* A, int lda, const R* B, int ldb,
+          double beta_re, double beta_im, R* C, int ldc, Prof* prof) {
+    gemm_impl<R, R>(s, ws, num_cu, ta, tb, m, n, k, alpha_re, alpha_im, A, lda, B, ldb, beta_re, beta_im, C, ldc,
+                    prof);
+}
+
+void gemm_x32(hipStream_t s, GemmWorkspace& ws, int num_cu, char ta, char tb, int m, int n, int k,
+              double alpha_re, double alpha_im, const double* A, int lda, const double* B, int ldb,
+              double beta_re, double beta_im, double* C, int ldc, Prof* prof) {
+    gemm_impl<float, double>(s, ws, num_cu, ta, tb, m, n, k, alpha_re, alpha_im, A, lda, B, ldb, beta_re, beta_im,
+                             C, ldc, prof);
 }
 
 template void gemm<double>(hipStream_t, GemmWorkspace&, int, char, char, int, int, int, double, double,

@@ -312,6 +312,13 @@ int apw_zgemm(apw_ctx* ctx, char transa, char transb, int m, int n, int k,
 int apw_cgemm(apw_ctx* ctx, char transa, char transb, int m, int n, int k,
               const float* alpha, const float* a, int lda, const float* b, int ldb,
               const float* beta, float* c, int ldc);
+/* No counterpart in the reference: apw_zgemm's c128 product computed on the FP32 matrix
+ * cores (operands rounded to FP32, FP32 partial sums over 32-64 k folded into FP64 ones,
+ * C in c128), for callers that accept an error of ~1e-7 relative to sum |a||b| for twice
+ * the FP64 rate.  The FP32-mixed Davidson pre-phase uses it for its basis products. */
+int apw_zgemm_x32(apw_ctx* ctx, char transa, char transb, int m, int n, int k,
+                  const double* alpha, const double* a, int lda, const double* b, int ldb,
+                  const double* beta, double* c, int ldc);
 /* dnevx_op<complex<double>, DEVICE_GPU> (dngvd_op.h:85-107): lowest nbands eigenpairs
  * of the Hermitian nstart x nstart matrix hcc (lower triangle, ld ldh).  eigenvalue:
  * device double[nbands]; vcc: device c128, ld ldh, eigenvector j in column j. */

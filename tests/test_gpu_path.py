@@ -240,6 +240,36 @@ def test_cgemm_vs_numpy(torch, ctx, ta, tb, m, n, k):
     assert (np.abs(got - ref) <= bound).all(), np.abs(got - ref).max()
 
 
+@pytest.mark.parametrize("ta,tb,m,n,k", [("N", "N", 70, 33, 19), ("C", "N", 512, 160, 51627),
+                                         ("C", "N", 160, 640, 51627), ("N", "N", 51627, 160, 640),
+                                         ("N", "N", 51627, 64, 512), ("T", "C", 17, 65, 129)])
+def test_zgemm_x32_vs_numpy(torch, ctx, ta, tb, m, n, k):
+    """c128 product on the FP32 matrix cores (apw_zgemm_x32): within 4 eps32 sum|a||b| of
+    the exact product, the bound of the FP32 operand rounding alone (a plain FP32 sum over
+    k would add ~sqrt(k) eps32 |partial sums|), and not the FP64 result (the FP32 path
+    ran).  Shapes: the split inner products and tall updates of the 64-atom solve."""
+    from apw import pw
+    rng = np.random.default_rng(m * 5 + n * 11 + k)
+
+    def rnd(r, c):
+        return rng.standard_normal((r, c)) + 1j * rng.standard_normal((r, c))
+
+    A = rnd(m, k) if ta == "N" else rnd(k, m)
+    Bm = rnd(k, n) if tb == "N" else rnd(n, k)
+    Cm = rnd(m, n)
+    op = {"N": lambda x: x, "T": lambda x: x.T, "C": lambda x: x.conj().T}
+    alpha, beta = 0.7 - 0.2j, 0.3 + 0.1j
+    oA, oB = op[ta](A), op[tb](Bm)
+    ref = alpha * oA @ oB + beta * Cm
+    dA, dB, dC = dev(torch, A.T.copy()), dev(torch, Bm.T.copy()), dev(torch, Cm.T.copy())
+    pw.gemm_op(ctx, ta, tb, m, n, k, alpha, dA, A.shape[0], dB, Bm.shape[0], beta, dC, m, x32=True)
+    got = host(ctx, dC).T
+    mag = abs(alpha) * (np.abs(oA) @ np.abs(oB))
+    err = np.abs(got - ref)
+    assert (err <= 4 * 6e-8 * mag + 1e-13 * (1 + np.abs(ref))).all(), (err / (mag + 1)).max()
+    assert err.max() > 1e-12 * np.abs(ref).max()
+
+
 # ------------------------------------------------------------------ density (psiToRho)
 def test_elecstate_golden(torch, ctx):
     """elecstate_op_test.cpp vectors through the device functors (nspin 1 and the
