@@ -15,11 +15,10 @@
 // One host sync per iteration remains: the nband eigenvalues come back for the
 // convergence test |e_new - e_old| < thr (diago_david.cpp:270-280).
 //
-// FP32-mixed form: a first diag_once runs with the H|psi> chain in FP32 and its
-// dim-long basis products on the FP32 matrix cores (FP64 data, FP64 sums every 32-64 k;
-// eigensolver, Cholesky and the small products in FP64) until the residual norms of the
-// corrected bands drop below 1e-5 or no band is left unconverged; its Ritz vectors,
-// formed in FP64, then restart the plain FP64 iteration, which alone decides convergence.
+// FP32-mixed form: a first diag_once runs with the H|psi> chain in FP32 (subspace work in
+// FP64) until the residual norms of the corrected bands drop below 1e-5 or no band is
+// left unconverged; its Ritz vectors then restart the plain FP64 iteration, which alone
+// decides convergence.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -219,19 +218,6 @@ struct David {
         apw::gemm<double>(st(), gws(), ctx->num_cu, ta, tb, m, n, k, ar, 0.0, A, lda, Bm, ldb, br, 0.0, C, ldc, &ctx->prof);
     }
 
-    // The products with a dim-long side (Gram and projection matrices, residuals, the
-    // orthonormalisation and refresh updates) of the FP32 pre-phase run on the FP32 matrix
-    // cores with FP64 operands and results (gemm_x32); every product of the FP64 iteration,
-    // the small nc x nc ones and the vectors handed over at the switch stay FP64.
-    void gemm_big(char ta, char tb, int m, int n, int k, double ar, const double* A, int lda, const double* Bm,
-                  int ldb, double br, double* C, int ldc) {
-        if (prec == APW_FP32_MIXED)
-            apw::gemm_x32(st(), gws(), ctx->num_cu, ta, tb, m, n, k, ar, 0.0, A, lda, Bm, ldb, br, 0.0, C, ldc,
-                          &ctx->prof);
-        else
-            gemm(ta, tb, m, n, k, ar, A, lda, Bm, ldb, br, C, ldc);
-    }
-
     // HX = H X[:, i0..i1]  (hpsi_func semantics: X = head of basis, HX = exact destination)
     void apply_h(double* HX, int i0, int i1, int p) {
         const int nb = i1 - i0 + 1;
@@ -287,11 +273,11 @@ struct David {
         double* G = reinterpret_cast<double*>(reinterpret_cast<double2*>(W) + c0);
         double* X = vc.get();                                          // ld x nc: [-L R^-1; R^-1]
         double* Rinv = reinterpret_cast<double*>(reinterpret_cast<double2*>(X) + c0);
-        gemm_big('C', 'N', ld, nc, dim, 1.0, basis.get(), dim, SV, dim, 0.0, W, ld);
+        gemm('C', 'N', ld, nc, dim, 1.0, basis.get(), dim, SV, dim, 0.0, W, ld);
         if (c0 > 0) gemm('C', 'N', nc, nc, c0, -1.0, W, ld, W, ld, 1.0, G, ld);
         chol_inv_upper(st(), nc, G, ld, Rinv, ld, status.get(), 1.0e-12, &ctx->prof);
         if (c0 > 0) gemm('N', 'N', c0, nc, nc, -1.0, W, ld, Rinv, ld, 0.0, X, ld);
-        gemm_big('N', 'N', dim, nc, ld, 1.0, basis.get(), dim, X, ld, 0.0, tmp.get(), dim);
+        gemm('N', 'N', dim, nc, ld, 1.0, basis.get(), dim, X, ld, 0.0, tmp.get(), dim);
         hipLaunchKernelGGL(k_copy_cols, dim3(gsz((size_t)dim * nc)), dim3(256), 0, st(), dim, nc,
                            reinterpret_cast<const double2*>(tmp.get()), dim, reinterpret_cast<double2*>(V), dim);
         APW_LAUNCH_CHECK();
@@ -331,7 +317,7 @@ struct David {
 
     // hcc[nbase .. nbase+nc, 0 .. nbase+nc) = basis_new^H hpsi   (cal_elem, diago_david.cpp:626-717)
     void cal_elem(int nbase, int nc) {
-        gemm_big('C', 'N', nc, nbase + nc, dim, 1.0, Bd(basis.get(), nbase), dim, hp.get(), dim, 0.0,
+        gemm('C', 'N', nc, nbase + nc, dim, 1.0, Bd(basis.get(), nbase), dim, hp.get(), dim, 0.0,
              reinterpret_cast<double*>(reinterpret_cast<double2*>(hcc.get()) + nbase), nbx);
     }
 
@@ -365,10 +351,10 @@ struct David {
             hipLaunchKernelGGL(k_gather_vc, dim3(gsz((size_t)nbase * notconv)), dim3(256), 0, st(), nbase, notconv,
                                reinterpret_cast<const double2*>(vcc.get()), nbx, unconv_d.get(), vcp);
             double* NB = Bd(basis.get(), nbase);
-            gemm_big('N', 'N', dim, notconv, nbase, 1.0, hp.get(), dim, vc.get(), nbase, 0.0, NB, dim);
+            gemm('N', 'N', dim, notconv, nbase, 1.0, hp.get(), dim, vc.get(), nbase, 0.0, NB, dim);
             hipLaunchKernelGGL(k_scale_vc, dim3(gsz((size_t)nbase * notconv)), dim3(256), 0, st(), nbase, notconv, vcp,
                                eig.get(), unconv_d.get());
-            gemm_big('N', 'N', dim, notconv, nbase, 1.0, S(), dim, vc.get(), nbase, 1.0, NB, dim);
+            gemm('N', 'N', dim, notconv, nbase, 1.0, S(), dim, vc.get(), nbase, 1.0, NB, dim);
             if (switch_thr > 0) {
                 hipLaunchKernelGGL(k_colres2, dim3(notconv), dim3(CR_NT), 0, st(), dim,
                                    reinterpret_cast<const double2*>(NB), dim, (const double2*)nullptr, 0,
@@ -397,18 +383,14 @@ struct David {
             }
             const bool leave_mixed = switch_thr > 0 && (rmax < switch_thr || notconv == 0);
             if (!notconv || nbase + notconv > nbx || it == maxiter || leave_mixed) {
-                const bool leaving = !notconv || it == maxiter || leave_mixed;
                 // psi = basis vcc (:292-306)
-                if (leaving)
-                    gemm('N', 'N', dim, nband, nbase, 1.0, basis.get(), dim, vcc.get(), nbx, 0.0, psi, ldpsi);
-                else
-                    gemm_big('N', 'N', dim, nband, nbase, 1.0, basis.get(), dim, vcc.get(), nbx, 0.0, psi, ldpsi);
+                gemm('N', 'N', dim, nband, nbase, 1.0, basis.get(), dim, vcc.get(), nbx, 0.0, psi, ldpsi);
                 if (ldpsi > dim) {
                     hipLaunchKernelGGL(k_zero_rows, dim3(gsz((size_t)(ldpsi - dim) * nband)), dim3(256), 0, st(), dim,
                                        ldpsi, nband, reinterpret_cast<double2*>(psi), ldpsi);
                     APW_LAUNCH_CHECK();
                 }
-                if (leaving) {
+                if (!notconv || it == maxiter || leave_mixed) {
                     // residual norms ||H psi_m - e_m psi_m|| of the returned vectors, from the
                     // subspace (hpsi vcc - e psi): one GEMM, no extra H application
                     gemm('N', 'N', dim, nband, nbase, 1.0, hp.get(), dim, vcc.get(), nbx, 0.0, tmp.get(), dim);
@@ -424,7 +406,7 @@ struct David {
                     break;
                 }
                 // refresh (:797-934)
-                gemm_big('N', 'N', dim, nband, nbase, 1.0, hp.get(), dim, vcc.get(), nbx, 0.0, tmp.get(), dim);
+                gemm('N', 'N', dim, nband, nbase, 1.0, hp.get(), dim, vcc.get(), nbx, 0.0, tmp.get(), dim);
                 hipLaunchKernelGGL(k_copy_cols, dim3(gsz((size_t)dim * nband)), dim3(256), 0, st(), dim, nband,
                                    reinterpret_cast<const double2*>(tmp.get()), dim,
                                    reinterpret_cast<double2*>(hp.get()), dim);

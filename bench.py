@@ -56,8 +56,12 @@ def run_solve(pw, ctx, ham, s, psi0_dev, precision):
 _PMC_FAMILY = {   # regular expressions over the kernel names of the committed traffic summary
     "eig_tridiag": r"apw::k_tridiag_(reg|lds)",
     # GEMMs: the path's updates are op(A) = N (tiled), its inner products op(A) = C (split-K)
-    "zgemm_f64_tiled": r"apw::zgemm_kernel<double, 0, ", "zgemm_f64_splitk": r"apw::zgemm_kernel<double, [12], ",
-    "cgemm_f32_tiled": r"apw::zgemm_kernel<float, 0, ", "cgemm_f32_splitk": r"apw::zgemm_kernel<float, [12], ",
+    # (template arguments: MFMA type, storage type, op(A), ...)
+    "zgemm_f64_tiled": r"apw::zgemm_kernel<double, double, 0, ",
+    "zgemm_f64_splitk": r"apw::zgemm_kernel<double, double, [12], ",
+    "cgemm_f32_tiled": r"apw::zgemm_kernel<float, float, 0, ", "cgemm_f32_splitk": r"apw::zgemm_kernel<float, float, [12], ",
+    "zgemm_x32_tiled": r"apw::zgemm_kernel<float, double, 0, ",
+    "zgemm_x32_splitk": r"apw::zgemm_kernel<float, double, [12], ",
     "fft_x_veff_fused": r"apw::k_xreg<\w+, \d+, 0>|apw::k_xpass<\w+, 0>",
     "fft_y_bwd": r"apw::k_yreg<\w+, \d+, 1>|apw::k_ybwd", "fft_y_fwd": r"apw::k_yreg<\w+, \d+, -1>|apw::k_yfwd",
     "fft_z_bwd_scatter": r"apw::k_zreg<\w+, \d+, 1>|apw::k_zbwd",
@@ -94,7 +98,8 @@ def _roofline(name, e):
     if name.startswith(("zgemm", "cgemm")):
         bound, unit = "mfma", "TFLOP/s"
         achieved = e["flops"] / max(1, e["launches"]) / avg_s / 1e12
-        peak = PEAK_TFLOPS["f64" if name.startswith("zgemm") else "f32"]
+        # zgemm_x32: c128 data on the FP32 matrix cores, priced against the FP32 peak
+        peak = PEAK_TFLOPS["f64" if name.startswith("zgemm_f64") else "f32"]
     else:
         bound, unit = "hbm", "GB/s"
         achieved = e["bytes"] / max(1, e["launches"]) / avg_s / 1e9

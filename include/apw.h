@@ -315,7 +315,8 @@ int apw_cgemm(apw_ctx* ctx, char transa, char transb, int m, int n, int k,
 /* No counterpart in the reference: apw_zgemm's c128 product computed on the FP32 matrix
  * cores (operands rounded to FP32, FP32 partial sums over 32-64 k folded into FP64 ones,
  * C in c128), for callers that accept an error of ~1e-7 relative to sum |a||b| for twice
- * the FP64 rate.  The FP32-mixed Davidson pre-phase uses it for its basis products. */
+ * ~1.5x the FP64 rate on the Davidson shapes (the c128 operands keep the FP64 loads).
+ * The Davidson solvers do not use it (measured: see the FP32-mixed notes). */
 int apw_zgemm_x32(apw_ctx* ctx, char transa, char transb, int m, int n, int k,
                   const double* alpha, const double* a, int lda, const double* b, int ldb,
                   const double* beta, double* c, int ldc);

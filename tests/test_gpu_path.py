@@ -613,3 +613,32 @@ def test_david_batched_uneven_cells(torch):
         u[1].close()
     for c in ctxs + [c0]:
         c.close()
+
+
+def test_david_mixed_batched_uneven_cells_all_converge(torch):
+    """The first 32 cells of the uneven batch (437 to 15203 plane waves, 20^3 to 64^3 grids)
+    in the FP32-mixed form, batched on two contexts: every cell converges (no
+    orthonormalisation breakdown in the FP32 pre-phase), and its eigenvalues are as close to
+    the converged ones (an FP64 solve at thr 1e-12) as the FP64 solve at thr 1e-9 is, within
+    1e-9 Ry.  (The |de| < thr stop bounds the last change, not the error: on some cells the
+    FP64 solve itself stops a few 1e-9 Ry from the converged values.)"""
+    from apw import pw
+    systems = [configs.make_cell(i) for i in range(32)]
+    res = {}
+    for prec, nc, thr in (("fp64", 1, 1e-9), ("fp32_mixed", 2, 1e-9), ("tight", 2, 1e-12)):
+        ctxs = [pw.Context(0) for _ in range(nc)]
+        units = [(pw.DiagoDavid(ctxs[i % nc], s.precondition, s.nband, s.npw, s.david_ndim),
+                  pw.HamiltPW(ctxs[i % nc], s), dev(torch, s.psi0), s.npwx, np.zeros(s.nband))
+                 for i, s in enumerate(systems)]
+        pw.david_diag_batched(units, thr, 500, precision="fp64" if prec == "tight" else prec)
+        res[prec] = [u[4].copy() for u in units]
+        for u in units:
+            u[0].close()
+            u[1].close()
+        for c in ctxs:
+            c.close()
+    e64 = np.array([np.abs(a - t).max() for a, t in zip(res["fp64"], res["tight"])])
+    emx = np.array([np.abs(a - t).max() for a, t in zip(res["fp32_mixed"], res["tight"])])
+    print("FP64 error per cell", e64)
+    print("mixed error per cell", emx)
+    assert (emx <= e64 + 1e-9).all()
