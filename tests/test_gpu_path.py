@@ -287,7 +287,7 @@ def test_elecstate_golden(torch, ctx):
     assert np.abs(got - np.array(g["expected_rho_2"])).max() < 6e-9
 
 
-@pytest.mark.parametrize("name,nb", [("si8", 24), ("kpts16:37", 10)])
+@pytest.mark.parametrize("name,nb", [("si8", 24), ("si8", 7), ("kpts16:37", 10)])
 def test_psi_to_rho_vs_oracle(torch, ctx, name, nb):
     """Fused band-density pass vs the oracle's rhoBandK (random bands and weights, one
     zero weight), then accumulation on top of an existing density."""
