@@ -257,8 +257,10 @@ struct David {
     void orthonormalize(int c0, int nc) {
         hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, st(), (size_t)2, reinterpret_cast<double*>(status.get()));
         APW_LAUNCH_CHECK();
+        // equal blocks rounded up to whole 32-column GEMM tiles (two blocks of 80 left 16
+        // columns of every third tile idle in the block's products; 96 + 64 fill them)
         const int nblk = (nc + CHOL_MAX - 1) / CHOL_MAX;
-        const int bs = (nc + nblk - 1) / nblk;
+        const int bs = std::min(CHOL_MAX, ((nc + nblk - 1) / nblk + 31) / 32 * 32);
         for (int b0 = 0; b0 < nc; b0 += bs) orthonormalize_block(c0 + b0, std::min(bs, nc - b0));
     }
 
