@@ -359,7 +359,6 @@ __global__ __launch_bounds__(NTR) void k_tridiag_reg(const double2* __restrict__
     extern __shared__ __align__(16) unsigned char smem[];
     double2* Al = reinterpret_cast<double2*>(smem);   // Al[lc * n + i] = A[i][c], c = blockIdx.x + lc * nwg
     double2* gs = Al + (size_t)ncol * n;                // gathered p and column, two buffers of 2n
-    __shared__ int stop;
     const int nwg = (int)gridDim.x, bid = (int)blockIdx.x;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     constexpr int NW = NTR / 64;
@@ -477,7 +476,9 @@ __global__ __launch_bounds__(NTR) void k_tridiag_reg(const double2* __restrict__
         double2* g = gs + (size_t)(k & 1) * 2 * n;
         // ---- gather p_k and column k+1 ----
         // (a thread's q-th value, once valid, is not polled again: bit q of got; at most
-        // ceil(2n / NTR) <= 8 values a thread for the sizes this form takes)
+        // ceil(2n / NTR) <= 8 values a thread for the sizes this form takes).  Each wave
+        // polls on its own until its values are in (no workgroup barrier per poll round);
+        // one barrier then publishes the gathered values to the whole workgroup.
         {
             unsigned spins = 0, got = 0;
             while (true) {
@@ -494,23 +495,24 @@ __global__ __launch_bounds__(NTR) void k_tridiag_reg(const double2* __restrict__
                         mine = false;
                     }
                 }
-                if (__syncthreads_and(mine)) break;
-                // the abort word and the spin bound every 32 failed polls: its load is a
-                // memory round trip that would otherwise lengthen every wait
+                if (__all(mine)) break;
+                // the abort word and the spin bound every 32 failed polls of this wave: its
+                // load is a memory round trip that would otherwise lengthen every wait.  Every
+                // wave leaves: its values arrive, it sees the abort word, or it raises it.
                 if ((++spins & 31u) == 0) {
-                    if (threadIdx.x == 0) {
-                        int st = __hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u;
+                    int st = 0;
+                    if (lane == 0) {
+                        st = __hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u;
                         if (!st && spins > SPIN_LIMIT) {
                             atomicExch(abort_flag, 1u);
                             st = 1;
                         }
-                        stop = st;
                     }
-                    __syncthreads();
-                    if (stop) break;
+                    if (__shfl(st, 0, 64)) break;
                 }
                 __builtin_amdgcn_s_sleep(1);
             }
+            __syncthreads();
         }
         // ---- y = tau p - (|tau|^2 mu / 2) v ----
         double mr = 0;
