@@ -200,7 +200,6 @@ __global__ __launch_bounds__(NT) void k_tridiag_lds(double2* __restrict__ A, int
     double2* y = v + n;
     double2* cn = y + n;
     double* red = reinterpret_cast<double*>(cn + n + ncol);
-    __shared__ int stop;
     const unsigned nwg = gridDim.x;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     auto colp = [&](int lc) -> double2* {
@@ -248,34 +247,44 @@ __global__ __launch_bounds__(NT) void k_tridiag_lds(double2* __restrict__ A, int
             for (int i = threadIdx.x; i < m; i += NT) ring_put(slot + n + i, col[i]);
         }
         // ---- gather p -> y and the published column -> cn ----
+        // Each wave polls on its own until its values are in, skipping those it already has
+        // (bit q of got for a thread's q-th value; 2m / NT <= 16 for n <= 2048); one barrier
+        // then publishes them to the workgroup.
         {
-            unsigned spins = 0;
+            const __amdgpu_buffer_rsrc_t rs = ring_rsrc(slot, 2 * n);
+            unsigned spins = 0, got = 0;
             while (true) {
                 bool mine = true;
-                for (int i = threadIdx.x; i < 2 * m; i += NT) {
+                int q = 0;
+                for (int i = threadIdx.x; i < 2 * m; i += NT, ++q) {
+                    if ((got >> q) & 1u) continue;
                     const bool isp = i < m;
                     const int ii = isp ? i : i - m;
                     double2 val;
-                    if (ring_get(slot + (isp ? ii : n + ii), val)) (isp ? y : cn)[ii] = val;
-                    else mine = false;
+                    if (ring_get16(rs, isp ? ii : n + ii, val)) {
+                        (isp ? y : cn)[ii] = val;
+                        got |= 1u << q;
+                    } else {
+                        mine = false;
+                    }
                 }
-                if (__syncthreads_and(mine)) break;
-                // the abort word and the spin bound every 32 failed polls: its load is a
-                // memory round trip that would otherwise lengthen every wait
+                if (__all(mine)) break;
+                // the abort word and the spin bound every 32 failed polls of this wave: its
+                // load is a memory round trip that would otherwise lengthen every wait
                 if ((++spins & 31u) == 0) {
-                    if (threadIdx.x == 0) {
-                        int st = __hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u;
+                    int st = 0;
+                    if ((threadIdx.x & 63) == 0) {
+                        st = __hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u;
                         if (!st && spins > SPIN_LIMIT) {
                             atomicExch(abort_flag, 1u);
                             st = 1;
                         }
-                        stop = st;
                     }
-                    __syncthreads();
-                    if (stop) break;
+                    if (__shfl(st, 0, 64)) break;
                 }
                 __builtin_amdgcn_s_sleep(1);
             }
+            __syncthreads();
         }
         // ---- y = tau p - (|tau|^2 mu / 2) v,  mu = v^H p;  cn -= rank-2 correction ----
         double mr = 0;

@@ -311,10 +311,11 @@ def test_psi_to_rho_vs_oracle(torch, ctx, name, nb):
 
 # ------------------------------------------------------------------ eigensolver
 @pytest.mark.parametrize("n,nband,ld", [(1, 1, 1), (2, 1, 3), (48, 24, 48), (200, 60, 256), (640, 160, 640),
-                                        (1024, 64, 1024), (1300, 40, 1300)])
+                                        (1024, 64, 1024), (1100, 50, 1100), (1300, 40, 1300)])
 def test_dnevx_vs_lapack(torch, ctx, n, nband, ld):
-    """Eigenpairs vs LAPACK.  n = 1024 uses 171 workgroups of the LDS-resident reduction;
-    n = 1300 does not fit it and takes the global-memory form."""
+    """Eigenpairs vs LAPACK.  n = 1024 uses 171 workgroups of the register form of the
+    reduction; n = 1100 the LDS-resident form with columns in LDS; n = 1300 does not fit
+    it and keeps its columns in global memory."""
     from apw import pw
     rng = np.random.default_rng(n)
     H = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
