@@ -136,9 +136,8 @@ __device__ void make_reflector(const double2* xs, int mx, double2* vo, double2& 
 
 // Cross-workgroup exchange of the LDS-resident reduction: a ring with one slot per
 // column step, pre-filled with a NaN bit pattern no arithmetic produces.  Producers store
-// each value with device-scope atomic stores (they bypass the non-coherent caches);
-// consumers poll the slots they need with device-scope atomic loads until no word holds
-// the pattern.  Every value validates itself, so there is no flag, no fence and no grid
+// each value with device-scope stores (they bypass the non-coherent caches); consumers
+// poll the slots they need with device-scope loads until no word holds the pattern.  Every value validates itself, so there is no flag, no fence and no grid
 // barrier: one memory round trip per column.
 constexpr unsigned long long RING_EMPTY = 0x7ff4dead5eed0001ull;
 
@@ -151,13 +150,6 @@ __global__ void k_fill_ring(double2* ring, size_t count) {
 __device__ inline void ring_put(double2* slot, double2 v) {
     __hip_atomic_store(&slot->x, v.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __hip_atomic_store(&slot->y, v.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-__device__ inline bool ring_get(const double2* slot, double2& v) {
-    v.x = __hip_atomic_load(&slot->x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    v.y = __hip_atomic_load(&slot->y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    return (unsigned long long)__double_as_longlong(v.x) != RING_EMPTY &&
-           (unsigned long long)__double_as_longlong(v.y) != RING_EMPTY;
 }
 
 // 16-byte forms: one vector buffer load / store per value with the device-scope cache
