@@ -137,8 +137,9 @@ __device__ void make_reflector(const double2* xs, int mx, double2* vo, double2& 
 // Cross-workgroup exchange of the LDS-resident reduction: a ring with one slot per
 // column step, pre-filled with a NaN bit pattern no arithmetic produces.  Producers store
 // each value with device-scope stores (they bypass the non-coherent caches); consumers
-// poll the slots they need with device-scope loads until no word holds the pattern.  Every value validates itself, so there is no flag, no fence and no grid
-// barrier: one memory round trip per column.
+// poll the slots they need with device-scope loads until no word holds the pattern.
+// Every value validates itself, so there is no flag, no fence and no grid barrier: one
+// memory round trip per column.
 constexpr unsigned long long RING_EMPTY = 0x7ff4dead5eed0001ull;
 
 __global__ void k_fill_ring(double2* ring, size_t count) {
