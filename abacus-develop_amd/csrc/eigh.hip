@@ -1214,6 +1214,12 @@ void heevx_lowest(hipStream_t s, EigWorkspace& ws, int num_cu, int n, int ldh, c
                 if (fixed_r + (size_t)nc * n * sizeof(double2) <= TRIDIAG_LDS_BYTES) break;
             }
             APW_CHECK(w <= std::min(num_cu, 256), "heevx: no workgroup count fits n=%d", n);
+            // about four columns a workgroup (at most one a wave in each step's update): with
+            // the per-wave polling of the ring, spreading the columns beats fewer exchange
+            // participants up to ~160 of them (measured n = 160..640: 5-17 % faster than the
+            // LDS minimum; 192 and 256 workgroups lost again at n = 640)
+            w = std::max(w, std::min(std::min(num_cu, 160), n / 4));
+            nc = (n + w - 1) / w;
             // tuning override: more workgroups than the LDS minimum (fewer columns each)
             if (const char* ev = std::getenv("APW_TRIDIAG_NWG")) {
                 const int want = std::atoi(ev);
