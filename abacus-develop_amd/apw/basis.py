@@ -96,6 +96,7 @@ class PWBasisK:
     ny: int = 0
     nz: int = 0
     kvec_d: np.ndarray = field(default_factory=lambda: np.zeros((1, 3)))
+    gridecut: float | None = None        # Ry; None: 4 * ecutwfc (esolver_fp.cpp:70)
 
     def __post_init__(self):
         lat = self.lat
@@ -103,8 +104,9 @@ class PWBasisK:
         self.nks = self.kvec_d.shape[0]
         if self.nx <= 0:
             # the wavefunction grid is the density grid: gridecut = 4 * ecutwfc (esolver_fp.cpp:70)
-            self.nx, self.ny, self.nz = grid_from_ecut(lat, 4.0 * self.ecutwfc)
-            self.gridecut_lat = 4.0 * self.ecutwfc / lat.tpiba2
+            gridecut = 4.0 * self.ecutwfc if self.gridecut is None else float(self.gridecut)
+            self.nx, self.ny, self.nz = grid_from_ecut(lat, gridecut)
+            self.gridecut_lat = gridecut / lat.tpiba2
         else:
             self.gridecut_lat = gridecut_of_box(lat, self.nx, self.ny, self.nz)
         nx, ny, nz = self.nx, self.ny, self.nz

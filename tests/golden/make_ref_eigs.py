@@ -1,0 +1,80 @@
+"""Write the reference's own converged DiagoDavid eigenvalues as a JSON fixture
+(tests/golden/ref_eigs.json), computed by the compiled reference (oracle/_ref):
+
+* ``plain``   -- libref_david.so: ABACUS's DiagoDavid<complex<double>, DEVICE_CPU>
+  (source/module_hsolver/diago_david.cpp:151-1240) with its CPU H|psi> functors, linked to
+  the image's LAPACK exactly as the sources are;
+* ``detsign`` -- libref_david_detsign.so: the same objects with zheevx's all-eigenpair
+  output re-phased to LAPACK's inverse-iteration convention (oracle/ref_lapack_signs.cpp),
+  the convention the GPU solver follows, so its trajectory is the GPU's.
+
+Each case runs the workload's own seeded start vectors at david_diag_thr 1e-9 Ry,
+ntry_max 5, notconv_max 0 (HSolverPW's defaults, hsolver_pw.cpp:472-545) and records the
+eigenvalues, the iteration count and the H|psi> call/band counts.  The cases: the 8-atom
+cell, k-points 0 and 37 of the 16-atom 4x4x4 mesh, cells 1, 3 and 6 of the uneven batch,
+and the 64-atom cell (about 5 min on 8 threads).  ``--si216`` adds the 216-atom cell
+(hours, ~30 GB); ``--only NAME`` recomputes one case and keeps the others.
+
+    OMP_NUM_THREADS=8 python tests/golden/make_ref_eigs.py [--si216] [--only si64]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path[:0] = [ROOT, os.path.join(ROOT, "abacus-develop_amd")]
+
+from apw import configs  # noqa: E402
+from oracle import ref_david  # noqa: E402
+
+OUT = os.path.join(HERE, "ref_eigs.json")
+CASES = [("si8", 0), ("kpts16", 0), ("kpts16", 37), ("cell1", 0), ("cell3", 0), ("cell6", 0), ("si64", 0)]
+
+
+def case_key(name, ik):
+    return f"{name}:{ik}"
+
+
+def solve(name, ik, out, save):
+    s = configs.make(name, ik=ik) if name == "kpts16" else configs.make(name)
+    out.update({"npw": s.npw, "npwx": s.npwx, "grid": [s.nx, s.ny, s.nz], "nband": s.nband, "nkb": s.nkb,
+                "david_ndim": s.david_ndim, "thr": 1e-9})
+    for label, det in (("plain", False), ("detsign", True)):
+        t0 = time.perf_counter()
+        eig, _, it, calls, bands = ref_david.david(s, thr=1e-9, maxiter=500, ntry_max=5, notconv_max=0, detsign=det)
+        dt = time.perf_counter() - t0
+        out[label] = {"iterations": int(it), "hpsi_calls": calls, "hpsi_bands": bands, "seconds": round(dt, 2),
+                      "eig": [float(x) for x in eig]}
+        print(f"{name}:{ik} {label}: {it} iterations, {bands} bands through H|psi>, {dt:.1f} s", flush=True)
+        save()
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--si216", action="store_true")
+    ap.add_argument("--only", default=None)
+    args = ap.parse_args()
+    cases = CASES + ([("si216", 0)] if args.si216 else [])
+    if args.only:
+        cases = [c for c in cases if c[0] == args.only] or [(args.only, 0)]
+    out = {}
+    if os.path.exists(OUT):
+        with open(OUT) as f:
+            out = json.load(f)
+    out["_threads"] = os.environ.get("OMP_NUM_THREADS", "default")
+    def save():
+        with open(OUT, "w") as f:
+            json.dump(out, f, indent=1)
+
+    for n, k in cases:
+        out[case_key(n, k)] = {}
+        solve(n, k, out[case_key(n, k)], save)
+
+
+if __name__ == "__main__":
+    main()
