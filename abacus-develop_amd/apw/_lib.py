@@ -44,6 +44,7 @@ class DiagStats(C.Structure):
 
 HPSI_CB = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int)
 SPSI_CB = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int)
+HPSI_CB_C64 = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int)
 
 _vp, _i, _d, _ch = C.c_void_p, C.c_int, C.c_double, C.c_char
 PROTOTYPES = {
@@ -69,6 +70,8 @@ PROTOTYPES = {
     "apw_david_destroy": (_i, [_vp]),
     "apw_david_diag": (_i, [_vp, _vp, HPSI_CB, SPSI_CB, _vp, _i, _vp, C.POINTER(_d), _d, _i, _i, _i, _i,
                             C.POINTER(DiagStats)]),
+    "apw_david_diag_c64": (_i, [_vp, HPSI_CB_C64, _vp, _i, _vp, C.POINTER(_d), _d, _i, _i, _i,
+                                C.POINTER(DiagStats)]),
     "apw_david_diag_batched": (_i, [_i, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_i), C.POINTER(_vp),
                                     C.POINTER(_vp), _d, _i, _i, _i, _i, C.POINTER(DiagStats), C.POINTER(_i)]),
     "apw_ekinetic_pw_op": (_i, [_vp, _i, _i, _i, _d, _vp, _vp, _vp]),

@@ -55,6 +55,18 @@ def build(jobs: int = 8) -> str:
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stdout + r.stderr)
     os.replace(tmp, LIB)
+    # the C++ adapter's test driver (tests/cpp/adapter_run.cpp): host code over libapw.so,
+    # built by the host compiler against the HIP runtime (no device code object of its own)
+    runner = os.path.join(LIBDIR, "apw_adapter_run")
+    rocm = os.path.dirname(os.path.dirname(os.path.realpath(HIPCC)))
+    cmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-D__HIP_PLATFORM_AMD__",
+           "-I", os.path.join(ROOT, "include"), "-I", os.path.join(rocm, "include"),
+           os.path.join(ROOT, "tests", "cpp", "adapter_run.cpp"), "-L", LIBDIR, "-lapw",
+           "-L", os.path.join(rocm, "lib"), "-lamdhip64", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath," + os.path.join(rocm, "lib"),
+           "-o", runner]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("adapter runner build failed:\n" + r.stdout + r.stderr)
     return LIB
 
 

@@ -57,6 +57,27 @@ __global__ void k_zero(size_t n, double* __restrict__ x) {
     for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) x[t] = 0.0;
 }
 
+// c128 <-> c64 column blocks (precision=single callers: c64 psi and callbacks)
+__global__ void k_cols_narrow(int rows, int ncols, const double2* __restrict__ src, int lds, float2* __restrict__ dst,
+                              int ldd) {
+    const size_t total = (size_t)rows * ncols;
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        const int r = (int)(t % rows), c = (int)(t / rows);
+        const double2 v = src[(size_t)c * lds + r];
+        dst[(size_t)c * ldd + r] = make_float2((float)v.x, (float)v.y);
+    }
+}
+
+__global__ void k_cols_widen(int rows, int ncols, const float2* __restrict__ src, int lds, double2* __restrict__ dst,
+                             int ldd) {
+    const size_t total = (size_t)rows * ncols;
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        const int r = (int)(t % rows), c = (int)(t / rows);
+        const float2 v = src[(size_t)c * lds + r];
+        dst[(size_t)c * ldd + r] = make_double2((double)v.x, (double)v.y);
+    }
+}
+
 // vc[i + m*nbase] = vcc[i + unconv[m]*nbx]     (cal_grad, diago_david.cpp:408-415)
 __global__ void k_gather_vc(int nbase, int nc, const double2* __restrict__ vcc, int nbx, const int* __restrict__ unconv,
                             double2* __restrict__ vc) {
@@ -140,6 +161,9 @@ __global__ void k_refresh_diag(int nband, double2* __restrict__ hcc, double2* __
     }
 }
 
+// Stands in for the c128 callback while a c64 one is set (apply_h calls the c64 one).
+int c64_placeholder(void*, const double*, double*, int, int, int, int) { return -1; }
+
 }  // namespace
 
 struct David {
@@ -153,7 +177,10 @@ struct David {
     // per call
     Ham* ham = nullptr;
     apw_hpsi_cb hcb = nullptr;
+    apw_hpsi_cb_c64 hcb32 = nullptr;   // c64 callback: X narrowed into x32, HX widened from hx32
     apw_spsi_cb scb = nullptr;
+    DevBuf<float> x32, hx32;
+    DevBuf<double> psi64;
     void* user = nullptr;
     int prec = APW_FP64;
     int hpsi_calls = 0;
@@ -225,6 +252,22 @@ struct David {
         APW_HIP(hipEventRecord(e0, st()));
         if (ham) {
             ham->hpsi(Bd(basis.get(), i0), dim, HX, dim, nb, p);
+        } else if (hcb32) {
+            // the callback sees a c64 copy of the basis columns it reads (same offsets from
+            // the head, leading dimension dim) and writes a c64 block that is widened into HX
+            float2* xs = reinterpret_cast<float2*>(x32.get());
+            hipLaunchKernelGGL(k_cols_narrow, dim3(gsz((size_t)dim * nb)), dim3(256), 0, st(), dim, nb,
+                               reinterpret_cast<const double2*>(Bd(basis.get(), i0)), dim, xs + (size_t)i0 * dim, dim);
+            APW_LAUNCH_CHECK();
+            APW_HIP(hipStreamSynchronize(st()));
+            const int rc = hcb32(user, x32.get(), hx32.get(), nbx, dim, i0, i1);
+            if (rc != 0) {
+                set_error("DiagoDavid: hpsi callback returned %d", rc);
+                throw Error{APW_ECALLBACK};
+            }
+            hipLaunchKernelGGL(k_cols_widen, dim3(gsz((size_t)dim * nb)), dim3(256), 0, st(), dim, nb,
+                               reinterpret_cast<const float2*>(hx32.get()), dim, reinterpret_cast<double2*>(HX), dim);
+            APW_LAUNCH_CHECK();
         } else {
             APW_HIP(hipStreamSynchronize(st()));
             const int rc = hcb(user, basis.get(), HX, nbx, dim, i0, i1);
@@ -532,6 +575,42 @@ int apw_david_diag(apw_david* dav, apw_ham* ham, apw_hpsi_cb hpsi_cb, apw_spsi_c
             stats->t_total_s = std::chrono::duration<double>(t1 - t0).count();
         }
         return total;
+    });
+}
+
+int apw_david_diag_c64(apw_david* dav, apw_hpsi_cb_c64 hpsi_cb, void* user, int ldpsi, float* psi, double* eig,
+                       double david_diag_thr, int david_maxiter, int ntry_max, int notconv_max,
+                       apw_diag_stats* stats) {
+    return guarded([&] {
+        APW_CHECK(dav && psi && eig && hpsi_cb, "apw_david_diag_c64: null argument");
+        apw::David& D = dav->d;
+        APW_CHECK(ldpsi >= D.dim, "apw_david_diag_c64: ldpsi < dim");
+        apw::DeviceGuard g(D.ctx->device);
+        // c128 shadow of psi, and the callback's c64 operand / result blocks
+        D.psi64.reserve((size_t)2 * ldpsi * D.nband);
+        D.x32.reserve((size_t)2 * D.dim * D.nbx);
+        D.hx32.reserve((size_t)2 * D.dim * D.nband);
+        hipLaunchKernelGGL(apw::k_cols_widen, dim3(apw::gsz((size_t)ldpsi * D.nband)), dim3(256), 0, D.ctx->stream,
+                           ldpsi, D.nband, reinterpret_cast<const float2*>(psi), ldpsi,
+                           reinterpret_cast<double2*>(D.psi64.get()), ldpsi);
+        APW_LAUNCH_CHECK();
+        D.hcb32 = hpsi_cb;
+        int rc = APW_OK;
+        try {
+            rc = apw_david_diag(dav, nullptr, &apw::c64_placeholder, nullptr, user, ldpsi, D.psi64.get(), eig,
+                                david_diag_thr, david_maxiter, ntry_max, notconv_max, APW_FP64, stats);
+        } catch (...) {
+            D.hcb32 = nullptr;
+            throw;
+        }
+        D.hcb32 = nullptr;
+        if (rc < 0) return rc;
+        hipLaunchKernelGGL(apw::k_cols_narrow, dim3(apw::gsz((size_t)ldpsi * D.nband)), dim3(256), 0, D.ctx->stream,
+                           ldpsi, D.nband, reinterpret_cast<const double2*>(D.psi64.get()), ldpsi,
+                           reinterpret_cast<float2*>(psi), ldpsi);
+        APW_LAUNCH_CHECK();
+        APW_HIP(hipStreamSynchronize(D.ctx->stream));
+        return rc;
     });
 }
 

@@ -225,6 +225,18 @@ int apw_david_diag(apw_david* dav, apw_ham* ham, apw_hpsi_cb hpsi_cb, apw_spsi_c
                    int david_maxiter, int ntry_max, int notconv_max, int precision,
                    apw_diag_stats* stats);
 
+/* DiagoDavid<std::complex<float>, DEVICE_GPU>::diag (diago_david.cpp:1246; ABACUS's
+ * precision=single, esolver.cpp:137-141): psi device c64 (ldpsi x nband) in/out, H applied
+ * by a c64 callback (HPsiFunc semantics, X = head of a c64 copy of the basis, HX = exact c64
+ * destination, ld = dim), S = I.  The Davidson arithmetic is the FP64 form's on c128 copies
+ * of psi and the blocks; the callback's FP32 products bound the attainable accuracy.
+ * eig: HOST double[nband].  Returns the iteration count or a negative status. */
+typedef int (*apw_hpsi_cb_c64)(void* user, const float* X, float* HX, int nvec, int dim, int id_start,
+                               int id_end);
+int apw_david_diag_c64(apw_david* dav, apw_hpsi_cb_c64 hpsi_cb, void* user, int ldpsi, float* psi,
+                       double* eig, double david_diag_thr, int david_maxiter, int ntry_max,
+                       int notconv_max, apw_diag_stats* stats);
+
 /* Batched form: nunits independent DiagoDavid::diag calls (k-points or cells, uneven sizes),
  * each unit with its own solver, Hamiltonian (library H|psi>), psi (device, ld ldpsi[u]),
  * eig (host) and stats (optional array).  Units created on different contexts run
