@@ -200,6 +200,16 @@ void gemm_x32(hipStream_t s, GemmWorkspace& ws, int num_cu, char ta, char tb, in
 // nothing afterwards.
 void gemm_reserve(GemmWorkspace& ws, int num_cu);
 
+// ---------------------------------------------------------------- column blocks (colops.hip)
+// dst[:, c] = src[:, c] for rows x ncols column-major c128 blocks
+void copy_cols(hipStream_t s, int rows, int ncols, const double2* src, int lds, double2* dst, int ldd);
+// x[r0:r1, 0:ncols] = 0
+void zero_rows(hipStream_t s, int r0, int r1, int ncols, double2* x, int ld);
+// x[0:n] = 0 (doubles)
+void zero(hipStream_t s, size_t n, double* x);
+// vc[i + m nbase] = vcc[i + sel[m] nbx]: the columns sel[0..nc) of vcc, rows 0..nbase, packed
+void gather_cols(hipStream_t s, int nbase, int nc, const double2* vcc, int nbx, const int* sel, double2* vc);
+
 // ---------------------------------------------------------------- eigensolver
 // Lowest nband eigenpairs of Hermitian hcc (n x n, lower triangle, ld ldh), n <= HEEVX_MAX.
 constexpr int HEEVX_MAX = 2000;
@@ -209,6 +219,9 @@ void heevx_lowest(hipStream_t s, EigWorkspace& ws, int num_cu, int n, int ldh, c
 // Sizes ws for every n <= nmax, so heevx_lowest allocates nothing (a growing workspace
 // frees and allocates, and hipFree waits for the device).
 void heevx_reserve(EigWorkspace& ws, int nmax, int nband);
+// Text of a non-zero fault word of the eigensolver (EigWorkspace::bar[1]): bit 1, an exchange
+// poll of the reduction timed out; bit 2, d or e of the tridiagonal matrix is not finite.
+const char* eig_fault_text(unsigned word);
 
 // ---------------------------------------------------------------- small dense helpers
 // Cholesky of Hermitian positive definite G (n x n, ld ldg, upper triangle read) and inverse

@@ -24,38 +24,6 @@ namespace {
 
 int gsz2(size_t n) { return (int)std::max<size_t>(1, std::min<size_t>((n + 255) / 256, 8192)); }
 
-__global__ void k_ds_copy_cols(int rows, int ncols, const double2* __restrict__ src, int lds, double2* __restrict__ dst,
-                               int ldd) {
-    const size_t total = (size_t)rows * ncols;
-    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
-        const int r = (int)(t % rows), c = (int)(t / rows);
-        dst[(size_t)c * ldd + r] = src[(size_t)c * lds + r];
-    }
-}
-
-__global__ void k_ds_zero_rows(int r0, int r1, int ncols, double2* __restrict__ x, int ld) {
-    const int rows = r1 - r0;
-    const size_t total = (size_t)rows * ncols;
-    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
-        const int r = r0 + (int)(t % rows), c = (int)(t / rows);
-        x[(size_t)c * ld + r] = make_double2(0.0, 0.0);
-    }
-}
-
-__global__ void k_ds_zero(size_t n, double* __restrict__ x) {
-    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) x[t] = 0.0;
-}
-
-// vc[i + m*nbase] = vcc[i + unconv[m]*nbx]   (cal_grad's in-place column compaction, :306-313)
-__global__ void k_ds_gather(int nbase, int nc, const double2* __restrict__ vcc, int nbx, const int* __restrict__ unconv,
-                            double2* __restrict__ vc) {
-    const size_t total = (size_t)nbase * nc;
-    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
-        const int i = (int)(t % nbase), m = (int)(t / nbase);
-        vc[t] = vcc[(size_t)unconv[m] * nbx + i];
-    }
-}
-
 // X[:, m] *= -e[m]   (:330-341)
 __global__ void k_ds_scale(int dim, int nc, double2* __restrict__ x, const double* __restrict__ e) {
     const size_t total = (size_t)dim * nc;
@@ -223,10 +191,10 @@ struct DavSub {
         gemm_reserve(ctx->gws, ctx->num_cu);
         unconv_d.reserve(nband);
         status.reserve(4);
-        hipLaunchKernelGGL(k_ds_zero, dim3(1), dim3(64), 0, st(), (size_t)2, reinterpret_cast<double*>(status.get()));
+        zero(st(), (size_t)2, reinterpret_cast<double*>(status.get()));
         for (DevBuf<double>* b : {&hcc, &scc, &vcc})
-            hipLaunchKernelGGL(k_ds_zero, dim3(gsz2((size_t)2 * nbx * nbx)), dim3(256), 0, st(), (size_t)2 * nbx * nbx,
-                               b->get());
+            zero(st(), (size_t)2 * nbx * nbx,
+                 b->get());
         APW_LAUNCH_CHECK();
         APW_HIP(hipStreamSynchronize(st()));
     }
@@ -258,7 +226,7 @@ struct DavSub {
     }
 
     void zero_status() {
-        hipLaunchKernelGGL(k_ds_zero, dim3(1), dim3(64), 0, st(), (size_t)2, reinterpret_cast<double*>(status.get()));
+        zero(st(), (size_t)2, reinterpret_cast<double*>(status.get()));
         APW_LAUNCH_CHECK();
     }
 
@@ -273,7 +241,7 @@ struct DavSub {
             throw Error{APW_EBREAKDOWN};
         }
         if (b[1] != 0) {
-            set_error("Diago_DavSubspace: subspace eigensolver grid barrier timed out");
+            set_error("Diago_DavSubspace: %s", eig_fault_text(b[1]));
             throw Error{APW_EEIG};
         }
     }
@@ -282,8 +250,8 @@ struct DavSub {
     void diagH_subspace(int ldpsi, double* psi, double* en) {
         // H psi into tmp (dim x nband, ld dim); the callback sees psi with ld = dim only
         // when ldpsi == dim, so gather first
-        hipLaunchKernelGGL(k_ds_copy_cols, dim3(gsz2((size_t)dim * nband)), dim3(256), 0, st(), dim, nband,
-                           C2(psi), ldpsi, C2(psi_iter.get()), dim);
+        copy_cols(st(), dim, nband,
+                  C2(psi), ldpsi, C2(psi_iter.get()), dim);
         APW_LAUNCH_CHECK();
         apply_h(psi_iter.get(), nband, tmp.get(), 0, nband - 1);
         gemm('C', 'N', nband, nband, dim, 1.0, psi_iter.get(), dim, tmp.get(), dim, 0.0, hs.get(), nband);
@@ -293,8 +261,8 @@ struct DavSub {
                      nband, status.get(), &ctx->prof);
         check_status("diagH_subspace: overlap matrix not positive definite");
         gemm('N', 'N', dim, nband, nband, 1.0, psi_iter.get(), dim, vs.get(), nband, 0.0, tmp.get(), dim);
-        hipLaunchKernelGGL(k_ds_copy_cols, dim3(gsz2((size_t)dim * nband)), dim3(256), 0, st(), dim, nband,
-                           C2(tmp.get()), dim, C2(psi), ldpsi);
+        copy_cols(st(), dim, nband,
+                  C2(tmp.get()), dim, C2(psi), ldpsi);
         APW_LAUNCH_CHECK();
         APW_HIP(hipMemcpyAsync(en, eig.get(), nband * sizeof(double), hipMemcpyDeviceToHost, st()));
         APW_HIP(hipStreamSynchronize(st()));
@@ -337,8 +305,8 @@ struct DavSub {
         for (int i = 0; i < nc; ++i) e[i] = e_iter[unconv[i]];
         APW_HIP(hipMemcpyAsync(unconv_d.get(), unconv.data(), nc * sizeof(int), hipMemcpyHostToDevice, st()));
         APW_HIP(hipMemcpyAsync(ec.get(), e.data(), nc * sizeof(double), hipMemcpyHostToDevice, st()));
-        hipLaunchKernelGGL(k_ds_gather, dim3(gsz2((size_t)nbase * nc)), dim3(256), 0, st(), nbase, nc, C2(vcc.get()),
-                           nbx, unconv_d.get(), C2(vc.get()));
+        gather_cols(st(), nbase, nc, C2(vcc.get()),
+                    nbx, unconv_d.get(), C2(vc.get()));
         APW_LAUNCH_CHECK();
         double* NB = Bd(psi_iter.get(), nbase);
         gemm('N', 'N', dim, nc, nbase, 1.0, psi_iter.get(), dim, vc.get(), nbase, 0.0, NB, dim);
@@ -359,8 +327,8 @@ struct DavSub {
     // refresh (:709-798); psi_iter[:, :nband] already holds the new vectors
     void refresh(int nbase, const double* eig_in) {
         gemm('N', 'N', dim, nband, nbase, 1.0, hphi.get(), dim, vcc.get(), nbx, 0.0, tmp.get(), dim);
-        hipLaunchKernelGGL(k_ds_copy_cols, dim3(gsz2((size_t)dim * nband)), dim3(256), 0, st(), dim, nband,
-                           C2(tmp.get()), dim, C2(hphi.get()), dim);
+        copy_cols(st(), dim, nband,
+                  C2(tmp.get()), dim, C2(hphi.get()), dim);
         APW_HIP(hipMemcpyAsync(ec.get(), eig_in, nband * sizeof(double), hipMemcpyHostToDevice, st()));
         hipLaunchKernelGGL(k_ds_refresh, dim3(gsz2((size_t)nband * nband)), dim3(256), 0, st(), nband, nbx,
                            C2(hcc.get()), C2(scc.get()), C2(vcc.get()), ec.get());
@@ -373,8 +341,8 @@ struct DavSub {
         std::vector<int> unconv(nband);
         for (int m = 0; m < nband; ++m) unconv[m] = m;
         notconv = nband;
-        hipLaunchKernelGGL(k_ds_copy_cols, dim3(gsz2((size_t)dim * nband)), dim3(256), 0, st(), dim, nband, C2(psi),
-                           ldpsi, C2(psi_iter.get()), dim);
+        copy_cols(st(), dim, nband, C2(psi),
+                  ldpsi, C2(psi_iter.get()), dim);
         APW_LAUNCH_CHECK();
         apply_h(psi_iter.get(), nbx, hphi.get(), 0, nband - 1);
         int nbase = cal_elem(0, nband);
@@ -395,12 +363,11 @@ struct DavSub {
             if (notconv == 0 || nbase + notconv + 1 > nbx || it == iter_nmax) {
                 gemm('N', 'N', dim, nband, nbase, 1.0, psi_iter.get(), dim, vcc.get(), nbx, 0.0, psi, ldpsi);
                 if (ldpsi > dim) {
-                    hipLaunchKernelGGL(k_ds_zero_rows, dim3(gsz2((size_t)(ldpsi - dim) * nband)), dim3(256), 0, st(),
-                                       dim, ldpsi, nband, C2(psi), ldpsi);
+                    zero_rows(st(), dim, ldpsi, nband, C2(psi), ldpsi);
                     APW_LAUNCH_CHECK();
                 }
-                hipLaunchKernelGGL(k_ds_copy_cols, dim3(gsz2((size_t)dim * nband)), dim3(256), 0, st(), dim, nband,
-                                   C2(psi), ldpsi, C2(psi_iter.get()), dim);
+                copy_cols(st(), dim, nband,
+                          C2(psi), ldpsi, C2(psi_iter.get()), dim);
                 APW_LAUNCH_CHECK();
                 // refresh on both branches, as the reference does: the next diag_once's
                 // is_subspace solve reads the identity it leaves in vcc

@@ -35,28 +35,6 @@ namespace {
 
 int gsz(size_t n) { return (int)std::max<size_t>(1, std::min<size_t>((n + 255) / 256, 8192)); }
 
-__global__ void k_copy_cols(int rows, int ncols, const double2* __restrict__ src, int lds, double2* __restrict__ dst,
-                            int ldd) {
-    const size_t total = (size_t)rows * ncols;
-    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
-        const int r = (int)(t % rows), c = (int)(t / rows);
-        dst[(size_t)c * ldd + r] = src[(size_t)c * lds + r];
-    }
-}
-
-__global__ void k_zero_rows(int r0, int r1, int ncols, double2* __restrict__ x, int ld) {
-    const int rows = r1 - r0;
-    const size_t total = (size_t)rows * ncols;
-    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
-        const int r = r0 + (int)(t % rows), c = (int)(t / rows);
-        x[(size_t)c * ld + r] = make_double2(0.0, 0.0);
-    }
-}
-
-__global__ void k_zero(size_t n, double* __restrict__ x) {
-    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) x[t] = 0.0;
-}
-
 // c128 <-> c64 column blocks (precision=single callers: c64 psi and callbacks)
 __global__ void k_cols_narrow(int rows, int ncols, const double2* __restrict__ src, int lds, float2* __restrict__ dst,
                               int ldd) {
@@ -75,16 +53,6 @@ __global__ void k_cols_widen(int rows, int ncols, const float2* __restrict__ src
         const int r = (int)(t % rows), c = (int)(t / rows);
         const float2 v = src[(size_t)c * lds + r];
         dst[(size_t)c * ldd + r] = make_double2((double)v.x, (double)v.y);
-    }
-}
-
-// vc[i + m*nbase] = vcc[i + unconv[m]*nbx]     (cal_grad, diago_david.cpp:408-415)
-__global__ void k_gather_vc(int nbase, int nc, const double2* __restrict__ vcc, int nbx, const int* __restrict__ unconv,
-                            double2* __restrict__ vc) {
-    const size_t total = (size_t)nbase * nc;
-    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
-        const int i = (int)(t % nbase), m = (int)(t / nbase);
-        vc[t] = vcc[(size_t)unconv[m] * nbx + i];
     }
 }
 
@@ -232,8 +200,8 @@ struct David {
         gemm_reserve(ctx->gws, ctx->num_cu);
         // pinned host words of the per-iteration reads (pageable copies go through staging)
         APW_HIP(hipHostMalloc(reinterpret_cast<void**>(&hbuf), hbuf_bytes(), hipHostMallocDefault));
-        hipLaunchKernelGGL(k_zero, dim3(gsz((size_t)2 * nbx * nbx)), dim3(256), 0, st(), (size_t)2 * nbx * nbx, vcc.get());
-        hipLaunchKernelGGL(k_zero, dim3(gsz((size_t)2 * nbx * nbx)), dim3(256), 0, st(), (size_t)2 * nbx * nbx, hcc.get());
+        zero(st(), (size_t)2 * nbx * nbx, vcc.get());
+        zero(st(), (size_t)2 * nbx * nbx, hcc.get());
         APW_LAUNCH_CHECK();
         APW_HIP(hipStreamSynchronize(st()));
     }
@@ -298,7 +266,7 @@ struct David {
     // block classical Gram-Schmidt with Pythagorean norms (SchmidtOrth, diago_david.cpp:950-1075),
     // in blocks of at most CHOL_MAX columns, each against everything before it.
     void orthonormalize(int c0, int nc) {
-        hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, st(), (size_t)2, reinterpret_cast<double*>(status.get()));
+        zero(st(), (size_t)2, reinterpret_cast<double*>(status.get()));
         APW_LAUNCH_CHECK();
         // equal blocks rounded up to whole 32-column GEMM tiles (two blocks of 80 left 16
         // columns of every third tile idle in the block's products; 96 + 64 fill them)
@@ -323,8 +291,8 @@ struct David {
         chol_inv_upper(st(), nc, G, ld, Rinv, ld, status.get(), 1.0e-12, &ctx->prof);
         if (c0 > 0) gemm('N', 'N', c0, nc, nc, -1.0, W, ld, Rinv, ld, 0.0, X, ld);
         gemm('N', 'N', dim, nc, ld, 1.0, basis.get(), dim, X, ld, 0.0, tmp.get(), dim);
-        hipLaunchKernelGGL(k_copy_cols, dim3(gsz((size_t)dim * nc)), dim3(256), 0, st(), dim, nc,
-                           reinterpret_cast<const double2*>(tmp.get()), dim, reinterpret_cast<double2*>(V), dim);
+        copy_cols(st(), dim, nc,
+                  reinterpret_cast<const double2*>(tmp.get()), dim, reinterpret_cast<double2*>(V), dim);
         APW_LAUNCH_CHECK();
         apply_s(c0, nc);
     }
@@ -351,7 +319,7 @@ struct David {
             throw Error{APW_EBREAKDOWN};
         }
         if (b[1] != 0) {
-            set_error("DiagoDavid: subspace eigensolver grid barrier timed out");
+            set_error("DiagoDavid: %s", eig_fault_text(b[1]));
             throw Error{APW_EEIG};
         }
     }
@@ -374,8 +342,8 @@ struct David {
         for (int m = 0; m < nband; ++m) unconv[m] = m;
         int notconv = nband;
         // initial orthonormalisation of the start vectors (:186-228)
-        hipLaunchKernelGGL(k_copy_cols, dim3(gsz((size_t)dim * nband)), dim3(256), 0, st(), dim, nband,
-                           reinterpret_cast<const double2*>(psi), ldpsi, reinterpret_cast<double2*>(basis.get()), dim);
+        copy_cols(st(), dim, nband,
+                  reinterpret_cast<const double2*>(psi), ldpsi, reinterpret_cast<double2*>(basis.get()), dim);
         APW_LAUNCH_CHECK();
         orthonormalize(0, nband);
         apply_h(hp.get(), 0, nband - 1, prec);                                    // :233
@@ -393,8 +361,8 @@ struct David {
             std::copy(unconv.begin(), unconv.begin() + notconv, h_unconv());
             APW_HIP(hipMemcpyAsync(unconv_d.get(), h_unconv(), notconv * sizeof(int), hipMemcpyHostToDevice, st()));
             double2* vcp = reinterpret_cast<double2*>(vc.get());
-            hipLaunchKernelGGL(k_gather_vc, dim3(gsz((size_t)nbase * notconv)), dim3(256), 0, st(), nbase, notconv,
-                               reinterpret_cast<const double2*>(vcc.get()), nbx, unconv_d.get(), vcp);
+            gather_cols(st(), nbase, notconv,
+                        reinterpret_cast<const double2*>(vcc.get()), nbx, unconv_d.get(), vcp);
             double* NB = Bd(basis.get(), nbase);
             gemm('N', 'N', dim, notconv, nbase, 1.0, hp.get(), dim, vc.get(), nbase, 0.0, NB, dim);
             hipLaunchKernelGGL(k_scale_vc, dim3(gsz((size_t)nbase * notconv)), dim3(256), 0, st(), nbase, notconv, vcp,
@@ -431,8 +399,8 @@ struct David {
                 // psi = basis vcc (:292-306)
                 gemm('N', 'N', dim, nband, nbase, 1.0, basis.get(), dim, vcc.get(), nbx, 0.0, psi, ldpsi);
                 if (ldpsi > dim) {
-                    hipLaunchKernelGGL(k_zero_rows, dim3(gsz((size_t)(ldpsi - dim) * nband)), dim3(256), 0, st(), dim,
-                                       ldpsi, nband, reinterpret_cast<double2*>(psi), ldpsi);
+                    zero_rows(st(), dim,
+                              ldpsi, nband, reinterpret_cast<double2*>(psi), ldpsi);
                     APW_LAUNCH_CHECK();
                 }
                 if (!notconv || it == maxiter || leave_mixed) {
@@ -452,20 +420,20 @@ struct David {
                 }
                 // refresh (:797-934)
                 gemm('N', 'N', dim, nband, nbase, 1.0, hp.get(), dim, vcc.get(), nbx, 0.0, tmp.get(), dim);
-                hipLaunchKernelGGL(k_copy_cols, dim3(gsz((size_t)dim * nband)), dim3(256), 0, st(), dim, nband,
-                                   reinterpret_cast<const double2*>(tmp.get()), dim,
-                                   reinterpret_cast<double2*>(hp.get()), dim);
+                copy_cols(st(), dim, nband,
+                          reinterpret_cast<const double2*>(tmp.get()), dim,
+                          reinterpret_cast<double2*>(hp.get()), dim);
                 if (scb) {
                     gemm('N', 'N', dim, nband, nbase, 1.0, sp.get(), dim, vcc.get(), nbx, 0.0, tmp.get(), dim);
-                    hipLaunchKernelGGL(k_copy_cols, dim3(gsz((size_t)dim * nband)), dim3(256), 0, st(), dim, nband,
-                                       reinterpret_cast<const double2*>(tmp.get()), dim,
-                                       reinterpret_cast<double2*>(sp.get()), dim);
+                    copy_cols(st(), dim, nband,
+                              reinterpret_cast<const double2*>(tmp.get()), dim,
+                              reinterpret_cast<double2*>(sp.get()), dim);
                 }
-                hipLaunchKernelGGL(k_copy_cols, dim3(gsz((size_t)dim * nband)), dim3(256), 0, st(), dim, nband,
-                                   reinterpret_cast<const double2*>(psi), ldpsi,
-                                   reinterpret_cast<double2*>(basis.get()), dim);
-                hipLaunchKernelGGL(k_zero, dim3(gsz((size_t)2 * nbx * nbx)), dim3(256), 0, st(), (size_t)2 * nbx * nbx,
-                                   hcc.get());
+                copy_cols(st(), dim, nband,
+                          reinterpret_cast<const double2*>(psi), ldpsi,
+                          reinterpret_cast<double2*>(basis.get()), dim);
+                zero(st(), (size_t)2 * nbx * nbx,
+                     hcc.get());
                 hipLaunchKernelGGL(k_refresh_diag, dim3(1), dim3(256), 0, st(), nband,
                                    reinterpret_cast<double2*>(hcc.get()), reinterpret_cast<double2*>(vcc.get()), nbx,
                                    eig.get());

@@ -618,15 +618,23 @@ __device__ int sturm(const double* d, const double* e2, int n, double x) {
 // Multisection: one wave per eigenvalue index j; each step the 64 lanes evaluate the Sturm
 // count at 64 interior points of [lo, hi] and the wave keeps the sub-interval holding the
 // j-th eigenvalue (a 65x shrink per step, ~9 steps to full precision).
+// A d or e that is not finite (a corrupted reduction) raises bit 2 of *bad (when given), so the
+// caller reports an eigensolver error instead of returning meaningless pairs.
 __global__ __launch_bounds__(256) void k_bisect(const double* __restrict__ d, const double* __restrict__ e, int n,
-                                                int nband, double* __restrict__ w) {
+                                                int nband, double* __restrict__ w, unsigned* __restrict__ bad) {
     extern __shared__ __align__(16) unsigned char smem[];
     double* ds = reinterpret_cast<double*>(smem);   // d, then e^2
     double* e2 = ds + n;
+    bool finite = true;
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
         ds[i] = d[i];
-        if (i < n - 1) e2[i] = e[i] * e[i];
+        finite = finite && isfinite(ds[i]);
+        if (i < n - 1) {
+            e2[i] = e[i] * e[i];
+            finite = finite && isfinite(e2[i]);
+        }
     }
+    if (!finite && bad && blockIdx.x == 0) atomicOr(bad, 2u);
     __syncthreads();
     const int lane = threadIdx.x & 63;
     const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -1288,7 +1296,7 @@ void heevx_lowest(hipStream_t s, EigWorkspace& ws, int num_cu, int n, int ldh, c
     {
     ProfScope ps(prof, s, "eig_bisect", 0.0, 0.0);
     hipLaunchKernelGGL(k_bisect, dim3((nband + 3) / 4), dim3(256), (size_t)2 * n * sizeof(double), s, ws.d.get(),
-                       ws.e.get(), n, nband, eig);
+                       ws.e.get(), n, nband, eig, n > 1 ? ws.bar.get() + 1 : nullptr);
     APW_LAUNCH_CHECK();
     }
     {
@@ -1331,6 +1339,12 @@ void heevx_lowest(hipStream_t s, EigWorkspace& ws, int num_cu, int n, int ldh, c
     APW_LAUNCH_CHECK();
 }
 
+const char* eig_fault_text(unsigned word) {
+    if (word & 1u) return "subspace eigensolver: an exchange poll of the tridiagonal reduction timed out";
+    if (word & 2u) return "subspace eigensolver: the tridiagonal reduction produced a non-finite value";
+    return "subspace eigensolver: no fault";
+}
+
 void chol_inv_upper(hipStream_t s, int n, const double* g, int ldg, double* rinv, int ldr, int* status,
                     double min_norm, Prof* prof) {
     APW_CHECK(n >= 1 && n <= CHOL_MAX, "chol_inv_upper: n=%d outside 1..%d", n, CHOL_MAX);
@@ -1359,7 +1373,7 @@ extern "C" int apw_dnevx_op(apw_ctx* ctx, int nstart, int ldh, const double* hcc
         APW_HIP(hipStreamSynchronize(ctx->stream));
         unsigned flag[2] = {0, 0};
         if (nstart > 1) APW_HIP(hipMemcpy(flag, ws.bar.get(), sizeof(flag), hipMemcpyDeviceToHost));
-        APW_CHECK(flag[1] == 0, "apw_dnevx_op: grid barrier timed out");
+        APW_CHECK(flag[1] == 0, "apw_dnevx_op: %s", apw::eig_fault_text(flag[1]));
         return APW_OK;
     });
 }

@@ -108,7 +108,7 @@ int apw_dngvx_op(apw_ctx* ctx, int nstart, int ldh, const double* hcc, const dou
             APW_HIP(hipMemcpyAsync(flag, ctx->ews.bar.get(), sizeof(flag), hipMemcpyDeviceToHost, ctx->stream));
         APW_HIP(hipStreamSynchronize(ctx->stream));
         APW_CHECK(st == 0, "apw_dngvx_op: B is not positive definite (block pivot %d)", st - 1);
-        APW_CHECK(flag[1] == 0, "apw_dngvx_op: grid barrier timed out");
+        APW_CHECK(flag[1] == 0, "apw_dngvx_op: %s", apw::eig_fault_text(flag[1]));
         return APW_OK;
     });
 }

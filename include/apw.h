@@ -332,6 +332,10 @@ int apw_cgemm(apw_ctx* ctx, char transa, char transb, int m, int n, int k,
 int apw_zgemm_x32(apw_ctx* ctx, char transa, char transb, int m, int n, int k,
                   const double* alpha, const double* a, int lda, const double* b, int ldb,
                   const double* beta, double* c, int ldc);
+/* Writes the (row, column) of the MFMA result each lane and accumulator register holds, for
+ * v_mfma_f64_16x16x4f64 and v_mfma_f32_16x16x4f32 (4 x 64 x 4 doubles at out_dev).  No
+ * counterpart in the reference: a check of the operand layout the GEMMs rely on. */
+int apw_probe_mfma_layout(apw_ctx* ctx, double* out_dev);
 /* dnevx_op<complex<double>, DEVICE_GPU> (dngvd_op.h:85-107): lowest nbands eigenpairs
  * of the Hermitian nstart x nstart matrix hcc (lower triangle, ld ldh).  eigenvalue:
  * device double[nbands]; vcc: device c128, ld ldh, eigenvector j in column j. */
