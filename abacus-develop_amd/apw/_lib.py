@@ -102,6 +102,7 @@ PROTOTYPES = {
     "apw_cgemm": (_i, [_vp, _ch, _ch, _i, _i, _i, C.POINTER(C.c_float), _vp, _i, _vp, _i, C.POINTER(C.c_float), _vp, _i]),
     "apw_dnevx_op": (_i, [_vp, _i, _i, _vp, _i, _vp, _vp]),
     "apw_probe_mfma_layout": (_i, [_vp, _vp]),
+    "apw_zgemm_noalloc": (_i, [_vp, _ch, _ch, _i, _i, _i, C.POINTER(_d), _vp, _i, _vp, _i, C.POINTER(_d), _vp, _i]),
 }
 
 _LIB = None

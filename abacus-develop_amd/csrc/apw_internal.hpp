@@ -85,6 +85,10 @@ struct DevBuf {
 
 struct GemmWorkspace {
     DevBuf<double> part;
+    // When ext is set, the split-K partials go there (ext_n doubles, not owned) and a
+    // product that needs more is an error: gemm then allocates nothing.
+    double* ext = nullptr;
+    size_t ext_n = 0;
 };
 
 // ---------------------------------------------------------------- kernel profiler
@@ -199,6 +203,9 @@ void gemm_x32(hipStream_t s, GemmWorkspace& ws, int num_cu, char ta, char tb, in
 // Sizes the split-K partials for every call on this GPU (num_cu CUs), so gemm allocates
 // nothing afterwards.
 void gemm_reserve(GemmWorkspace& ws, int num_cu);
+// Device address of this GPU's copy of the fixed split-K workspace of apw_zgemm_noalloc
+// (looked up once per device; call with that device current).
+double* gemm_scratch_init(int device);
 
 // ---------------------------------------------------------------- column blocks (colops.hip)
 // dst[:, c] = src[:, c] for rows x ncols column-major c128 blocks

@@ -136,6 +136,13 @@ int apw_ctx_create(apw_ctx** out, int device, void* hip_stream) {
         hipDeviceProp_t prop;
         APW_HIP(hipGetDeviceProperties(&prop, device));
         c->num_cu = prop.multiProcessorCount;
+        try {
+            (void)apw::gemm_scratch_init(device);   // per device, once: apw_zgemm_noalloc's workspace
+        } catch (...) {
+            if (c->own_stream) (void)hipStreamDestroy(c->stream);
+            delete c;
+            throw;
+        }
         *out = c;
         return (int)APW_OK;
     });

@@ -17,6 +17,7 @@
 // f32 one (checked on the device by the layout probe below and its unit test).
 #include <cstdio>
 #include <cstdlib>
+#include <mutex>
 #include <type_traits>
 #include <utility>
 
@@ -425,8 +426,16 @@ void gemm_impl(hipStream_t s, GemmWorkspace& ws, int num_cu, char ta, char tb, i
     S* part = nullptr;
     if (splits > 1) {
         const size_t need = (size_t)splits * m * n * 2 * sizeof(S) / sizeof(double) + 2;
-        ws.part.reserve(need);
-        part = reinterpret_cast<S*>(ws.part.get());
+        if (ws.ext) {
+            APW_CHECK(need <= ws.ext_n,
+                      "gemm: %c%c m=%d n=%d k=%d splits over k into %d partial products of %zu bytes, more than "
+                      "the %zu-byte fixed workspace", ta, tb, m, n, k, splits, need * sizeof(double),
+                      ws.ext_n * sizeof(double));
+            part = reinterpret_cast<S*>(ws.ext);
+        } else {
+            ws.part.reserve(need);
+            part = reinterpret_cast<S*>(ws.part.get());
+        }
     }
     static const bool trace = std::getenv("APW_GEMM_TRACE") != nullptr;   // tuning aid: one line per call
     if (trace)
@@ -525,7 +534,55 @@ __global__ void probe_mfma_kernel(double* out) {
     }
 }
 
+// ---------------------------------------------------------------- fixed split-K workspace
+// The split-K partials of apw_zgemm_noalloc live in this file-scope device array instead of
+// memory the library allocates.  The default plan keeps every launch resident at once, so
+// splits x m x n <= (resident workgroups per CU) x CUs x tile = 3 x 256 x 64 x 64 complex
+// values for every shape on a 256-CU GPU: 48 MiB of c128 covers them all.  Each GPU holds
+// its own copy from the time the library is loaded until the process ends.
+constexpr size_t SCRATCH_DOUBLES = (size_t)3 * 256 * 64 * 64 * 2;
+static __device__ __align__(256) double scratch[SCRATCH_DOUBLES];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[MAX_DEVICES] = {};
+}  // namespace
+
+// The device address of this GPU's copy of scratch; `scratch` itself names only a host-side
+// stand-in.  Looked up once per device, with that device current (apw_ctx_create).
+double* gemm_scratch_init(int device) {
+    APW_CHECK(device >= 0 && device < MAX_DEVICES, "gemm scratch: device %d out of range", device);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        int cur = -1;
+        APW_HIP(hipGetDevice(&cur));
+        APW_CHECK(cur == device, "gemm scratch: device %d is not current (%d)", device, cur);
+        void* ptr = nullptr;
+        APW_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        APW_CHECK(ptr != nullptr, "gemm scratch: no device address on device %d", device);
+        g_scratch_ptr[device] = static_cast<double*>(ptr);
+    }
+    return g_scratch_ptr[device];
+}
+
 }  // namespace apw
+
+extern "C" int apw_zgemm_noalloc(apw_ctx* ctx, char transa, char transb, int m, int n, int k, const double* alpha,
+                                 const double* a, int lda, const double* b, int ldb, const double* beta, double* c,
+                                 int ldc) {
+    return apw::guarded([&] {
+        APW_CHECK(ctx && alpha && beta && c, "apw_zgemm_noalloc: null argument");
+        APW_CHECK(k == 0 || (a && b), "apw_zgemm_noalloc: null A/B");
+        apw::DeviceGuard g(ctx->device);
+        apw::GemmWorkspace ws;   // owns nothing: the partials go to the fixed array
+        ws.ext = apw::gemm_scratch_init(ctx->device);   // looked up when the context was created
+        ws.ext_n = apw::SCRATCH_DOUBLES;
+        apw::gemm<double>(ctx->stream, ws, ctx->num_cu, transa, transb, m, n, k, alpha[0], alpha[1], a, lda, b, ldb,
+                          beta[0], beta[1], c, ldc, nullptr);
+        return (int)APW_OK;
+    });
+}
 
 extern "C" int apw_probe_mfma_layout(apw_ctx* ctx, double* out_dev) {
     return apw::guarded([&] {

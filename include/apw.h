@@ -332,6 +332,18 @@ int apw_cgemm(apw_ctx* ctx, char transa, char transb, int m, int n, int k,
 int apw_zgemm_x32(apw_ctx* ctx, char transa, char transb, int m, int n, int k,
                   const double* alpha, const double* a, int lda, const double* b, int ldb,
                   const double* beta, double* c, int ldc);
+/* No counterpart in the reference: apw_zgemm for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same product, plan and result as apw_zgemm, but its split-K partials go to a fixed 48 MiB
+ * device array of the library (one per GPU, held until the process ends) instead of a
+ * workspace it allocates; it calls no hipMalloc/hipFree.  A product whose split-K partials
+ * would not fit is rejected with APW_EINVAL (the default plan fits every shape on GPUs of
+ * up to 256 CUs).  Asynchronous on the context's stream (no synchronisation).  Calls on one
+ * GPU share the array: calls that overlap in time on one GPU (two contexts, two streams)
+ * race and must be serialised by the caller. */
+int apw_zgemm_noalloc(apw_ctx* ctx, char transa, char transb, int m, int n, int k,
+                      const double* alpha, const double* a, int lda, const double* b, int ldb,
+                      const double* beta, double* c, int ldc);
 /* Writes the (row, column) of the MFMA result each lane and accumulator register holds, for
  * v_mfma_f64_16x16x4f64 and v_mfma_f32_16x16x4f32 (4 x 64 x 4 doubles at out_dev).  No
  * counterpart in the reference: a check of the operand layout the GEMMs rely on. */
