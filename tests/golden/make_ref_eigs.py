@@ -13,7 +13,8 @@ ntry_max 5, notconv_max 0 (HSolverPW's defaults, hsolver_pw.cpp:472-545) and rec
 eigenvalues, the iteration count and the H|psi> call/band counts.  The cases: the 8-atom
 cell, k-points 0 and 37 of the 16-atom 4x4x4 mesh, cells 1, 3 and 6 of the uneven batch,
 and the 64-atom cell (about 5 min on 8 threads).  ``--si216`` adds the 216-atom cell
-(hours, ~30 GB); ``--only NAME`` recomputes one case and keeps the others.
+(about 35 min per build on 6 threads, ~30 GB); ``--only NAME`` recomputes one case and keeps
+the others.  Each case records the OpenMP thread count it ran with.
 
     OMP_NUM_THREADS=8 python tests/golden/make_ref_eigs.py [--si216] [--only si64]
 """
@@ -43,7 +44,7 @@ def case_key(name, ik):
 def solve(name, ik, out, save):
     s = configs.make(name, ik=ik) if name == "kpts16" else configs.make(name)
     out.update({"npw": s.npw, "npwx": s.npwx, "grid": [s.nx, s.ny, s.nz], "nband": s.nband, "nkb": s.nkb,
-                "david_ndim": s.david_ndim, "thr": 1e-9})
+                "david_ndim": s.david_ndim, "thr": 1e-9, "threads": int(os.environ.get("OMP_NUM_THREADS", "0"))})
     for label, det in (("plain", False), ("detsign", True)):
         t0 = time.perf_counter()
         eig, _, it, calls, bands = ref_david.david(s, thr=1e-9, maxiter=500, ntry_max=5, notconv_max=0, detsign=det)
@@ -66,7 +67,6 @@ def main():
     if os.path.exists(OUT):
         with open(OUT) as f:
             out = json.load(f)
-    out["_threads"] = os.environ.get("OMP_NUM_THREADS", "default")
     def save():
         with open(OUT, "w") as f:
             json.dump(out, f, indent=1)
