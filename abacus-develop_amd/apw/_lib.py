@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(os.path.dirname(HERE), "lib", "libapw.so")
 APW_OK = 0
 APW_FP64 = 0
 APW_FP32_MIXED = 1
+APW_FP32_MIXED_X32 = 2
 
 STATUS = {-1: "APW_EINVAL", -2: "APW_EHIP", -3: "APW_ENOMEM", -4: "APW_EBREAKDOWN", -5: "APW_EEIG",
           -6: "APW_ECALLBACK"}
@@ -54,6 +55,7 @@ PROTOTYPES = {
     "apw_ctx_synchronize": (_i, [_vp]),
     "apw_ctx_stream": (_vp, [_vp]),
     "apw_ctx_set_profiling": (_i, [_vp, _i]),
+    "apw_ctx_set_eig_single_wg": (_i, [_vp, _i]),
     "apw_ctx_profile_count": (_i, [_vp]),
     "apw_ctx_profile_entry": (_i, [_vp, _i, C.c_char_p, _i, C.POINTER(C.c_longlong), C.POINTER(_d), C.POINTER(_d),
                                    C.POINTER(_d)]),
@@ -102,6 +104,12 @@ PROTOTYPES = {
     "apw_cgemm": (_i, [_vp, _ch, _ch, _i, _i, _i, C.POINTER(C.c_float), _vp, _i, _vp, _i, C.POINTER(C.c_float), _vp, _i]),
     "apw_dnevx_op": (_i, [_vp, _i, _i, _vp, _i, _vp, _vp]),
     "apw_probe_mfma_layout": (_i, [_vp, _vp]),
+    "apw_tune_set_mode": (_i, [_i]),
+    "apw_tune_load": (_i, [C.c_char_p]),
+    "apw_tune_save": (_i, [C.c_char_p]),
+    "apw_tune_clear": (_i, []),
+    "apw_tune_stats": (_i, [C.POINTER(C.c_longlong)] * 4),
+    "apw_tune_entry": (_i, [_i, C.c_char_p, _i, C.POINTER(_i), C.POINTER(_i), C.POINTER(_d)]),
     "apw_zgemm_noalloc": (_i, [_vp, _ch, _ch, _i, _i, _i, C.POINTER(_d), _vp, _i, _vp, _i, C.POINTER(_d), _vp, _i]),
 }
 

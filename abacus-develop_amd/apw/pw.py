@@ -24,10 +24,11 @@ import os
 import numpy as np
 
 from . import _lib
-from ._lib import APW_FP32_MIXED, APW_FP64, check
+from ._lib import APW_FP32_MIXED, APW_FP32_MIXED_X32, APW_FP64, check
 
 _PREC = {"fp64": APW_FP64, "double": APW_FP64, "fp32_mixed": APW_FP32_MIXED, "mixed": APW_FP32_MIXED,
-         APW_FP64: APW_FP64, APW_FP32_MIXED: APW_FP32_MIXED}
+         "fp32_mixed_x32": APW_FP32_MIXED_X32,
+         APW_FP64: APW_FP64, APW_FP32_MIXED: APW_FP32_MIXED, APW_FP32_MIXED_X32: APW_FP32_MIXED_X32}
 
 
 def _torch():
@@ -61,6 +62,11 @@ class Context:
 
     def pre(self):
         _torch().cuda.synchronize(self.device)
+
+    def set_eig_single_wg(self, nmax: int):
+        """Subspace matrices up to nmax are reduced by one workgroup (no device-wide ordering
+        of the reductions: concurrent small solves on several contexts)."""
+        check(self.lib.apw_ctx_set_eig_single_wg(self.handle, int(nmax)))
 
     def set_profiling(self, on: bool):
         check(self.lib.apw_ctx_set_profiling(self.handle, int(on)))
@@ -553,3 +559,53 @@ def probe_mfma_layout(ctx):
     check(ctx.lib.apw_probe_mfma_layout(ctx.handle, _ptr(out)))
     ctx.synchronize()
     return out.view(4, 64, 4).cpu().numpy()
+
+
+# ---------------------------------------------------------------- autotuner
+TUNE_FIXED, TUNE_CACHE, TUNE_MEASURE = 0, 1, 2
+TUNE_CACHE_FILE = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "tune_cache.json")
+
+
+def _tlib():
+    # torch before the library: libapw.so then binds to torch's HIP runtime (a process that
+    # loads the system runtime first holds two, and the second finds no device)
+    _torch()
+    return _lib.load()
+
+
+def tune_set_mode(mode: int):
+    """0: fixed rules; 1: the measured table, fixed rules on a miss; 2: as 1, and a miss is
+    measured where it occurs (every candidate plan timed on the calling stream)."""
+    check(_tlib().apw_tune_set_mode(int(mode)))
+
+
+def tune_load(path: str = TUNE_CACHE_FILE) -> int:
+    return check(_tlib().apw_tune_load(path.encode()))
+
+
+def tune_save(path: str = TUNE_CACHE_FILE) -> int:
+    return check(_tlib().apw_tune_save(path.encode()))
+
+
+def tune_clear():
+    check(_tlib().apw_tune_clear())
+
+
+def tune_stats():
+    """{entries, hits, misses, tuned} of the process-wide table."""
+    v = [C.c_longlong() for _ in range(4)]
+    check(_tlib().apw_tune_stats(*[C.byref(x) for x in v]))
+    return dict(zip(("entries", "hits", "misses", "tuned"), (x.value for x in v)))
+
+
+def tune_entries():
+    """[(key, (a, b), us)] of the table (a, b: the recorded choice)."""
+    lib = _tlib()
+    n = tune_stats()["entries"]
+    buf = C.create_string_buffer(256)
+    out = []
+    for i in range(n):
+        a, b, us = C.c_int(), C.c_int(), C.c_double()
+        check(lib.apw_tune_entry(i, buf, 256, C.byref(a), C.byref(b), C.byref(us)))
+        out.append((buf.value.decode(), (a.value, b.value), us.value))
+    return out

@@ -12,7 +12,8 @@ struct Ham {
     int npw = 0, npwx = 0, nkb = 0, nhm = 0, nat = 0;
     double tpiba2 = 0;
     int precision = APW_FP64;
-    int chunk = 1;                 // bands per local-pass launch
+    int chunk = 1;                 // bands per local-pass launch (FP64)
+    int chunk32 = 2;               // and FP32
     DevBuf<double> gk2, veff, vkb, deeq;
     DevBuf<float> veff32, vkb32;
     DevBuf<int> kb_iat, kb_off, kb_nh;
@@ -26,6 +27,7 @@ struct Ham {
     void local(const double* psi, int ld, double* out, int ldh, int nb, GatherMode mode, int prec);
     void nonlocal(const double* psi, int ld, double* out, int ldh, int nb, int prec);
     void act_ekinetic(int nb, int nbasis, const double* psi, double* out);
+    void tune_local();   // autotuner of the local pass: chunk and pass schedule per precision
 };
 
 }  // namespace apw

@@ -8,6 +8,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <deque>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -89,6 +90,7 @@ struct GemmWorkspace {
     // product that needs more is an error: gemm then allocates nothing.
     double* ext = nullptr;
     size_t ext_n = 0;
+    DevBuf<double> tune_c;   // the autotuner's scratch C (only when it measures)
 };
 
 // ---------------------------------------------------------------- kernel profiler
@@ -146,6 +148,9 @@ struct EigWorkspace {
     DevBuf<double> ring;   // 2n^2 c128 exchange ring of the reduction
     DevBuf<double> z;      // n x nband real eigenvectors of T
     DevBuf<unsigned> bar;  // [1]: abort word of the exchange polls
+    // n <= single_max: the reduction runs in one workgroup (ordinary launch, not ordered
+    // against other streams); 0 = always the multi-workgroup cooperative forms
+    int single_max = 0;
 };
 
 // work arrays of the generalized (pencil) solver
@@ -160,6 +165,8 @@ struct apw_ctx {
     hipStream_t stream = nullptr;
     bool own_stream = false;
     int num_cu = 0;
+    // subspace matrices up to this size are reduced by one workgroup (apw_ctx_set_eig_single_wg)
+    int eig_single_max = 0;
     // scratch shared by the functor-level entry points
     apw::DevBuf<double> scratch;
     apw::GemmWorkspace gws;
@@ -186,6 +193,15 @@ struct DeviceGuard {
         if (cur != prev) (void)hipSetDevice(prev);
     }
 };
+
+// ---------------------------------------------------------------- autotuner (tune.hip)
+// Process-wide table of measured choices (two ints per key).  tune_mode(): 0 = fixed rules,
+// 1 = the table, fixed rules on a miss, 2 = as 1 and a miss is measured and recorded.
+int tune_mode();
+bool tune_find(const std::string& key, int& a, int& b);
+void tune_store(const std::string& key, int a, int b, double us);
+// Fastest of reps timed runs of run() on stream s (after one untimed run), in microseconds.
+double time_on_stream(hipStream_t s, int reps, const std::function<void()>& run);
 
 // ---------------------------------------------------------------- GEMM
 // Column-major complex GEMM, C = alpha op(A) op(B) + beta C.  op: 'N', 'T', 'C'.

@@ -174,6 +174,14 @@ int apw_ctx_synchronize(apw_ctx* ctx) {
 
 void* apw_ctx_stream(apw_ctx* ctx) { return ctx ? reinterpret_cast<void*>(ctx->stream) : nullptr; }
 
+int apw_ctx_set_eig_single_wg(apw_ctx* ctx, int nmax) {
+    return guarded([&] {
+        APW_CHECK(ctx && nmax >= 0, "apw_ctx_set_eig_single_wg: null ctx or negative size");
+        ctx->eig_single_max = nmax;
+        return (int)APW_OK;
+    });
+}
+
 int apw_ctx_set_profiling(apw_ctx* ctx, int on) {
     return guarded([&] {
         APW_CHECK(ctx, "apw_ctx_set_profiling: null ctx");

@@ -257,6 +257,7 @@ struct DavSub {
         gemm('C', 'N', nband, nband, dim, 1.0, psi_iter.get(), dim, tmp.get(), dim, 0.0, hs.get(), nband);
         gemm('C', 'N', nband, nband, dim, 1.0, psi_iter.get(), dim, psi_iter.get(), dim, 0.0, ss.get(), nband);
         zero_status();
+        ews.single_max = ctx->eig_single_max;
         hegvx_lowest(st(), gws(), ews, gen, ctx->num_cu, nband, nband, hs.get(), ss.get(), nband, eig.get(), vs.get(),
                      nband, status.get(), &ctx->prof);
         check_status("diagH_subspace: overlap matrix not positive definite");
@@ -288,9 +289,11 @@ struct DavSub {
                                eig.get());
             APW_LAUNCH_CHECK();
         } else if (init) {
+            ews.single_max = ctx->eig_single_max;
             heevx_lowest(st(), ews, ctx->num_cu, nbase, nbx, hcc.get(), nband, eig.get(), vcc.get(), nbx, &ctx->prof);
         } else {
             zero_status();
+            ews.single_max = ctx->eig_single_max;
             hegvx_lowest(st(), gws(), ews, gen, ctx->num_cu, nbase, nbx, hcc.get(), scc.get(), nband, eig.get(),
                          vcc.get(), nbx, status.get(), &ctx->prof);
         }

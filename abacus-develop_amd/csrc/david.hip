@@ -212,6 +212,18 @@ struct David {
               double br, double* C, int ldc) {
         apw::gemm<double>(st(), gws(), ctx->num_cu, ta, tb, m, n, k, ar, 0.0, A, lda, Bm, ldb, br, 0.0, C, ldc, &ctx->prof);
     }
+    // the dim-long products of cal_grad and cal_elem: in the FP32 pre-phase on the FP32
+    // matrix cores (c128 operands rounded to FP32 as they are staged, FP64 sums), else FP64.
+    // Orthonormalisation, the restart vectors and refresh stay FP64.
+    bool sub_x32 = false;
+    void gemm_sub(char ta, char tb, int m, int n, int k, double ar, const double* A, int lda, const double* Bm,
+                  int ldb, double br, double* C, int ldc) {
+        if (sub_x32)
+            apw::gemm_x32(st(), gws(), ctx->num_cu, ta, tb, m, n, k, ar, 0.0, A, lda, Bm, ldb, br, 0.0, C, ldc,
+                          &ctx->prof);
+        else
+            gemm(ta, tb, m, n, k, ar, A, lda, Bm, ldb, br, C, ldc);
+    }
 
     // HX = H X[:, i0..i1]  (hpsi_func semantics: X = head of basis, HX = exact destination)
     void apply_h(double* HX, int i0, int i1, int p) {
@@ -325,13 +337,14 @@ struct David {
     }
 
     void solve_subspace(int nbase) {
+        ews.single_max = ctx->eig_single_max;
         heevx_lowest(st(), ews, ctx->num_cu, nbase, nbx, hcc.get(), nband, eig.get(), vcc.get(), nbx, &ctx->prof);
     }
 
     // hcc[nbase .. nbase+nc, 0 .. nbase+nc) = basis_new^H hpsi   (cal_elem, diago_david.cpp:626-717)
     void cal_elem(int nbase, int nc) {
-        gemm('C', 'N', nc, nbase + nc, dim, 1.0, Bd(basis.get(), nbase), dim, hp.get(), dim, 0.0,
-             reinterpret_cast<double*>(reinterpret_cast<double2*>(hcc.get()) + nbase), nbx);
+        gemm_sub('C', 'N', nc, nbase + nc, dim, 1.0, Bd(basis.get(), nbase), dim, hp.get(), dim, 0.0,
+                 reinterpret_cast<double*>(reinterpret_cast<double2*>(hcc.get()) + nbase), nbx);
     }
 
     // one diag_once (diago_david.cpp:151-341).  switch_thr > 0: FP32 pre-phase that stops once
@@ -364,10 +377,10 @@ struct David {
             gather_cols(st(), nbase, notconv,
                         reinterpret_cast<const double2*>(vcc.get()), nbx, unconv_d.get(), vcp);
             double* NB = Bd(basis.get(), nbase);
-            gemm('N', 'N', dim, notconv, nbase, 1.0, hp.get(), dim, vc.get(), nbase, 0.0, NB, dim);
+            gemm_sub('N', 'N', dim, notconv, nbase, 1.0, hp.get(), dim, vc.get(), nbase, 0.0, NB, dim);
             hipLaunchKernelGGL(k_scale_vc, dim3(gsz((size_t)nbase * notconv)), dim3(256), 0, st(), nbase, notconv, vcp,
                                eig.get(), unconv_d.get());
-            gemm('N', 'N', dim, notconv, nbase, 1.0, S(), dim, vc.get(), nbase, 1.0, NB, dim);
+            gemm_sub('N', 'N', dim, notconv, nbase, 1.0, S(), dim, vc.get(), nbase, 1.0, NB, dim);
             if (switch_thr > 0) {
                 hipLaunchKernelGGL(k_colres2, dim3(notconv), dim3(CR_NT), 0, st(), dim,
                                    reinterpret_cast<const double2*>(NB), dim, (const double2*)nullptr, 0,
@@ -491,7 +504,8 @@ int apw_david_diag(apw_david* dav, apw_ham* ham, apw_hpsi_cb hpsi_cb, apw_spsi_c
         apw::David& D = dav->d;
         APW_CHECK(ham || hpsi_cb, "apw_david_diag: need a Hamiltonian or an hpsi callback");
         APW_CHECK(ldpsi >= D.dim, "apw_david_diag: ldpsi < dim");
-        APW_CHECK(precision == APW_FP64 || precision == APW_FP32_MIXED, "apw_david_diag: bad precision");
+        APW_CHECK(precision == APW_FP64 || precision == APW_FP32_MIXED || precision == APW_FP32_MIXED_X32,
+                  "apw_david_diag: bad precision");
         APW_CHECK(david_maxiter >= 1, "apw_david_diag: david_maxiter < 1");
         if (ham) APW_CHECK(ham->h.npw == D.dim, "apw_david_diag: ham npw %d != dim %d", ham->h.npw, D.dim);
         apw::DeviceGuard g(D.ctx->device);
@@ -507,11 +521,14 @@ int apw_david_diag(apw_david* dav, apw_ham* ham, apw_hpsi_cb hpsi_cb, apw_spsi_c
         int total = 0, ntry = 0, notconv = 0, switch_iter = -1;
         int rc = APW_OK;
         try {
-            if (precision == APW_FP32_MIXED && ham) {
+            D.sub_x32 = false;
+            if ((precision == APW_FP32_MIXED || precision == APW_FP32_MIXED_X32) && ham) {
                 D.prec = APW_FP32_MIXED;
+                D.sub_x32 = precision == APW_FP32_MIXED_X32;
                 total += D.diag_once(ldpsi, psi, eig, david_diag_thr, david_maxiter, notconv, 1.0e-5);
                 switch_iter = total;
             }
+            D.sub_x32 = false;
             D.prec = APW_FP64;
             do {
                 total += D.diag_once(ldpsi, psi, eig, david_diag_thr, david_maxiter, notconv, 0.0);

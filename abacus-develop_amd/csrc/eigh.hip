@@ -5,8 +5,9 @@
 // (rocm/dngvd_op.hip.cu:146-170).  Same algorithm family as zheevx, all on the device:
 //   1. Householder reduction of the lower triangle to a real symmetric tridiagonal T
 //      (zhetrd 'L' form, unblocked) in one cooperative kernel: the trailing matrix is
-//      partitioned by columns over the workgroups and held in their LDS (global memory
-//      for n too large for that), one grid barrier per column;
+//      partitioned by columns over the workgroups and held in their registers and LDS
+//      (global memory for n too large for that), one exchange through a ring per column
+//      (or in one workgroup with an ordinary launch, for many small concurrent solves);
 //   2. bisection with Sturm counts for eigenvalues 0..nband-1 of T (dstebz form);
 //   3. inverse iteration with partial-pivoting tridiagonal LU and Gram-Schmidt inside
 //      clusters of close eigenvalues (dstein form);
@@ -14,9 +15,9 @@
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
-
 #include <cstdlib>
 #include <mutex>
+#include <vector>
 
 #include "apw_internal.hpp"
 
@@ -134,12 +135,14 @@ __device__ void make_reflector(const double2* xs, int mx, double2* vo, double2& 
     __syncthreads();
 }
 
-// Cross-workgroup exchange of the LDS-resident reduction: a ring with one slot per
-// column step, pre-filled with a NaN bit pattern no arithmetic produces.  Producers store
-// each value with device-scope stores (they bypass the non-coherent caches); consumers
-// poll the slots they need with device-scope loads until no word holds the pattern.
-// Every value validates itself, so there is no flag, no fence and no grid barrier: one
-// memory round trip per column.
+// Cross-workgroup exchange of the reduction: a ring with one slot per column step,
+// pre-filled with a NaN bit pattern no arithmetic produces.  Producers store each value with
+// device-scope stores (they bypass the non-coherent per-XCD caches); consumers poll the
+// slots they need with device-scope loads until no word holds the pattern.  Every value
+// validates itself, so there is no flag, no fence and no grid barrier: one memory round trip
+// per column.  The check assumes a 16-byte load returns each 8-byte half whole (one
+// request); a torn half could only pass as a wrong value, which k_bisect's finiteness test
+// catches when it is not finite.
 constexpr unsigned long long RING_EMPTY = 0x7ff4dead5eed0001ull;
 
 __global__ void k_fill_ring(double2* ring, size_t count) {
@@ -148,13 +151,8 @@ __global__ void k_fill_ring(double2* ring, size_t count) {
         ring[i] = make_double2(e, e);
 }
 
-__device__ inline void ring_put(double2* slot, double2 v) {
-    __hip_atomic_store(&slot->x, v.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(&slot->y, v.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
 // 16-byte forms: one vector buffer load / store per value with the device-scope cache
-// policy (sc1, the bits the 8-byte device-scope atomics above carry).  Each 8-byte half
+// policy (sc1, the bits of the 8-byte device-scope atomics).  Each 8-byte half
 // still validates itself, so a load that sees one half written and the other not simply
 // polls again.  r spans one step's slot (2n values); i is the value's index in it.
 typedef unsigned ring_u4 __attribute__((ext_vector_type(4)));
@@ -171,44 +169,50 @@ __device__ inline bool ring_get16(__amdgpu_buffer_rsrc_t r, int i, double2& v) {
            (unsigned long long)__double_as_longlong(v.y) != RING_EMPTY;
 }
 
-// Householder tridiagonalisation (lower) with the trailing matrix resident in LDS.
-// Column c belongs to workgroup c % nwg (slot c / nwg) for the whole reduction and lives
-// in that workgroup's LDS, so the hemv reads and rank-2 writes never touch global memory.
-// Per column, what crosses workgroups is p = A v on each workgroup's columns and the one
-// column that yields the next reflector (its owner publishes it un-updated; every
-// workgroup applies the rank-2 correction itself and forms the reflector redundantly).
-// Both go through the ring: slot k * 2n + i holds p_i of step k, k * 2n + n + i the
-// published column.  abort_flag: raised when a poll exceeds its spin bound.
-// LDS: ncol x n columns, v, y, c (n each), ncol spare, reduction scratch.
+// Householder tridiagonalisation (lower, zhetrd 'L' steps).  Column c belongs to workgroup
+// c % nwg for the whole reduction, in its LDS (GCOL = false) or in A itself (GCOL = true: the
+// copy of hcc, updated in place).  Per column step k, what crosses workgroups is
+// p_k = A_k^H v_k on each workgroup's columns and column k+1 of A_k; both go through the
+// ring (slot k * 2n + i holds p_i, k * 2n + n + i the column).  Every workgroup applies the
+// rank-2 correction of step k to that column and forms the next reflector v_{k+1} itself
+// (redundantly) before it touches its own columns, so one pass over each own column both
+// applies the rank-2 update of step k and accumulates p_{k+1} = A_{k+1}^H v_{k+1}: a trailing
+// entry is read and written once per step.  The owner of column k+2 publishes it updated.
+// abort_flag: raised when a poll exceeds its spin bound.
+// LDS: [ncol x n columns], v_k and v_{k+1}, y, the gathered column (n each), scratch.
 template <bool GCOL>
-__global__ __launch_bounds__(NT) void k_tridiag_lds(double2* __restrict__ A, int n, int ncol,
-                                                    double2* __restrict__ V, double2* __restrict__ tau,
-                                                    double* __restrict__ d, double* __restrict__ e,
-                                                    double2* __restrict__ ring, unsigned* abort_flag) {
+__global__ __launch_bounds__(NT) void k_tridiag_fused(double2* __restrict__ A, int n, int ncol,
+                                                      double2* __restrict__ V, double2* __restrict__ tau,
+                                                      double* __restrict__ d, double* __restrict__ e,
+                                                      double2* __restrict__ ring, unsigned* abort_flag) {
     extern __shared__ __align__(16) unsigned char smem[];
-    // own column slot lc (c = blockIdx.x + lc * nwg): LDS copy, or (GCOL) the column of A
-    // itself, which no other workgroup writes after the start
     double2* Al = reinterpret_cast<double2*>(smem);
-    double2* v = Al + (GCOL ? 0 : (size_t)ncol * n);
-    double2* y = v + n;
+    double2* vb = Al + (GCOL ? 0 : (size_t)ncol * n);
+    double2* y = vb + 2 * n;
     double2* cn = y + n;
-    double* red = reinterpret_cast<double*>(cn + n + ncol);
-    const unsigned nwg = gridDim.x;
+    double* red = reinterpret_cast<double*>(cn + n);
+    const int nwg = (int)gridDim.x, bid = (int)blockIdx.x;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    constexpr int NW = NT / 64;
     auto colp = [&](int lc) -> double2* {
-        return GCOL ? A + (size_t)(blockIdx.x + lc * nwg) * n : Al + (size_t)lc * n;
+        return GCOL ? A + (size_t)(bid + lc * nwg) * n : Al + (size_t)lc * n;
     };
+    // first own column slot with c >= x
+    auto first_slot = [&](int x) { return x > bid ? (x - bid + nwg - 1) / nwg : 0; };
     if (!GCOL) {
         for (int lc = 0; lc < ncol; ++lc) {
-            const int c = blockIdx.x + lc * nwg;
+            const int c = bid + lc * nwg;
             if (c >= n) break;
             for (int i = threadIdx.x; i < n; i += NT) Al[(size_t)lc * n + i] = A[(size_t)c * n + i];
         }
+        __syncthreads();
     }
+    double2* v = vb;        // v_k: rows k+1 .. n-1 (v[0] = 1 at row k+1)
+    double2* vn = vb + n;   // v_{k+1}
     double2 t;
     double beta;
     make_reflector(A + 1, n - 1, v, t, beta, red);
-    if (blockIdx.x == 0) {
+    if (bid == 0) {
         for (int i = threadIdx.x; i < n - 1; i += NT) V[1 + i] = v[i];
         if (threadIdx.x == 0) {
             tau[0] = t;
@@ -216,15 +220,14 @@ __global__ __launch_bounds__(NT) void k_tridiag_lds(double2* __restrict__ A, int
             e[0] = beta;
         }
     }
-    for (int k = 0; k < n - 1; ++k) {
-        const int m = n - k - 1;   // trailing rows/cols k+1 .. n-1
-        double2* slot = ring + (size_t)k * 2 * n;
-        // ---- p = A_trail^H v on own columns c >= k+1 (one wave per column) ----
-        const int lc0 = (k + 1 - (int)blockIdx.x + (int)nwg - 1) / (int)nwg;   // first own slot with c >= k+1
-        for (int lc = lc0 + wave; lc < ncol; lc += NT / 64) {
-            const int c = blockIdx.x + lc * nwg;
+    // p_0 = A^H v_0 on own columns c >= 1, and column 1 (as loaded), into slot 0
+    {
+        const __amdgpu_buffer_rsrc_t rs = ring_rsrc(ring, 2 * n);
+        const int m = n - 1;
+        for (int lc = first_slot(1) + wave; lc < ncol; lc += NW) {
+            const int c = bid + lc * nwg;
             if (c >= n) break;
-            const double2* col = colp(lc) + (k + 1);
+            const double2* col = colp(lc) + 1;
             double pr = 0, pi = 0;
             for (int i = lane; i < m; i += 64) {
                 const double2 q = cconjmul(col[i], v[i]);
@@ -233,18 +236,17 @@ __global__ __launch_bounds__(NT) void k_tridiag_lds(double2* __restrict__ A, int
             }
             pr = wave_sum(pr);
             pi = wave_sum(pi);
-            if (lane == 0) ring_put(slot + (c - (k + 1)), make_double2(pr, pi));
+            if (lane == 0) ring_put16(rs, c - 1, make_double2(pr, pi));
+            if (c == 1)
+                for (int i = lane; i < m; i += 64) ring_put16(rs, n + i, col[i]);
         }
-        if ((k + 1) % (int)nwg == (int)blockIdx.x) {   // owner of column k+1 publishes it
-            const double2* col = colp((k + 1) / nwg) + (k + 1);
-            for (int i = threadIdx.x; i < m; i += NT) ring_put(slot + n + i, col[i]);
-        }
-        // ---- gather p -> y and the published column -> cn ----
-        // Each wave polls on its own until its values are in, skipping those it already has
-        // (bit q of got for a thread's q-th value; 2m / NT <= 16 for n <= 2048); one barrier
-        // then publishes them to the workgroup.
+    }
+    for (int k = 0; k < n - 1; ++k) {
+        const int m = n - k - 1;   // trailing rows/cols k+1 .. n-1
+        // ---- gather p_k -> y and column k+1 of A_k -> cn (each wave polls on its own,
+        // skipping the values it already has; one barrier publishes them) ----
         {
-            const __amdgpu_buffer_rsrc_t rs = ring_rsrc(slot, 2 * n);
+            const __amdgpu_buffer_rsrc_t rs = ring_rsrc(ring + (size_t)k * 2 * n, 2 * n);
             unsigned spins = 0, got = 0;
             while (true) {
                 bool mine = true;
@@ -262,11 +264,9 @@ __global__ __launch_bounds__(NT) void k_tridiag_lds(double2* __restrict__ A, int
                     }
                 }
                 if (__all(mine)) break;
-                // the abort word and the spin bound every 32 failed polls of this wave: its
-                // load is a memory round trip that would otherwise lengthen every wait
                 if ((++spins & 31u) == 0) {
                     int st = 0;
-                    if ((threadIdx.x & 63) == 0) {
+                    if (lane == 0) {
                         st = __hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u;
                         if (!st && spins > SPIN_LIMIT) {
                             atomicExch(abort_flag, 1u);
@@ -279,7 +279,7 @@ __global__ __launch_bounds__(NT) void k_tridiag_lds(double2* __restrict__ A, int
             }
             __syncthreads();
         }
-        // ---- y = tau p - (|tau|^2 mu / 2) v,  mu = v^H p;  cn -= rank-2 correction ----
+        // ---- y = tau p - (|tau|^2 mu / 2) v,  mu = v^H p ----
         double mr = 0;
         for (int i = threadIdx.x; i < m; i += NT) mr += cconjmul(v[i], y[i]).x;
         const double mu = block_sum2(mr, 0.0, red).x;
@@ -289,6 +289,7 @@ __global__ __launch_bounds__(NT) void k_tridiag_lds(double2* __restrict__ A, int
             y[i] = make_double2(tp.x - c2 * v[i].x, tp.y - c2 * v[i].y);
         }
         __syncthreads();
+        // ---- column k+1 of A_{k+1} ----
         {
             const double2 v0 = v[0], y0 = y[0];
             for (int i = threadIdx.x; i < m; i += NT) {
@@ -298,15 +299,30 @@ __global__ __launch_bounds__(NT) void k_tridiag_lds(double2* __restrict__ A, int
                 cn[i].y -= a1.y + a2.y;
             }
         }
-        // ---- rank-2 update of own columns c >= k+2, rows k+2 .. n-1 ----
-        {
-            const int lc1 = (k + 2 - (int)blockIdx.x + (int)nwg - 1) / (int)nwg;
-            for (int lc = lc1 + wave; lc < ncol; lc += NT / 64) {
-                const int c = blockIdx.x + lc * nwg;
+        __syncthreads();
+        const double dk1 = cn[0].x;
+        if (m >= 2) {
+            // ---- v_{k+1} from rows k+2 .. n-1 of that column (every workgroup) ----
+            double2 tn;
+            double bn;
+            make_reflector(cn + 1, m - 1, vn, tn, bn, red);
+            if (bid == 0) {
+                for (int i = threadIdx.x; i < m - 1; i += NT) V[(size_t)(k + 1) * n + (k + 2) + i] = vn[i];
+                if (threadIdx.x == 0) {
+                    tau[k + 1] = tn;
+                    d[k + 1] = dk1;
+                    e[k + 1] = bn;
+                }
+            }
+            // ---- rank-2 update of own columns c >= k+2 (rows k+2 ..) and p_{k+1} ----
+            const __amdgpu_buffer_rsrc_t rs1 = ring_rsrc(ring + (size_t)(k + 1) * 2 * n, 2 * n);
+            for (int lc = first_slot(k + 2) + wave; lc < ncol; lc += NW) {
+                const int c = bid + lc * nwg;
                 if (c >= n) break;
-                double2* col = colp(lc) + (k + 1);
+                double2* col = colp(lc) + (k + 1);   // relative row i = row k+1+i
                 const int jj = c - (k + 1);
                 const double2 vj = v[jj], yj = y[jj];
+                double pr = 0, pi = 0;
                 for (int i = 1 + lane; i < m; i += 64) {
                     const double2 a1 = cconjmul(vj, y[i]);
                     const double2 a2 = cconjmul(yj, v[i]);
@@ -314,23 +330,22 @@ __global__ __launch_bounds__(NT) void k_tridiag_lds(double2* __restrict__ A, int
                     a.x -= a1.x + a2.x;
                     a.y -= a1.y + a2.y;
                     col[i] = a;
+                    const double2 q = cconjmul(a, vn[i - 1]);
+                    pr += q.x;
+                    pi += q.y;
                 }
+                pr = wave_sum(pr);
+                pi = wave_sum(pi);
+                if (lane == 0) ring_put16(rs1, c - (k + 2), make_double2(pr, pi));
+                if (c == k + 2)   // owner: column k+2 of A_{k+1}, rows k+2 .. n-1 (its own writes)
+                    for (int i = 1 + lane; i < m; i += 64) ring_put16(rs1, n + (i - 1), col[i]);
             }
-        }
-        __syncthreads();
-        // ---- next reflector from the updated column (every workgroup, redundantly) ----
-        if (m >= 2) {
-            make_reflector(cn + 1, m - 1, v, t, beta, red);
-            if (blockIdx.x == 0) {
-                for (int i = threadIdx.x; i < m - 1; i += NT) V[(size_t)(k + 1) * n + (k + 2) + i] = v[i];
-                if (threadIdx.x == 0) {
-                    tau[k + 1] = t;
-                    d[k + 1] = cn[0].x;
-                    e[k + 1] = beta;
-                }
-            }
-        } else if (blockIdx.x == 0 && threadIdx.x == 0) {
-            d[k + 1] = cn[0].x;
+            t = tn;
+            double2* sw = v;
+            v = vn;
+            vn = sw;
+        } else if (bid == 0 && threadIdx.x == 0) {
+            d[k + 1] = dk1;
         }
         __syncthreads();
     }
@@ -347,7 +362,7 @@ __device__ inline double2 row_bcast(const double2 (&x)[R], int i) {
     return make_double2(lane_bcast(t.x, i & 63), lane_bcast(t.y, i & 63));
 }
 // Householder tridiagonalisation (lower), LDS-resident columns, vectors in registers.
-// Same ownership and exchange as k_tridiag_lds; what changes is where the per-column work
+// Same ownership and exchange as k_tridiag_fused; what changes is where the per-column work
 // happens.  Every wave holds the full vectors v_k, y_k and the next reflector v_{k+1} in
 // registers (lane l: rows l + 64 r, r < R), so y, the rank-2-corrected column and the next
 // reflector are formed with wave reductions only, and one pass over each own column both
@@ -1202,18 +1217,37 @@ void heevx_lowest(hipStream_t s, EigWorkspace& ws, int num_cu, int n, int ldh, c
         // (8 n^2); the per-column hemv and rank-2 update work on LDS-resident columns
         ProfScope ps(prof, s, "eig_tridiag", 24.0 * n * (double)n, 16.0 / 3.0 * n * (double)n * n);
         // LDS-resident form: the fewest workgroups whose column share fits the LDS budget
-        const size_t fixed = (size_t)3 * n * sizeof(double2) + 64 * sizeof(double);
+        // fused form: v_k, v_{k+1}, y and the gathered column (n each) + scratch, and the
+        // own columns when they live in LDS
+        const size_t fixed = (size_t)4 * n * sizeof(double2) + 64 * sizeof(double);
         int nwg = -1, ncol = 0;
         for (int w = std::max(1, std::min(n / 8, 16)); w <= std::min(num_cu, 256); ++w) {
             const int nc = (n + w - 1) / w;
-            if (fixed + (size_t)nc * (n + 1) * sizeof(double2) <= TRIDIAG_LDS_BYTES) {
+            if (fixed + (size_t)nc * n * sizeof(double2) <= TRIDIAG_LDS_BYTES) {
                 nwg = w;
                 ncol = nc;
                 break;
             }
         }
         const int R = (n + 63) / 64;
-        if (R <= 16) {
+        if (n <= ws.single_max) {
+            // one workgroup owns every column (kept in the copy A) and exchanges with itself:
+            // an ordinary launch, not ordered against other streams' reductions, so many small
+            // solves (batched k-points or cells) reduce at the same time
+            const int ncol1 = n;
+            const size_t lds = fixed;
+            const size_t nring = (size_t)2 * n * (n - 1);
+            ws.ring.reserve(2 * nring);
+            double2* ring = reinterpret_cast<double2*>(ws.ring.get());
+            hipLaunchKernelGGL(k_fill_ring, dim3((unsigned)std::min<size_t>(1024, (nring + 255) / 256)), dim3(256), 0, s,
+                               ring, nring);
+            APW_LAUNCH_CHECK();
+            APW_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_tridiag_fused<true>),
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+            hipLaunchKernelGGL(k_tridiag_fused<true>, dim3(1), dim3(NT), lds, s, A, n, ncol1, Vp, taup, dp, ep, ring,
+                               barp + 1);
+            APW_LAUNCH_CHECK();
+        } else if (R <= 16) {
             // register form: LDS = own columns + two gather buffers of 2n
             const size_t fixed_r = (size_t)4 * n * sizeof(double2);
             int w = std::max(1, std::min(n / 8, 16)), nc = 0;
@@ -1222,27 +1256,15 @@ void heevx_lowest(hipStream_t s, EigWorkspace& ws, int num_cu, int n, int ldh, c
                 if (fixed_r + (size_t)nc * n * sizeof(double2) <= TRIDIAG_LDS_BYTES) break;
             }
             APW_CHECK(w <= std::min(num_cu, 256), "heevx: no workgroup count fits n=%d", n);
+            const int w_min = w;
             // about four columns a workgroup (at most one a wave in each step's update): with
             // the per-wave polling of the ring, spreading the columns beats fewer exchange
             // participants up to ~160 of them (measured n = 160..640: 5-17 % faster than the
             // LDS minimum; 192 and 256 workgroups lost again at n = 640)
             w = std::max(w, std::min(std::min(num_cu, 160), n / 4));
-            nc = (n + w - 1) / w;
-            // tuning override: more workgroups than the LDS minimum (fewer columns each)
-            if (const char* ev = std::getenv("APW_TRIDIAG_NWG")) {
-                const int want = std::atoi(ev);
-                if (want > w) {
-                    w = std::min(want, std::min(num_cu, 256));
-                    nc = (n + w - 1) / w;
-                }
-            }
-            const size_t lds = fixed_r + (size_t)nc * n * sizeof(double2);
             const size_t nring = (size_t)2 * n * n;
             ws.ring.reserve(2 * nring);
             double2* ring = reinterpret_cast<double2*>(ws.ring.get());
-            hipLaunchKernelGGL(k_fill_ring, dim3((unsigned)std::min<size_t>(1024, (nring + 255) / 256)), dim3(256), 0, s,
-                               ring, nring);
-            APW_LAUNCH_CHECK();
             const void* fn = nullptr;
             // threads per workgroup: 512 for 6 <= R <= 12 (measured faster at n = 480, 640: each
             // wave updates fewer columns); 256 otherwise (small n: the extra waves cost more in
@@ -1260,12 +1282,48 @@ void heevx_lowest(hipStream_t s, EigWorkspace& ws, int num_cu, int n, int ldh, c
                 case 12: fn = reinterpret_cast<const void*>(k_tridiag_reg<12, 512>); ntr = 512; break;
                 default: fn = reinterpret_cast<const void*>(k_tridiag_reg<16, 256>); ntr = 256; break;
             }
-            APW_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-            const double2* Ac = A;
-            unsigned* abortp = barp + 1;
-            int ncol_r = nc;
-            void* args[] = {&Ac, &n, &ncol_r, &Vp, &taup, &dp, &ep, &ring, &abortp};
-            coop_launch(fn, dim3(w), dim3(ntr), args, (unsigned)lds, s);
+            // one reduction with wg workgroups (A is only read: it can run again)
+            auto run_reg = [&](int wg) {
+                int ncol_r = (n + wg - 1) / wg;
+                const size_t lds = fixed_r + (size_t)ncol_r * n * sizeof(double2);
+                hipLaunchKernelGGL(k_fill_ring, dim3((unsigned)std::min<size_t>(1024, (nring + 255) / 256)), dim3(256),
+                                   0, s, ring, nring);
+                APW_LAUNCH_CHECK();
+                APW_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+                const double2* Ac = A;
+                unsigned* abortp = barp + 1;
+                void* args[] = {&Ac, &n, &ncol_r, &Vp, &taup, &dp, &ep, &ring, &abortp};
+                coop_launch(fn, dim3(wg), dim3(ntr), args, (unsigned)lds, s);
+            };
+            // autotuned workgroup count
+            const int tmode = tune_mode();
+            if (tmode > 0) {
+                char key[96];
+                std::snprintf(key, sizeof(key), "tridiag reg n=%d cu=%d", (n + 31) / 32 * 32, num_cu);
+                int a = 0, b = 0;
+                const int wmax = std::min(num_cu, 256);
+                if (tune_find(key, a, b) && a >= w_min && a <= wmax) {
+                    w = a;
+                } else if (tmode == 2) {
+                    std::vector<int> cand;
+                    for (int c : {w_min, n / 8, n / 6, n / 4, n / 3, n / 2, 128, 160, 192, 256}) {
+                        c = std::max(w_min, std::min(c, wmax));
+                        if (std::find(cand.begin(), cand.end(), c) == cand.end()) cand.push_back(c);
+                    }
+                    double best = 1e300;
+                    for (int c : cand) {
+                        const double us = time_on_stream(s, 2, [&] { run_reg(c); });
+                        if (us < best) {
+                            best = us;
+                            w = c;
+                        }
+                    }
+                    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(256), 0, s, ws.bar.get(), 64 + 256);
+                    APW_LAUNCH_CHECK();
+                    tune_store(key, w, 0, best);
+                }
+            }
+            run_reg(w);
         } else {
             // columns in LDS when the share fits, else in A itself (one column per
             // workgroup slot, L2/Infinity-Cache resident); the ring as above
@@ -1274,16 +1332,15 @@ void heevx_lowest(hipStream_t s, EigWorkspace& ws, int num_cu, int n, int ldh, c
                 nwg = std::min(num_cu, 256);
                 ncol = (n + nwg - 1) / nwg;
             }
-            const size_t lds = gcol ? fixed + (size_t)ncol * sizeof(double2)
-                                    : fixed + (size_t)ncol * (n + 1) * sizeof(double2);
+            const size_t lds = gcol ? fixed : fixed + (size_t)ncol * n * sizeof(double2);
             const size_t nring = (size_t)2 * n * (n - 1);
             ws.ring.reserve(2 * nring);
             double2* ring = reinterpret_cast<double2*>(ws.ring.get());
             hipLaunchKernelGGL(k_fill_ring, dim3((unsigned)std::min<size_t>(1024, (nring + 255) / 256)), dim3(256), 0, s,
                                ring, nring);
             APW_LAUNCH_CHECK();
-            const void* fn = gcol ? reinterpret_cast<const void*>(k_tridiag_lds<true>)
-                                  : reinterpret_cast<const void*>(k_tridiag_lds<false>);
+            const void* fn = gcol ? reinterpret_cast<const void*>(k_tridiag_fused<true>)
+                                  : reinterpret_cast<const void*>(k_tridiag_fused<false>);
             APW_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
             unsigned* abortp = barp + 1;
             void* args[] = {&A, &n, &ncol, &Vp, &taup, &dp, &ep, &ring, &abortp};
@@ -1369,6 +1426,7 @@ extern "C" int apw_dnevx_op(apw_ctx* ctx, int nstart, int ldh, const double* hcc
         APW_CHECK(ctx && hcc && eigenvalue && vcc, "apw_dnevx_op: null argument");
         apw::DeviceGuard g(ctx->device);
         apw::EigWorkspace& ws = ctx->ews;
+        ws.single_max = ctx->eig_single_max;
         apw::heevx_lowest(ctx->stream, ws, ctx->num_cu, nstart, ldh, hcc, nbands, eigenvalue, vcc, ldh, &ctx->prof);
         APW_HIP(hipStreamSynchronize(ctx->stream));
         unsigned flag[2] = {0, 0};

@@ -16,10 +16,12 @@
 // column l % 16; drow is (l / 16) + 4 r for the f64 instruction and 4 (l / 16) + r for the
 // f32 one (checked on the device by the layout probe below and its unit test).
 #include <cstdio>
+#include <algorithm>
 #include <cstdlib>
 #include <mutex>
 #include <type_traits>
 #include <utility>
+#include <vector>
 
 #include "apw_internal.hpp"
 
@@ -90,7 +92,11 @@ __global__ __launch_bounds__(NT) void zgemm_kernel(int M, int N, int K, int kchu
     // workgroup L runs on XCD L % 8, so each XCD takes whole m-tiles and walks their n-tiles
     // back to back; the A tile they share stays in that XCD's L2.  The grid is padded to a
     // multiple of 8 m-tiles; padding workgroups leave before any barrier.
-    int bm = blockIdx.x, bn = blockIdx.y;
+    // MAP 3 (XCD-aware split-K, 1-D grid padded to a multiple of 8): the workgroups of one
+    // k split run on one or two XCDs, so the A and B slices that all tiles of the split read
+    // at the same k come from that XCD's L2 instead of once per tile from HBM.  XCD x takes
+    // the x-th eighth of the (split, tile) sequence, tiles of a split consecutive.
+    int bm = blockIdx.x, bn = blockIdx.y, sp = blockIdx.z;
     if (MAP == 1) {   // blockIdx.x walks n
         bm = blockIdx.y;
         bn = blockIdx.x;
@@ -100,9 +106,18 @@ __global__ __launch_bounds__(NT) void zgemm_kernel(int M, int N, int K, int kchu
         bm = (L % 8) + 8 * (L / (8 * ntn));
         bn = (L / 8) % ntn;
         if (bm >= tmt) return;
+    } else if (MAP == 3) {
+        const int tmt = (M + BM - 1) / BM, tiles = tmt * ((N + BN - 1) / BN);
+        const int per = (int)gridDim.x / 8;
+        const int g = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+        sp = g / tiles;
+        if (sp * kchunk >= K) return;   // padding workgroups: no barrier has been reached
+        const int t = g - sp * tiles;
+        bm = t % tmt;
+        bn = t / tmt;
     }
     const int m0 = bm * BM, n0 = bn * BN;
-    const int kbeg = blockIdx.z * kchunk;
+    const int kbeg = sp * kchunk;
     const int kend = min(K, kbeg + kchunk);
 
     acc_t acc_re[TN][TM], acc_im[TN][TM];
@@ -267,7 +282,7 @@ __global__ __launch_bounds__(NT) void zgemm_kernel(int M, int N, int K, int kchu
                     c_t v;
                     v.x = (S)xr;
                     v.y = (S)xi;
-                    part[(size_t)blockIdx.z * M * N + gm + (size_t)gn * M] = v;
+                    part[(size_t)sp * M * N + gm + (size_t)gn * M] = v;
                 } else {
                     double cr = ar * xr - ai * xi;
                     double ci = ar * xi + ai * xr;
@@ -352,8 +367,9 @@ void launch(ProfScope& ps, hipStream_t s, int splits, int M, int N, int K, int k
             const S* B, int ldb, S* C, int ldc, double ar, double ai, double br, double bi, S* part) {
     using c_t = typename Mfma<S>::c_t;
     const int tm = (M + 32 * TM - 1) / (32 * TM), tn = (N + 32 * TN - 1) / (32 * TN);
-    const dim3 grid = MAP == 2 ? dim3((unsigned)(((tm + 7) / 8) * 8 * tn), 1, splits)
-                               : (MAP == 1 ? dim3(tn, tm, splits) : dim3(tm, tn, splits));
+    const dim3 grid = MAP == 3   ? dim3((unsigned)((tm * tn * splits + 7) / 8 * 8), 1, 1)
+                      : MAP == 2 ? dim3((unsigned)(((tm + 7) / 8) * 8 * tn), 1, splits)
+                                 : (MAP == 1 ? dim3(tn, tm, splits) : dim3(tm, tn, splits));
     APW_PLAUNCH(ps, (zgemm_kernel<R, S, TA, TB, TM, TN, BK, MAP>), grid, dim3(NT), 0, s, M, N, K, kchunk,
                        reinterpret_cast<const c_t*>(A), lda, reinterpret_cast<const c_t*>(B), ldb,
                        reinterpret_cast<c_t*>(C), ldc, ar, ai, br, bi, reinterpret_cast<c_t*>(part));
@@ -381,106 +397,159 @@ void gemm_impl(hipStream_t s, GemmWorkspace& ws, int num_cu, char ta, char tb, i
         APW_LAUNCH_CHECK();
         return;
     }
-    // Tile choice (measured on the Davidson shapes).  Inner products (k >> m n, split over k
-    // with a fixed-order reduction): 64 x 32 or 64 x 64 tiles (below).  Updates (no split):
-    // 128 x 64 or 128 x 32 for tall C (m >> n; the
-    // 32-wide tile unless n is a multiple of 64: less waste in the last tile column, more
-    // workgroups) in the XCD-aware order, 64 x 128 for wide, else 64 x 64.
-    // Split count: every workgroup of a split launch is resident at once (floor of the
-    // resident slots over the tiles; a launch with a partial last round of workgroups ran
-    // up to 1.5x longer), with at least 64 of k per split.
-    static const int occ_env = [] {
-        const char* e = std::getenv("APW_GEMM_WG_PER_CU");   // tuning override
-        return e ? std::atoi(e) : 0;
-    }();
+    // Plans: a tile variant and a split rule.  Variants (TM x TN MFMA sub-tiles per wave, map):
+    // 0: 64 x 64, m-major; 1: 64 x 32, m-major; 2: 128 x 64, n-major; 3: 128 x 64 and 4: 128 x 32,
+    // XCD-aware (each XCD takes whole m-tiles); 5: 64 x 128 with k step 16; 6: 64 x 64 and
+    // 7: 64 x 32, XCD-aware split-K (the tiles of one k split share one XCD's L2).
+    // occ > 0 splits k so that every workgroup of the launch is resident at once (occ
+    // workgroups per CU: floor of the resident slots over the tiles, at least 64 of k per
+    // split; a launch with a partial last round of workgroups ran up to 1.5x longer); occ = 0
+    // does not split.
+    static const int VBM[8] = {64, 64, 128, 128, 128, 64, 64, 64}, VBN[8] = {64, 32, 64, 64, 32, 128, 64, 32};
+    static const int VOCC[8] = {3, 6, 3, 3, 3, 2, 3, 6};   // resident workgroups per CU
     const int cus = std::max(num_cu, 64);
     auto tiles_of = [&](int bm, int bn) { return (long)((m + bm - 1) / bm) * ((n + bn - 1) / bn); };
-    auto split_count = [&](int bm, int bn, int occ) {
-        const long tiles = tiles_of(bm, bn), slots = (long)(occ_env > 0 ? occ_env : occ) * cus;
+    auto splits_of = [&](int v, int occ) {
+        if (occ <= 0) return 1;
+        const long tiles = tiles_of(VBM[v], VBN[v]), slots = (long)occ * cus;
         int sp = tiles < slots ? (int)(slots / tiles) : 1;
-        sp = std::min(sp, std::max(1, k / 64));
-        return std::max(1, std::min(sp, 256));
+        sp = std::max(1, std::min(std::min(sp, std::max(1, k / 64)), 256));
+        int kc = (k + sp - 1) / sp;
+        kc = (kc + BKMAX - 1) / BKMAX * BKMAX;
+        return (k + kc - 1) / kc;
     };
     const bool tall = m >= 4 * n && m >= 256, wide = n >= 4 * m && n >= 256;
     const bool inner = tiles_of(64, 64) < 2 * cus && k / 64 > 1;
-    int variant;   // 0: 2x2 map0, 1: 2x1 map0, 2: 4x2 map1, 3: 4x2 map2, 4: 4x1 map2, 5: 2x4 bk16 map0
-    // inner products: 64 x 32 tiles when they cover n with fewer columns than 64 x 64 ones
-    // (n <= 32, 65..96), else 64 x 64 (measured; 128 x 64 lost on every si64 shape)
+    // Fixed rule (measured on the Davidson shapes): inner products (k >> m n) split over k
+    // with 64 x 32 tiles when they cover n with fewer columns than 64 x 64 ones (n <= 32,
+    // 65..96), else 64 x 64; tall updates 128 x 64 (n a multiple of 64) or 128 x 32 in the
+    // XCD-aware order; wide 64 x 128; else 64 x 64.  FP32-MFMA on FP64 storage cannot hold
+    // the 128 x 64 tile's two accumulator sets: 128 x 32 instead.
+    int variant, occ = 0;
     if (inner) variant = (n <= 32 || (n > 64 && n <= 96)) ? 1 : 0;
     else if (tall) variant = n % 64 == 0 ? 3 : 4;
     else variant = wide ? 5 : 0;
-    static const int variant_env = [] {
-        const char* e = std::getenv("APW_GEMM_VARIANT");   // tuning override of the tile choice
-        return e ? std::atoi(e) : -1;
-    }();
-    if (variant_env >= 0 && variant_env <= 5) variant = variant_env;
-    // FP32-MFMA on FP64 storage: the 128 x 64 tiles would hold 2 x 128 accumulator registers
-    // (FP32 and FP64 sums); the 128 x 32 tile in the same XCD-aware order takes their place
     if (X32 && (variant == 2 || variant == 3)) variant = 4;
-    static const int VBM[6] = {64, 64, 128, 128, 128, 64}, VBN[6] = {64, 32, 64, 64, 32, 128};
-    static const int VOCC[6] = {3, 6, 3, 3, 3, 2};   // resident workgroups per CU (measured best)
-    int splits = inner ? split_count(VBM[variant], VBN[variant], VOCC[variant]) : 1;
-    int kchunk = (k + splits - 1) / splits;
-    kchunk = (kchunk + BKMAX - 1) / BKMAX * BKMAX;
-    splits = (k + kchunk - 1) / kchunk;
-    S* part = nullptr;
-    if (splits > 1) {
-        const size_t need = (size_t)splits * m * n * 2 * sizeof(S) / sizeof(double) + 2;
-        if (ws.ext) {
-            APW_CHECK(need <= ws.ext_n,
-                      "gemm: %c%c m=%d n=%d k=%d splits over k into %d partial products of %zu bytes, more than "
-                      "the %zu-byte fixed workspace", ta, tb, m, n, k, splits, need * sizeof(double),
-                      ws.ext_n * sizeof(double));
-            part = reinterpret_cast<S*>(ws.ext);
-        } else {
-            ws.part.reserve(need);
-            part = reinterpret_cast<S*>(ws.part.get());
+    if (inner) occ = VOCC[variant];
+    const char* prec_tag = X32 ? "x32" : (std::is_same<R, double>::value ? "f64" : "f32");
+    auto need_of = [&](int v, int oc) {
+        const int sp = splits_of(v, oc);
+        return sp > 1 ? (size_t)sp * m * n * 2 * sizeof(S) / sizeof(double) + 2 : (size_t)0;
+    };
+    // runs a plan: C = alpha op(A) op(B) + beta C through the split partials when it splits
+    auto run = [&](int v, int oc, S* Cp, int ldcp, double b_re, double b_im, Prof* pf) {
+        const int splits = splits_of(v, oc);
+        int kchunk = (k + splits - 1) / splits;
+        kchunk = (kchunk + BKMAX - 1) / BKMAX * BKMAX;
+        S* part = nullptr;
+        if (splits > 1) {
+            const size_t need = need_of(v, oc);
+            if (ws.ext) {
+                APW_CHECK(need <= ws.ext_n,
+                          "gemm: %c%c m=%d n=%d k=%d splits over k into %d partial products of %zu bytes, more than "
+                          "the %zu-byte fixed workspace", ta, tb, m, n, k, splits, need * sizeof(double),
+                          ws.ext_n * sizeof(double));
+                part = reinterpret_cast<S*>(ws.ext);
+            } else {
+                ws.part.reserve(need);
+                part = reinterpret_cast<S*>(ws.part.get());
+            }
+        }
+        const double esz = 2.0 * sizeof(S);
+        const bool beta_nz = (b_re != 0.0 || b_im != 0.0);
+        const char* pname = X32 ? (splits > 1 ? "zgemm_x32_splitk" : "zgemm_x32_tiled")
+                                : std::is_same<R, double>::value ? (splits > 1 ? "zgemm_f64_splitk" : "zgemm_f64_tiled")
+                                                                 : (splits > 1 ? "cgemm_f32_splitk" : "cgemm_f32_tiled");
+        {
+            ProfScope ps(pf, s, pname,
+                         esz * ((double)m * k + (double)k * n + (double)m * n * (splits > 1 ? splits : (beta_nz ? 2 : 1))),
+                         8.0 * m * n * (double)k, true);
+#define APW_GEMM_CASE(X, Y)                                                                                          \
+    if (oa == X && ob == Y) {                                                                                       \
+        switch (v) {                                                                                                \
+            case 1: launch<R, S, X, Y, 2, 1, 8, 0>(ps, s, splits, m, n, k, kchunk, A, lda, B, ldb, Cp, ldcp, alpha_re, \
+                                               alpha_im, b_re, b_im, part); break;                                  \
+            case 2: launch<R, S, X, Y, 4, 2, 8, 1>(ps, s, splits, m, n, k, kchunk, A, lda, B, ldb, Cp, ldcp, alpha_re, \
+                                               alpha_im, b_re, b_im, part); break;                                  \
+            case 3: launch<R, S, X, Y, 4, 2, 8, 2>(ps, s, splits, m, n, k, kchunk, A, lda, B, ldb, Cp, ldcp, alpha_re, \
+                                               alpha_im, b_re, b_im, part); break;                                  \
+            case 4: launch<R, S, X, Y, 4, 1, 8, 2>(ps, s, splits, m, n, k, kchunk, A, lda, B, ldb, Cp, ldcp, alpha_re, \
+                                               alpha_im, b_re, b_im, part); break;                                  \
+            case 5: launch<R, S, X, Y, 2, 4, 16, 0>(ps, s, splits, m, n, k, kchunk, A, lda, B, ldb, Cp, ldcp, alpha_re, \
+                                                alpha_im, b_re, b_im, part); break;                                 \
+            case 6: launch<R, S, X, Y, 2, 2, 8, 3>(ps, s, splits, m, n, k, kchunk, A, lda, B, ldb, Cp, ldcp, alpha_re, \
+                                               alpha_im, b_re, b_im, part); break;                                  \
+            case 7: launch<R, S, X, Y, 2, 1, 8, 3>(ps, s, splits, m, n, k, kchunk, A, lda, B, ldb, Cp, ldcp, alpha_re, \
+                                               alpha_im, b_re, b_im, part); break;                                  \
+            default: launch<R, S, X, Y, 2, 2, 8, 0>(ps, s, splits, m, n, k, kchunk, A, lda, B, ldb, Cp, ldcp, alpha_re, \
+                                                alpha_im, b_re, b_im, part); break;                                 \
+        }                                                                                                           \
+    }
+            APW_GEMM_CASE(0, 0) APW_GEMM_CASE(0, 1) APW_GEMM_CASE(0, 2)
+            APW_GEMM_CASE(1, 0) APW_GEMM_CASE(1, 1) APW_GEMM_CASE(1, 2)
+            APW_GEMM_CASE(2, 0) APW_GEMM_CASE(2, 1) APW_GEMM_CASE(2, 2)
+#undef APW_GEMM_CASE
+            APW_LAUNCH_CHECK();
+        }
+        if (splits > 1) {
+            ProfScope pr(pf, s, "gemm_splitk_reduce", esz * (double)m * n * (splits + (beta_nz ? 2 : 1)),
+                         2.0 * m * n * (double)splits, true);
+            const size_t total = (size_t)m * n;
+            const int blocks = (int)std::min<size_t>((total + 255) / 256, 65536);
+            APW_PLAUNCH(pr, splitk_reduce<S>, dim3(blocks), dim3(256), 0, s, m, n, splits,
+                        reinterpret_cast<const c_t*>(part), reinterpret_cast<c_t*>(Cp), ldcp, alpha_re, alpha_im,
+                        b_re, b_im);
+            APW_LAUNCH_CHECK();
+        }
+    };
+    // autotuned plan (the fixed-workspace entry point keeps the fixed rule, whose partials
+    // it is sized for)
+    const int tmode = ws.ext ? 0 : tune_mode();
+    if (tmode > 0) {
+        auto bucket = [](int x) { return x <= 2048 ? (x + 31) / 32 * 32 : (x + 1023) / 1024 * 1024; };
+        char key[160];
+        std::snprintf(key, sizeof(key), "gemm %s %c%c m=%d n=%d k=%d cu=%d", prec_tag, "NTC"[oa], "NTC"[ob],
+                      bucket(m), bucket(n), bucket(k), cus);
+        int a = 0, b = 0;
+        if (tune_find(key, a, b) && a >= 0 && a < 8 && b >= 0 && b <= 6 && !(X32 && (a == 2 || a == 3))) {
+            variant = a;
+            occ = b;
+        } else if (tmode == 2) {
+            // candidates: every tile/map of the shape's kind, and for inner products every
+            // split rule; timed into a scratch C (beta 0) so the caller's C is untouched
+            std::vector<std::pair<int, int>> cand;
+            if (inner) {
+                for (int v : {0, 1, 6, 7})
+                    for (int oc : {1, 2, 3, 4, 6}) cand.emplace_back(v, oc);
+            } else if (tall) {
+                for (int v : {3, 4, 0, 2}) cand.emplace_back(v, 0);
+            } else {
+                for (int v : {0, 1, 5}) cand.emplace_back(v, 0);
+            }
+            ws.tune_c.reserve((size_t)m * n * 2 * sizeof(S) / sizeof(double) + 2);
+            S* Ct = reinterpret_cast<S*>(ws.tune_c.get());
+            double best = 1e300;
+            std::vector<std::pair<int, int>> seen;   // (variant, splits) already timed
+            for (auto& c : cand) {
+                if (X32 && (c.first == 2 || c.first == 3)) continue;
+                const std::pair<int, int> vs(c.first, splits_of(c.first, c.second));
+                if (std::find(seen.begin(), seen.end(), vs) != seen.end()) continue;
+                seen.push_back(vs);
+                const double us = time_on_stream(s, 2, [&] { run(c.first, c.second, Ct, m, 0.0, 0.0, nullptr); });
+                if (us < best) {
+                    best = us;
+                    variant = c.first;
+                    occ = c.second;
+                }
+            }
+            tune_store(key, variant, occ, best);
         }
     }
-    static const bool trace = std::getenv("APW_GEMM_TRACE") != nullptr;   // tuning aid: one line per call
+    static const bool trace = std::getenv("APW_GEMM_TRACE") != nullptr;   // debugging aid: one line per call
     if (trace)
-        std::fprintf(stderr, "apw_gemm %s %c%c m=%d n=%d k=%d variant=%d splits=%d\n",
-                     X32 ? "x32" : (std::is_same<R, double>::value ? "f64" : "f32"), ta, tb, m, n, k, variant, splits);
-    const double esz = 2.0 * sizeof(S);
-    const bool beta_nz = (beta_re != 0.0 || beta_im != 0.0);
-    const char* pname = X32 ? (splits > 1 ? "zgemm_x32_splitk" : "zgemm_x32_tiled")
-                            : std::is_same<R, double>::value ? (splits > 1 ? "zgemm_f64_splitk" : "zgemm_f64_tiled")
-                                                             : (splits > 1 ? "cgemm_f32_splitk" : "cgemm_f32_tiled");
-    ProfScope ps(prof, s, pname,
-                 esz * ((double)m * k + (double)k * n + (double)m * n * (splits > 1 ? splits : (beta_nz ? 2 : 1))),
-                 8.0 * m * n * (double)k, true);
-#define APW_GEMM_CASE(X, Y)                                                                                  \
-    if (oa == X && ob == Y) {                                                                               \
-        switch (variant) {                                                                                  \
-            case 1: launch<R, S, X, Y, 2, 1, 8, 0>(ps, s, splits, m, n, k, kchunk, A, lda, B, ldb, C, ldc, alpha_re,    \
-                                               alpha_im, beta_re, beta_im, part); break;                    \
-            case 2: launch<R, S, X, Y, 4, 2, 8, 1>(ps, s, splits, m, n, k, kchunk, A, lda, B, ldb, C, ldc, alpha_re,    \
-                                               alpha_im, beta_re, beta_im, part); break;                    \
-            case 3: launch<R, S, X, Y, 4, 2, 8, 2>(ps, s, splits, m, n, k, kchunk, A, lda, B, ldb, C, ldc, alpha_re,    \
-                                               alpha_im, beta_re, beta_im, part); break;                    \
-            case 4: launch<R, S, X, Y, 4, 1, 8, 2>(ps, s, splits, m, n, k, kchunk, A, lda, B, ldb, C, ldc, alpha_re,    \
-                                               alpha_im, beta_re, beta_im, part); break;                    \
-            case 5: launch<R, S, X, Y, 2, 4, 16, 0>(ps, s, splits, m, n, k, kchunk, A, lda, B, ldb, C, ldc, alpha_re,   \
-                                                alpha_im, beta_re, beta_im, part); break;                   \
-            default: launch<R, S, X, Y, 2, 2, 8, 0>(ps, s, splits, m, n, k, kchunk, A, lda, B, ldb, C, ldc, alpha_re,   \
-                                                alpha_im, beta_re, beta_im, part); break;                   \
-        }                                                                                                   \
-    }
-    APW_GEMM_CASE(0, 0) APW_GEMM_CASE(0, 1) APW_GEMM_CASE(0, 2)
-    APW_GEMM_CASE(1, 0) APW_GEMM_CASE(1, 1) APW_GEMM_CASE(1, 2)
-    APW_GEMM_CASE(2, 0) APW_GEMM_CASE(2, 1) APW_GEMM_CASE(2, 2)
-#undef APW_GEMM_CASE
-    APW_LAUNCH_CHECK();
-    if (splits > 1) {
-        ProfScope pr(prof, s, "gemm_splitk_reduce", esz * (double)m * n * (splits + (beta_nz ? 2 : 1)),
-                     2.0 * m * n * (double)splits, true);
-        const size_t total = (size_t)m * n;
-        const int blocks = (int)std::min<size_t>((total + 255) / 256, 65536);
-        APW_PLAUNCH(pr, splitk_reduce<S>, dim3(blocks), dim3(256), 0, s, m, n, splits,
-                           reinterpret_cast<const c_t*>(part), reinterpret_cast<c_t*>(C), ldc, alpha_re,
-                           alpha_im, beta_re, beta_im);
-        APW_LAUNCH_CHECK();
-    }
+        std::fprintf(stderr, "apw_gemm %s %c%c m=%d n=%d k=%d variant=%d splits=%d\n", prec_tag, ta, tb, m, n, k,
+                     variant, splits_of(variant, occ));
+    run(variant, occ, C, ldc, beta_re, beta_im, prof);
 }
 
 }  // namespace

@@ -99,6 +99,7 @@ int apw_dngvx_op(apw_ctx* ctx, int nstart, int ldh, const double* hcc, const dou
         apw::DeviceGuard g(ctx->device);
         ctx->status.reserve(4);
         APW_HIP(hipMemsetAsync(ctx->status.get(), 0, 4 * sizeof(int), ctx->stream));
+        ctx->ews.single_max = ctx->eig_single_max;
         apw::hegvx_lowest(ctx->stream, ctx->gws, ctx->ews, ctx->gen, ctx->num_cu, nstart, ldh, hcc, scc, m, eigenvalue,
                           vcc, ldh, ctx->status.get(), &ctx->prof);
         int st = 0;

@@ -1048,14 +1048,23 @@ const cplx<R>* tw_ptr(const PwGrid& g, int off) {
 
 // x length for the register x pass; 0 (the run-time-plan kernel) when the x slots are more
 // than two runs or a band's slots exceed the 31-bit byte offsets of its buffer loads
-int xreg_len(const PwGrid& g) { return (g.xrun_ok && g.bytes_X(sizeof(double2)) < ((size_t)1 << 31)) ? g.nx : 0; }
+int xreg_len(const PwGrid& g, int sched) {
+    return (!(sched & 1) && g.xrun_ok && g.bytes_X(sizeof(double2)) < ((size_t)1 << 31)) ? g.nx : 0;
+}
 // y length for the register y passes (0: run-time-plan kernels): sticks of every slot in
 // at most two runs of y, a band's sticks within 31-bit byte offsets
-int yreg_len(const PwGrid& g) { return (g.yrun_ok && g.bytes_S(sizeof(double2)) < ((size_t)1 << 31)) ? g.ny : 0; }
+int yreg_len(const PwGrid& g, int sched) {
+    return (!(sched & 2) && g.yrun_ok && g.bytes_S(sizeof(double2)) < ((size_t)1 << 31)) ? g.ny : 0;
+}
 // z length for the register z passes: sphere offsets (16 B per plane wave) within 31 bits
-int zreg_len(const PwGrid& g, int ld, int ldh) {
+int zreg_len(const PwGrid& g, int ld, int ldh, int sched) {
     const size_t m = (size_t)std::max(std::max(ld, ldh), g.npw) * 16;
-    return m < ((size_t)1 << 31) ? g.nz : 0;
+    return (!(sched & 4) && m < ((size_t)1 << 31)) ? g.nz : 0;
+}
+// the schedule bits of the precision R (autotuner's choice; 0 = register kernels throughout)
+template <class R>
+int sched_of(const PwGrid& g) {
+    return g.sched[std::is_same<R, float>::value ? 1 : 0];
 }
 
 template <class R>
@@ -1074,7 +1083,7 @@ void pass_forward_half(hipStream_t s, const PwGrid& g, R* S, R* X, int nb, doubl
                            s, reinterpret_cast<const C*>(X), reinterpret_cast<C*>(S), g.ns, g.nz, g.nxr,         \
                            g.slot_start.get(), g.yrun.get(), tw_ptr<R>(g, g.twoff_y));                       \
         break;
-        switch (yreg_len(g)) {
+        switch (yreg_len(g, sched_of<R>(g))) {
             APW_FFT_SIZES(APW_YREG_LAUNCH)
             default:
                 APW_PLAUNCH(ps, (k_yfwd<R>), dim3(g.nxr, (g.nz + Ly - 1) / Ly, nb), dim3(NT),
@@ -1094,7 +1103,7 @@ void pass_forward_half(hipStream_t s, const PwGrid& g, R* S, R* X, int nb, doubl
                            ld, g.stick_ig.get(), g.ns, reinterpret_cast<C*>(S), hpsi, ldh, gk2, tpiba2, (int)mode, \
                            scale, tw_ptr<R>(g, g.twoff_z));                                                  \
         break;
-    switch (zreg_len(g, ld, ldh)) {
+    switch (zreg_len(g, ld, ldh, sched_of<R>(g))) {
         APW_FFT_SIZES(APW_ZREG_LAUNCH)
         default:
             APW_PLAUNCH(ps, (k_zfwd<R>), dim3((g.ns + Lz - 1) / Lz, nb), dim3(NT), lds_bytes<R>(g.nz, Lz), s,
@@ -1120,7 +1129,7 @@ void pass_backward_half(hipStream_t s, const PwGrid& g, const double2* psi, int 
                            ld, g.stick_ig.get(), g.ns, reinterpret_cast<C*>(S), (double2*)nullptr, 0,          \
                            (const double*)nullptr, 0.0, 0, 0.0, tw_ptr<R>(g, g.twoff_z));                    \
         break;
-        switch (zreg_len(g, ld, 0)) {
+        switch (zreg_len(g, ld, 0, sched_of<R>(g))) {
             APW_FFT_SIZES(APW_ZREG_LAUNCH)
             default:
                 APW_PLAUNCH(ps, (k_zbwd<R>), dim3((g.ns + Lz - 1) / Lz, nb), dim3(NT), lds_bytes<R>(g.nz, Lz),
@@ -1138,7 +1147,7 @@ void pass_backward_half(hipStream_t s, const PwGrid& g, const double2* psi, int 
                            s, reinterpret_cast<const C*>(S), reinterpret_cast<C*>(X), g.ns, g.nz, g.nxr,         \
                            g.slot_start.get(), g.yrun.get(), tw_ptr<R>(g, g.twoff_y));                       \
         break;
-    switch (yreg_len(g)) {
+    switch (yreg_len(g, sched_of<R>(g))) {
         APW_FFT_SIZES(APW_YREG_LAUNCH)
         default:
             APW_PLAUNCH(ps, (k_ybwd<R>), dim3(g.nxr, (g.nz + Ly - 1) / Ly, nb), dim3(NT), lds_bytes<R>(g.ny, Ly),
@@ -1152,6 +1161,27 @@ void pass_backward_half(hipStream_t s, const PwGrid& g, const double2* psi, int 
 }
 
 }  // namespace
+
+bool xreg_available(const PwGrid& g) {
+    switch (xreg_len(g, 0)) {
+#define APW_LEN_CASE(N) case N: return true;
+        APW_FFT_SIZES(APW_LEN_CASE)
+        default: return false;
+    }
+}
+bool yreg_available(const PwGrid& g) {
+    switch (yreg_len(g, 0)) {
+        APW_FFT_SIZES(APW_LEN_CASE)
+        default: return false;
+    }
+}
+bool zreg_available(const PwGrid& g, int ld) {
+    switch (zreg_len(g, ld, ld, 0)) {
+        APW_FFT_SIZES(APW_LEN_CASE)
+#undef APW_LEN_CASE
+        default: return false;
+    }
+}
 
 void PwGrid::build(int nx_, int ny_, int nz_, int npw_, const int* ig2ixyz) {
     nx = nx_;
@@ -1280,7 +1310,7 @@ void local_hpsi(hipStream_t s, const PwGrid& g, const double* psi, int ld, doubl
                            reinterpret_cast<C*>(X), g.nxr, g.ny, g.nz, g.xrun, veff, (double2*)nullptr, 0,  \
                            0.0, tw_ptr<R>(g, g.twoff_x));                                                     \
         break;
-        switch (xreg_len(g)) {
+        switch (xreg_len(g, sched_of<R>(g))) {
             APW_FFT_SIZES(APW_XREG_LAUNCH)
             default:
                 APW_PLAUNCH(ps, (k_xpass<R, 0>), dim3(g.ny, (g.nz + Lx - 1) / Lx, nb), dim3(NT),
@@ -1311,7 +1341,7 @@ void sphere_to_box(hipStream_t s, const PwGrid& g, const double* psi, int ld, do
                            s, reinterpret_cast<C*>(X), g.nxr, g.ny, g.nz, g.xrun, (const double*)nullptr,   \
                            reinterpret_cast<double2*>(box), add ? 1 : 0, factor, tw_ptr<double>(g, g.twoff_x));    \
         break;
-    switch (xreg_len(g)) {
+    switch (xreg_len(g, g.sched[0])) {
         APW_FFT_SIZES(APW_XREG_LAUNCH)
         default:
             hipLaunchKernelGGL((k_xpass<double, 1>), dim3(g.ny, (g.nz + Lx - 1) / Lx, nb), dim3(NT),
@@ -1335,7 +1365,7 @@ void box_to_sphere(hipStream_t s, const PwGrid& g, const double* box, double* hp
                            s, reinterpret_cast<C*>(X), g.nxr, g.ny, g.nz, g.xrun, (const double*)nullptr,   \
                            bx, 0, 0.0, tw_ptr<double>(g, g.twoff_x));                                              \
         break;
-    switch (xreg_len(g)) {
+    switch (xreg_len(g, g.sched[0])) {
         APW_FFT_SIZES(APW_XREG_LAUNCH)
         default:
             hipLaunchKernelGGL((k_xpass<double, 2>), dim3(g.ny, (g.nz + Lx - 1) / Lx, nb), dim3(NT),
@@ -1367,7 +1397,7 @@ void psi_to_rho(hipStream_t s, const PwGrid& g, const double* psi, int ld, int n
                            reinterpret_cast<const C*>(X), g.nxr, g.ny, g.nz, nb, g.xrun, w, rho, add ? 1 : 0, \
                            tw_ptr<double>(g, g.twoff_x));                                                    \
         break;
-    switch (xreg_len(g)) {
+    switch (xreg_len(g, g.sched[0])) {
         APW_FFT_SIZES(APW_XREG_LAUNCH)
         default:
             APW_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_xrho),

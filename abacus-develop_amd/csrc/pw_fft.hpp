@@ -38,6 +38,10 @@ struct PwGrid {
     bool xrun_ok = false;     // the slots are at most two runs
     DevBuf<int4> yrun;        // nxr: each slot's sticks as runs of y (same convention)
     bool yrun_ok = false;     // every slot's sticks are at most two runs
+    // pass schedule per precision ([0] FP64, [1] FP32), chosen by the autotuner: bit 0 runs
+    // the x pass, bit 1 the y passes, bit 2 the z passes on the LDS Stockham kernels with the
+    // run-time radix plan instead of the register four-step kernels (0: register throughout)
+    int sched[2] = {0, 0};
     DevBuf<double> tw64;      // twiddles exp(-2 pi i k / n) for x, y, z (c128), offsets below
     DevBuf<float> tw32;
     int twoff_x = 0, twoff_y = 0, twoff_z = 0;
@@ -46,6 +50,12 @@ struct PwGrid {
     size_t bytes_S(size_t elem) const { return (size_t)ns * nz * elem; }
     size_t bytes_X(size_t elem) const { return (size_t)nxr * ny * nz * elem; }
 };
+
+// whether the register four-step kernels can run the x, y and z passes of this grid (the
+// alternative schedules the autotuner may choose)
+bool xreg_available(const PwGrid& g);
+bool yreg_available(const PwGrid& g);
+bool zreg_available(const PwGrid& g, int ld);
 
 enum GatherMode { GATHER_SET = 0, GATHER_ADD = 1, GATHER_SET_KIN = 2 };
 

@@ -256,19 +256,78 @@ void Ham::create(apw_ctx* c, const apw_ham_desc* d) {
         up(kb_off, off_of);
         up(kb_nh, nh_of);
     }
-    // band chunk of the local pass: keep the S + X intermediates of one chunk near the
-    // size of the Infinity Cache so the five passes mostly hit it
+    // band chunk of the local pass (fixed rule): keep the S + X intermediates of one chunk
+    // near the size of the Infinity Cache so the five passes mostly hit it; FP32
+    // intermediates take half the bytes: twice the bands per chunk
     const size_t per_band = grid.bytes_S(16) + grid.bytes_X(16);
     chunk = (int)std::max<size_t>(1, std::min<size_t>(256, (size_t)(192u << 20) / per_band));
-    S.reserve(2 * grid.bytes_S(1) * chunk);   // sized here: no allocation inside H|psi>
-    X.reserve(2 * grid.bytes_X(1) * chunk);
+    chunk32 = std::min(256, 2 * chunk);
+    if (tune_mode() > 0) tune_local();
+    const size_t bands16 = (size_t)std::max(chunk, (chunk32 + 1) / 2);   // c128-equivalent bands
+    S.reserve(2 * grid.bytes_S(1) * bands16);   // sized here: no allocation inside H|psi>
+    X.reserve(2 * grid.bytes_X(1) * bands16);
     APW_HIP(hipStreamSynchronize(ctx->stream));
+}
+
+// Autotuner of the local pass, per precision: the bands per launch and the pass schedule
+// (register four-step kernels, or the LDS Stockham kernels with the run-time radix plan on
+// the x, y or z passes), each candidate timed over 64 bands of the sphere.
+void Ham::tune_local() {
+    const int nbt = 64;
+    const size_t per_band16 = grid.bytes_S(16) + grid.bytes_X(16);
+    for (int pi = 0; pi < 2; ++pi) {
+        int& ch = pi ? chunk32 : chunk;
+        char key[160];
+        std::snprintf(key, sizeof(key), "fft %s %dx%dx%d npw=%d cu=%d", pi ? "f32" : "f64", grid.nx, grid.ny, grid.nz,
+                      npw, ctx->num_cu);
+        int a = 0, b = 0;
+        if (tune_find(key, a, b) && a >= 1 && a <= 256 && b >= 0 && b <= 7) {
+            ch = a;
+            grid.sched[pi] = b;
+            continue;
+        }
+        if (tune_mode() != 2) continue;
+        DevBuf<double> tp, th;
+        tp.reserve((size_t)2 * npwx * nbt);
+        th.reserve((size_t)2 * npwx * nbt);
+        APW_HIP(hipMemsetAsync(tp.get(), 0, (size_t)16 * npwx * nbt, ctx->stream));
+        std::vector<int> chunks;
+        const int rule = ch;
+        const size_t per_band = pi ? per_band16 / 2 : per_band16;
+        for (int c : {rule / 2, rule, 2 * rule, 4 * rule, 8, 16, 32, 64}) {
+            c = std::max(1, std::min(c, nbt));
+            if ((size_t)c * per_band > ((size_t)2 << 30) && c > rule) continue;   // at most 2 GiB of intermediates
+            if (std::find(chunks.begin(), chunks.end(), c) == chunks.end()) chunks.push_back(c);
+        }
+        std::vector<int> scheds = {0};
+        if (xreg_available(grid)) scheds.push_back(1);
+        if (yreg_available(grid)) scheds.push_back(2);
+        if (zreg_available(grid, npwx)) scheds.push_back(4);
+        double best = 1e300;
+        int best_c = rule, best_s = 0;
+        const int prec = pi ? APW_FP32_MIXED : APW_FP64;
+        for (int sc : scheds)
+            for (int c : chunks) {
+                grid.sched[pi] = sc;
+                ch = c;
+                const double us = time_on_stream(ctx->stream, 2, [&] {
+                    local(tp.get(), npwx, th.get(), npwx, nbt, GATHER_SET, prec);
+                });
+                if (us < best) {
+                    best = us;
+                    best_c = c;
+                    best_s = sc;
+                }
+            }
+        ch = best_c;
+        grid.sched[pi] = best_s;
+        tune_store(key, best_c, best_s, best);
+    }
 }
 
 void Ham::local(const double* psi, int ld, double* hpsi, int ldh, int nb, GatherMode mode, int prec) {
     const double scale = 1.0 / (double)((size_t)grid.nx * grid.ny * grid.nz);
-    // FP32 intermediates take half the bytes: twice the bands per chunk, half the launches
-    const int cb = prec == APW_FP64 ? chunk : 2 * chunk;
+    const int cb = prec == APW_FP64 ? chunk : chunk32;
     for (int b0 = 0; b0 < nb; b0 += cb) {
         const int n = std::min(cb, nb - b0);
         if (prec == APW_FP64) {

@@ -37,7 +37,11 @@ enum {
     APW_ECALLBACK = -6    /* a user callback returned non-zero            */
 };
 
-enum { APW_FP64 = 0, APW_FP32_MIXED = 1 };
+/* APW_FP32_MIXED_X32 (Davidson only; no counterpart in the reference): the FP32-mixed form
+ * with the pre-phase's subspace products (cal_grad's residual block, cal_elem's new hcc rows)
+ * also on the FP32 matrix cores (c128 operands rounded to FP32, FP64 sums); the
+ * orthonormalisation, the restart vectors and the FP64 refinement stay FP64. */
+enum { APW_FP64 = 0, APW_FP32_MIXED = 1, APW_FP32_MIXED_X32 = 2 };
 
 typedef struct apw_ctx apw_ctx;
 typedef struct apw_ham apw_ham;
@@ -53,6 +57,12 @@ int apw_ctx_destroy(apw_ctx* ctx);
 int apw_ctx_synchronize(apw_ctx* ctx);
 void* apw_ctx_stream(apw_ctx* ctx);
 
+/* No counterpart in the reference.  Subspace matrices of the solvers on this context (and of
+ * apw_dnevx_op / apw_dngvx_op) with n <= nmax are reduced to tridiagonal form by a single
+ * workgroup with an ordinary launch instead of the cooperative multi-workgroup reduction, so
+ * many small solves on different streams (batched k-points or cells) run at the same time;
+ * 0 (the default) never does.  The eigenpairs are those of the cooperative form. */
+int apw_ctx_set_eig_single_wg(apw_ctx* ctx, int nmax);
 /* Kernel profiler (no counterpart in the reference, whose ModuleBase::timer is host-side,
  * timer.cpp:71-130): when on, the library's kernels are bracketed by HIP events on the
  * context stream and accumulated per kernel name with their algorithmic HBM bytes and
@@ -218,7 +228,8 @@ typedef struct apw_diag_stats {
  *   ham != NULL: H|psi> applied by the library (S = I); hpsi_cb/spsi_cb ignored.
  *   ham == NULL: hpsi_cb required; spsi_cb NULL means S = I.
  *   psi: device c128, ldpsi x nband, start vectors in, eigenvectors out.
- *   eig: HOST double[nband] out.  precision: APW_FP64 or APW_FP32_MIXED (library H only).
+ *   eig: HOST double[nband] out.  precision: APW_FP64, APW_FP32_MIXED or
+ *   APW_FP32_MIXED_X32 (the mixed forms with the library's H only).
  * Returns the total Davidson iteration count (>= 0) or a negative status. */
 int apw_david_diag(apw_david* dav, apw_ham* ham, apw_hpsi_cb hpsi_cb, apw_spsi_cb spsi_cb,
                    void* user, int ldpsi, double* psi, double* eig, double david_diag_thr,
@@ -348,6 +359,19 @@ int apw_zgemm_noalloc(apw_ctx* ctx, char transa, char transb, int m, int n, int 
  * v_mfma_f64_16x16x4f64 and v_mfma_f32_16x16x4f32 (4 x 64 x 4 doubles at out_dev).  No
  * counterpart in the reference: a check of the operand layout the GEMMs rely on. */
 int apw_probe_mfma_layout(apw_ctx* ctx, double* out_dev);
+/* ---- autotuner.  No counterpart in the reference (its GPU path calls vendor libraries with
+ * fixed arguments).  A process-wide table of measured choices per shape: GEMM plans (tile
+ * shape, workgroup-to-XCD mapping, split-K rule), the subspace eigensolver's workgroup count,
+ * and each FFT grid's pass schedule (register four-step or LDS Stockham kernels per axis) and
+ * bands per launch.  mode 0: fixed rules (the default); 1: the table, fixed rules on a miss;
+ * 2: as 1, and a miss is measured where it occurs (every candidate timed on the calling
+ * stream, the fastest recorded).  The table is saved to / loaded from a JSON file. */
+int apw_tune_set_mode(int mode);
+int apw_tune_load(const char* path);   /* returns the number of entries read */
+int apw_tune_save(const char* path);   /* returns the number of entries written */
+int apw_tune_clear(void);
+int apw_tune_stats(long long* entries, long long* hits, long long* misses, long long* tuned);
+int apw_tune_entry(int i, char* key, int key_cap, int* a, int* b, double* us);
 /* dnevx_op<complex<double>, DEVICE_GPU> (dngvd_op.h:85-107): lowest nbands eigenpairs
  * of the Hermitian nstart x nstart matrix hcc (lower triangle, ld ldh).  eigenvalue:
  * device double[nbands]; vcc: device c128, ld ldh, eigenvector j in column j. */

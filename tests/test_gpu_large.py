@@ -110,6 +110,15 @@ def test_large_hpsi_and_david(torch, ctx, name):
     assert dm < 1e-9
     assert ovl_m < 1e-10 and rq_m < 1e-10
     del psim
+    # the mixed form with the pre-phase's subspace products on the FP32 matrix cores too
+    psix, ex32, itx, stx = _solve(torch, ctx, s, ham, "fp32_mixed_x32")
+    ovl_x, rq_x, res_x = _invariants(torch, ctx, s, ham, psix, ex32, stx)
+    dx = np.abs(ex32 - e64).max()
+    print(name, "mixed x32", itx, "iterations (FP64 from", stx["fp64_switch_iter"], "); |x32 - fp64|", dx,
+          "|psi^H psi - I|", ovl_x, "|rq - e|", rq_x, flush=True)
+    assert dx < 1e-9
+    assert ovl_x < 1e-10 and rq_x < 1e-10
+    del psix
     ham.close()
     ref = _ref(f"{name}:0")
     if not ref or not ("plain" in ref or "detsign" in ref):
@@ -125,6 +134,9 @@ def test_large_hpsi_and_david(torch, ctx, name):
         # all-eigenpair zheevx calls, so its trajectory differs; both stop within one threshold
         d_plain = np.abs(e64 - np.array(ref["plain"]["eig"])).max()
         dm_plain = np.abs(emx - np.array(ref["plain"]["eig"])).max()
-        print(name, "fp64 vs reference as built", d_plain, "| mixed vs reference as built", dm_plain)
+        dx_plain = np.abs(ex32 - np.array(ref["plain"]["eig"])).max()
+        print(name, "fp64 vs reference as built", d_plain, "| mixed vs reference as built", dm_plain,
+              "| mixed x32", dx_plain)
         assert d_plain < 1e-9
         assert dm_plain < 1e-9
+        assert dx_plain < 1e-9

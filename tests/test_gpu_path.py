@@ -334,6 +334,34 @@ def test_dnevx_vs_lapack(torch, ctx, n, nband, ld):
     assert np.abs(V.conj().T @ V - np.eye(nband)).max() < 1e-11
 
 
+@pytest.mark.parametrize("n,nband", [(2, 1), (48, 24), (192, 48), (384, 96), (640, 160)])
+def test_dnevx_single_workgroup(torch, n, nband):
+    """The one-workgroup reduction (ordinary launch, for concurrent small solves): eigenpairs
+    vs LAPACK, and the same eigenvectors (phases included) as the cooperative form."""
+    from apw import pw
+    rng = np.random.default_rng(1000 + n)
+    H = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    H = H + H.conj().T
+    w = np.linalg.eigvalsh(H)
+    dh = dev(torch, np.tril(H).T.copy())
+    out = {}
+    for single in (0, n):
+        c = pw.Context(0)
+        c.set_eig_single_wg(single)
+        de = torch.zeros(nband, dtype=torch.float64, device="cuda")
+        dv = torch.zeros((n, n), dtype=torch.complex128, device="cuda")
+        pw.dnevx_op(c, n, n, dh, nband, de, dv)
+        out[single] = (host(c, de), host(c, dv).T[:, :nband])
+        c.close()
+    e, V = out[n]
+    scale = max(1.0, np.abs(w).max())
+    assert np.abs(e - w[:nband]).max() < 1e-12 * scale
+    assert np.abs(H @ V - V * e).max() < 1e-11 * scale
+    assert np.abs(V.conj().T @ V - np.eye(nband)).max() < 1e-11
+    assert np.abs(e - out[0][0]).max() < 1e-12 * scale
+    assert np.abs(V - out[0][1]).max() < 1e-10
+
+
 @pytest.mark.parametrize("n,nband", [(48, 24), (640, 160)])
 def test_dnevx_phases_match_oracle(torch, ctx, n, nband):
     """Eigenvector phases: the oracle's zheevx inverse-iteration convention, to 1e-10."""
@@ -614,6 +642,44 @@ def test_david_batched_kpoints_match_sequential(torch):
         dav.close()
         ham.close()
     for c in ctxs + [c0]:
+        c.close()
+
+
+def test_david_batched_single_workgroup_eigensolves(torch):
+    """Eight k-points on eight contexts whose subspace reductions run in one workgroup each
+    (no device-wide ordering, so the eight solves overlap): the same iterations and
+    eigenvalues as the same setting one after another, and within 1e-9 Ry of the cooperative
+    reduction's solves."""
+    from apw import pw
+    systems = [configs.make("kpts16", ik=k) for k in range(0, 64, 8)]
+    seq = {}
+    for single in (0, 192):
+        c0 = pw.Context(0)
+        c0.set_eig_single_wg(single)
+        seq[single] = []
+        for s in systems:
+            ham = pw.HamiltPW(c0, s)
+            dav = pw.DiagoDavid(c0, s.precondition, s.nband, s.npw, s.david_ndim)
+            eig = np.zeros(s.nband)
+            it = dav.diag(ham, None, s.npwx, dev(torch, s.psi0), eig, 1e-9, 500, 5, 0)
+            seq[single].append((it, eig.copy()))
+            dav.close()
+            ham.close()
+        c0.close()
+    ctxs = [pw.Context(0) for _ in systems]
+    units = []
+    for c, s in zip(ctxs, systems):
+        c.set_eig_single_wg(192)
+        units.append((pw.DiagoDavid(c, s.precondition, s.nband, s.npw, s.david_ndim), pw.HamiltPW(c, s),
+                      dev(torch, s.psi0), s.npwx, np.zeros(s.nband)))
+    its = pw.david_diag_batched(units, 1e-9, 500)
+    for k, u in enumerate(units):
+        assert its[k] == seq[192][k][0]
+        assert np.abs(u[4] - seq[192][k][1]).max() < 1e-12
+        assert np.abs(u[4] - seq[0][k][1]).max() < 1e-9
+        u[0].close()
+        u[1].close()
+    for c in ctxs:
         c.close()
 
 
