@@ -55,7 +55,7 @@ def run_solve(pw, ctx, ham, s, psi0_dev, precision):
 
 # library profiler name -> kernel-name prefixes in the rocprofv3 traffic summary
 _PMC_FAMILY = {   # regular expressions over the kernel names of the committed traffic summary
-    "eig_tridiag": r"apw::k_tridiag_(reg|lds)",
+    "eig_tridiag": r"apw::k_tridiag_(reg|fused|lds)",
     # GEMMs: the path's updates are op(A) = N (tiled), its inner products op(A) = C (split-K)
     # (template arguments: MFMA type, storage type, op(A), ...)
     "zgemm_f64_tiled": r"apw::zgemm_kernel<double, double, 0, ",
@@ -287,80 +287,89 @@ def measure_next(pw, ctx, ham, s, psi0_dev, eig_david, reps=5):
     return out
 
 
-def measure_cells(pw, ncells=32):
-    """The uneven-cell batch (cells 0..ncells-1 of the 128-cell set: 2-32 atoms, 18^3-64^3
-    grids, nbands = 3 atoms): the plain FP64 per-cell loop on one context vs the FP32-mixed
-    batched solve on two contexts, with the per-cell eigenvalue difference between them."""
+def _batched_run(pw, systems, ncontexts, precision, single_wg, thr=1e-9):
+    """apw_david_diag_batched over the systems spread round-robin on ncontexts contexts;
+    single_wg: subspace reductions up to that size in one workgroup (no device-wide ordering
+    of the eigensolves, so the units' solves overlap).  Returns (seconds, iterations, eigs)."""
     import numpy as np
     import torch
+    ctxs = [pw.Context(0) for _ in range(ncontexts)]
+    for c in ctxs:
+        c.set_eig_single_wg(single_wg)
+    units = []
+    for i, s in enumerate(systems):
+        c = ctxs[i % ncontexts]
+        units.append((pw.DiagoDavid(c, s.precondition, s.nband, s.npw, s.david_ndim), pw.HamiltPW(c, s),
+                      torch.from_numpy(np.ascontiguousarray(s.psi0)).cuda(), s.npwx, np.zeros(s.nband)))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    its = pw.david_diag_batched(units, thr, 500, precision=precision)
+    for c in ctxs:
+        c.synchronize()
+    dt = time.perf_counter() - t0
+    eigs = [u[4].copy() for u in units]
+    for u in units:
+        u[0].close()
+        u[1].close()
+    for c in ctxs:
+        c.close()
+    return dt, int(sum(its)), eigs
+
+
+def _err_vs(eigs, ref):
+    """(largest per-unit max |difference|, units within 1e-9 Ry)"""
+    import numpy as np
+    d = [float(np.abs(a - b).max()) for a, b in zip(eigs, ref)]
+    return max(d), int(sum(x < 1e-9 for x in d))
+
+
+def measure_cells(pw, ncells=128, ncontexts=4):
+    """The uneven-cell batch (all 128 cells: 2-32 atoms, 18^3-72^3 grids, nbands = 3 atoms):
+    the plain FP64 per-cell loop on one context vs the batched solve on ncontexts contexts (one
+    stream each; the GPU has 4 hardware queues), FP64 and FP32-mixed.  Each form's per-cell
+    eigenvalue error is taken against a converged FP64 solve (thr 1e-12 Ry), since the
+    reference's stopping rule (eigenvalue change below 1e-9 Ry) bounds the change between
+    iterations, not the distance from the converged values."""
     from apw import configs
     systems = [configs.make_cell(i) for i in range(ncells)]
-    res, eig64 = {}, None
-    for label, nc, prec in (("fp64_loop_1_context", 1, "fp64"), ("fp32_mixed_batched_2_contexts", 2, "fp32_mixed")):
-        ctxs = [pw.Context(0) for _ in range(nc)]
-        units = []
-        for i, s in enumerate(systems):
-            c = ctxs[i % nc]
-            units.append((pw.DiagoDavid(c, s.precondition, s.nband, s.npw, s.david_ndim), pw.HamiltPW(c, s),
-                          torch.from_numpy(np.ascontiguousarray(s.psi0)).cuda(), s.npwx, np.zeros(s.nband)))
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        its = pw.david_diag_batched(units, 1e-9, 500, precision=prec)
-        for c in ctxs:
-            c.synchronize()
-        dt = time.perf_counter() - t0
-        eigs = [u[4].copy() for u in units]
-        entry = {"time_s": round(dt, 4), "iterations": int(sum(its))}
-        if eig64 is None:
-            eig64 = eigs
-        else:
-            d = [float(np.abs(a - b).max()) for a, b in zip(eigs, eig64)]
-            entry["max_abs_eig_diff_vs_fp64_Ry"] = max(d)
-            entry["cells_within_1e-9_Ry"] = int(sum(x < 1e-9 for x in d))
-        res[label] = entry
-        for u in units:
-            u[0].close()
-            u[1].close()
-        for c in ctxs:
-            c.close()
+    res = {}
+    _, _, eig_conv = _batched_run(pw, systems, ncontexts, "fp64", 0, thr=1e-12)
+    dt, it, eig64 = _batched_run(pw, systems, 1, "fp64", 0)
+    e, n = _err_vs(eig64, eig_conv)
+    res["fp64_loop_1_context"] = {"time_s": round(dt, 4), "iterations": it, "max_abs_eig_err_vs_converged_Ry": e,
+                                  "cells_within_1e-9_Ry_of_converged": n}
+    for label, prec in (("fp64_batched", "fp64"), ("fp32_mixed_batched", "fp32_mixed")):
+        dt, it, eigs = _batched_run(pw, systems, ncontexts, prec, 0)
+        e, n = _err_vs(eigs, eig_conv)
+        e64, n64 = _err_vs(eigs, eig64)
+        res[f"{label}_{ncontexts}_contexts"] = {"time_s": round(dt, 4), "iterations": it,
+                                               "max_abs_eig_err_vs_converged_Ry": e,
+                                               "cells_within_1e-9_Ry_of_converged": n,
+                                               "max_abs_eig_diff_vs_fp64_loop_Ry": e64,
+                                               "cells_within_1e-9_Ry_of_fp64_loop": n64}
     res["cells"] = ncells
     res["npw_range"] = [int(min(s.npw for s in systems)), int(max(s.npw for s in systems))]
     res["grids"] = sorted({f"{s.nx}^3" for s in systems}, key=lambda g: int(g.split("^")[0]))
     return res
 
 
-def measure_batched(pw, nk=16, ncontexts=(1, 2)):
-    """The batched form on k-points of the 16-atom 4x4x4 mesh (every fourth of the 64,
-    uneven plane-wave counts): apw_david_diag_batched with the units spread over 1 context
-    (one after another) and over 2 (two streams, two host threads), FP64 to 1e-9 Ry."""
-    import numpy as np
-    import torch
+def measure_batched(pw, nk=64, ncontexts=4):
+    """The batched form on all 64 k-points of the 16-atom 4x4x4 mesh (uneven plane-wave
+    counts, shared 48x48x96 grid): the FP64 per-k loop on one context vs the batched solve on
+    ncontexts contexts (one stream each; the GPU has 4 hardware queues), FP64 and FP32-mixed,
+    and the FP64 batch with one-workgroup subspace reductions (no device-wide ordering of the
+    eigensolves) beside the cooperative ones."""
     from apw import configs
     systems = [configs.make("kpts16", ik=k) for k in range(0, 64, 64 // nk)]
-    res, eig_ref = {}, None
-    for nc in ncontexts:
-        ctxs = [pw.Context(0) for _ in range(nc)]
-        units = []
-        for i, s in enumerate(systems):
-            c = ctxs[i % nc]
-            units.append((pw.DiagoDavid(c, s.precondition, s.nband, s.npw, s.david_ndim), pw.HamiltPW(c, s),
-                          torch.from_numpy(np.ascontiguousarray(s.psi0)).cuda(), s.npwx, np.zeros(s.nband)))
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        its = pw.david_diag_batched(units, 1e-9, 500)
-        for c in ctxs:
-            c.synchronize()
-        dt = time.perf_counter() - t0
-        eig = np.concatenate([u[4] for u in units])
-        if eig_ref is None:
-            eig_ref = eig
-        res[f"contexts_{nc}"] = {"time_s": round(dt, 4), "iterations": int(sum(its)),
-                                 "max_abs_eig_diff_vs_1_context_Ry": float(np.abs(eig - eig_ref).max())}
-        for u in units:
-            u[0].close()
-            u[1].close()
-        for c in ctxs:
-            c.close()
+    nbx = max(s.nband * s.david_ndim for s in systems)
+    res = {}
+    dt, it, eig_loop = _batched_run(pw, systems, 1, "fp64", 0)
+    res["fp64_loop_1_context"] = {"time_s": round(dt, 4), "iterations": it}
+    for label, prec, single in (("fp64_batched", "fp64", 0), ("fp32_mixed_batched", "fp32_mixed", 0),
+                                ("fp64_batched_one_workgroup_eigensolves", "fp64", nbx)):
+        dt, it, eigs = _batched_run(pw, systems, ncontexts, prec, single)
+        res[f"{label}_{ncontexts}_contexts"] = {"time_s": round(dt, 4), "iterations": it,
+                                               "max_abs_eig_diff_vs_fp64_loop_Ry": _err_vs(eigs, eig_loop)[0]}
     res["kpoints"] = len(systems)
     res["npw_range"] = [int(min(s.npw for s in systems)), int(max(s.npw for s in systems))]
     return res
@@ -373,6 +382,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="si216")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="fixed rules only (do not load the autotuner's cache abacus-develop_amd/tune_cache.json)")
     ap.add_argument("--no-next", action="store_true", help="skip the getvnl / psiToRho / DavSubspace / wfc-file lines")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (the reference for HBM-byte counters) and exit")
@@ -400,6 +411,12 @@ def main():
 
     global WORKLOAD
     WORKLOAD = args.workload
+    # autotuned plans (GEMM tiles/mapping/splits, eigensolver workgroups, FFT schedule and
+    # chunk) from the committed cache, measured beforehand by abacus-develop_amd/tune_workloads.py
+    tune = None
+    if not args.no_tune and os.path.exists(pw.TUNE_CACHE_FILE):
+        tune = {"cache": os.path.relpath(pw.TUNE_CACHE_FILE, ROOT), "entries": pw.tune_load()}
+        pw.tune_set_mode(pw.TUNE_CACHE)
     t_build = time.perf_counter()
     s = configs.make(args.workload)
     t_build = time.perf_counter() - t_build
@@ -432,9 +449,24 @@ def main():
     prof = ctx.profile()
     ctx.set_profiling(False)
 
-    # FP64 form beside it (not part of the timed steps)
+    # FP64 form beside it (not part of the timed steps), and the mixed form with the
+    # pre-phase's subspace products on the FP32 matrix cores too
     dt64, it64, eig64, st64 = run_solve(pw, ctx, ham, s, psi0, "fp64")
     print(f"[bench] fp64: {dt64:.3f} s, {it64} iterations", file=sys.stderr, flush=True)
+    dtx, itx, eigx, stx = run_solve(pw, ctx, ham, s, psi0, "fp32_mixed_x32")
+    print(f"[bench] fp32_mixed_x32: {dtx:.3f} s, {itx} iterations", file=sys.stderr, flush=True)
+
+    if tune is not None:
+        st = pw.tune_stats()
+        tune.update(hits=st["hits"], misses=st["misses"])
+        # the same FP32-mixed solve with the fixed rules (a Hamiltonian built without the table)
+        pw.tune_set_mode(pw.TUNE_FIXED)
+        ham0 = pw.HamiltPW(ctx, s)
+        dt0, it0, _, _ = run_solve(pw, ctx, ham0, s, psi0, "fp32_mixed")
+        ham0.close()
+        pw.tune_set_mode(pw.TUNE_CACHE)
+        tune["fixed_rules_fp32_mixed_s"] = round(dt0, 4)
+        print(f"[bench] fixed rules: {dt0:.3f} s, {it0} iterations", file=sys.stderr, flush=True)
 
     fb = flops_per_band(s)
     it_m, eig_m, st_m = mixed
@@ -447,7 +479,8 @@ def main():
         for label in ("plain", "detsign"):
             if label in fix:
                 r = np.array(fix[label]["eig"])
-                err[label] = (float(np.abs(eig64 - r).max()), float(np.abs(eig_m - r).max()))
+                err[label] = (float(np.abs(eig64 - r).max()), float(np.abs(eig_m - r).max()),
+                              float(np.abs(eigx - r).max()))
     out = {
         "metric": METRIC,
         "value": round(sum(times) / len(times), 4),
@@ -469,6 +502,10 @@ def main():
                        "fp64_switch_iter": st_m["fp64_switch_iter"], "hpsi_bands": st_m["hpsi_bands"],
                        "hpsi_gflops": round(st_m["hpsi_bands"] * fb / max(st_m["t_hpsi_s"], 1e-12) / 1e9, 1),
                        "max_residual": st_m["max_residual"]},
+        "fp32_mixed_x32": {"time_s": round(dtx, 4), "iterations": itx, "fp64_switch_iter": stx["fp64_switch_iter"],
+                           "speedup_vs_fp64": round(dt64 / dtx, 3),
+                           "max_abs_eig_diff_vs_fp64_Ry": float(np.abs(eigx - eig64).max()),
+                           "max_residual": stx["max_residual"]},
         "fp64": {"time_s": round(dt64, 4), "iterations": it64, "hpsi_bands": st64["hpsi_bands"],
                  "hpsi_gflops": round(st64["hpsi_bands"] * fb / max(st64["t_hpsi_s"], 1e-12) / 1e9, 1),
                  "max_residual": st64["max_residual"]},
@@ -476,9 +513,11 @@ def main():
                            "per band; the stick-pruned FFT passes perform fewer",
         "speedup_mixed_vs_fp64": round(dt64 / (sum(times) / len(times)), 3),
         "max_abs_eig_diff_mixed_vs_fp64_Ry": float(np.abs(eig_m - eig64).max()),
-        "eig_err_vs_reference_Ry": ({label: {"fp64": e[0], "fp32_mixed": e[1]} for label, e in err.items()}
+        "eig_err_vs_reference_Ry": ({label: {"fp64": e[0], "fp32_mixed": e[1], "fp32_mixed_x32": e[2]}
+                                     for label, e in err.items()}
                                     if err else None),
         "warmup_s": [round(t, 4) for t in warm],
+        "autotune": tune,
         "roofline": roof,
         "rooflines": [_roofline(k, v) for k, v in top[:8]],
         "kernels": {k: {"launches": v["launches"], "ms": round(v["ms"], 3),
