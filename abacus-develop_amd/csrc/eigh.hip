@@ -4,10 +4,10 @@
 // hcc to the host and calls LAPACK zheevx every Davidson iteration
 // (rocm/dngvd_op.hip.cu:146-170).  Same algorithm family as zheevx, all on the device:
 //   1. Householder reduction of the lower triangle to a real symmetric tridiagonal T
-//      (zhetrd 'L' form, unblocked) in one cooperative kernel: the trailing matrix is
-//      partitioned by columns over the workgroups and held in their registers and LDS
-//      (global memory for n too large for that), one exchange through a ring per column
-//      (or in one workgroup with an ordinary launch, for many small concurrent solves);
+//      (zhetrd 'L' form, unblocked) in one kernel: the trailing matrix is partitioned by
+//      columns over workgroups that are all resident at once and held in their registers
+//      and LDS (global memory for n too large for that), one exchange through a ring per
+//      column (or in one workgroup, for many small concurrent solves);
 //   2. bisection with Sturm counts for eigenvalues 0..nband-1 of T (dstebz form);
 //   3. inverse iteration with partial-pivoting tridiagonal LU and Gram-Schmidt inside
 //      clusters of close eigenvalues (dstein form);
@@ -16,6 +16,7 @@
 #include <cfloat>
 #include <cmath>
 #include <cstdlib>
+#include <deque>
 #include <mutex>
 #include <vector>
 
@@ -1151,28 +1152,50 @@ __global__ __launch_bounds__(1024) void k_chol_inv(int n, const double2* __restr
 
 }  // namespace
 
-// Cooperative launches of the reduction never overlap one another, whatever stream they are
-// on: each waits for the previous one.  Two partially resident cooperative grids could each
-// spin on workgroups that cannot start; one cooperative grid beside ordinary kernels always
-// gets its workgroups once those finish.
+// Launches of the multi-workgroup reduction, whose workgroups wait on each other's ring
+// values, may run at the same time on several streams only while all their workgroups fit
+// on the GPU together (each workgroup fits one CU): then none can wait on a workgroup that
+// cannot start, and ordinary kernels beside them always finish and free their CUs.  A
+// device-wide ledger keeps the reductions not known to have finished (launch order, width);
+// a new one whose width would take the running total past the CU count first makes its
+// stream wait for the oldest of them, as many as needed.  Every set of reductions that can
+// run together then has at most num_cu workgroups: a later one overlaps an earlier one only
+// if it did not wait for it, and then the earlier one was counted against its width.
 namespace {
-struct CoopChain {
+struct SpinLedger {
     std::mutex m;
-    hipEvent_t ev = nullptr;
-    bool recorded = false;
+    std::deque<std::pair<hipEvent_t, int>> live;   // (completion event, workgroups)
+    std::vector<hipEvent_t> pool;
 };
-CoopChain g_coop[64];
+SpinLedger g_ledger[64];
 
-void coop_launch(const void* fn, dim3 grid, dim3 block, void** args, unsigned lds, hipStream_t s) {
+void spin_launch(const void* fn, dim3 grid, dim3 block, void** args, unsigned lds, hipStream_t s, int num_cu) {
     int dev = 0;
     APW_HIP(hipGetDevice(&dev));
-    CoopChain& c = g_coop[dev & 63];
-    std::lock_guard<std::mutex> lock(c.m);
-    if (!c.ev) APW_HIP(hipEventCreateWithFlags(&c.ev, hipEventDisableTiming));
-    if (c.recorded) APW_HIP(hipStreamWaitEvent(s, c.ev, 0));
-    APW_HIP(hipLaunchCooperativeKernel(fn, grid, block, args, lds, s));
-    APW_HIP(hipEventRecord(c.ev, s));
-    c.recorded = true;
+    SpinLedger& L = g_ledger[dev & 63];
+    std::lock_guard<std::mutex> lock(L.m);
+    const int w = (int)grid.x;
+    APW_CHECK(w <= num_cu, "tridiagonal reduction: %d workgroups on %d CUs", w, num_cu);
+    while (!L.live.empty() && hipEventQuery(L.live.front().first) == hipSuccess) {
+        L.pool.push_back(L.live.front().first);
+        L.live.pop_front();
+    }
+    int used = 0;
+    for (const auto& g : L.live) used += g.second;
+    for (auto it = L.live.begin(); used + w > num_cu && it != L.live.end(); ++it) {
+        APW_HIP(hipStreamWaitEvent(s, it->first, 0));
+        used -= it->second;
+    }
+    APW_HIP(hipLaunchKernel(fn, grid, block, args, lds, s));
+    hipEvent_t ev;
+    if (!L.pool.empty()) {
+        ev = L.pool.back();
+        L.pool.pop_back();
+    } else {
+        APW_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    }
+    APW_HIP(hipEventRecord(ev, s));
+    L.live.emplace_back(ev, w);
 }
 }  // namespace
 
@@ -1293,7 +1316,7 @@ void heevx_lowest(hipStream_t s, EigWorkspace& ws, int num_cu, int n, int ldh, c
                 const double2* Ac = A;
                 unsigned* abortp = barp + 1;
                 void* args[] = {&Ac, &n, &ncol_r, &Vp, &taup, &dp, &ep, &ring, &abortp};
-                coop_launch(fn, dim3(wg), dim3(ntr), args, (unsigned)lds, s);
+                spin_launch(fn, dim3(wg), dim3(ntr), args, (unsigned)lds, s, num_cu);
             };
             // autotuned workgroup count
             const int tmode = tune_mode();
@@ -1344,7 +1367,7 @@ void heevx_lowest(hipStream_t s, EigWorkspace& ws, int num_cu, int n, int ldh, c
             APW_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
             unsigned* abortp = barp + 1;
             void* args[] = {&A, &n, &ncol, &Vp, &taup, &dp, &ep, &ring, &abortp};
-            coop_launch(fn, dim3(nwg), dim3(NT), args, (unsigned)lds, s);
+            spin_launch(fn, dim3(nwg), dim3(NT), args, (unsigned)lds, s, num_cu);
         }
     } else {
         // n == 1: T = [Re a00]; the real part is the first double of A

@@ -261,8 +261,14 @@ def measure_next(pw, ctx, ham, s, psi0_dev, eig_david, reps=5):
     dt = time.perf_counter() - t0
     st = dict(ds.stats)
     ds.close()
+    # residual norms of the returned vectors from an independent H|psi> (the solver reports none)
+    hp = ham.hPsi(psi, s.nband)
+    ctx.synchronize()
+    res = torch.linalg.vector_norm(hp[:, : s.npw] - torch.from_numpy(eig).cuda()[:, None] * psi[:, : s.npw], dim=1)
     out["dav_subspace"] = {"time_s": round(dt, 4), "iterations": it, "hpsi_bands": st["hpsi_bands"],
+                           "notconv": st["notconv"], "max_residual": float(res.max().item()),
                            "max_abs_eig_diff_vs_david_Ry": float(np.abs(eig - eig_david).max())}
+    del hp, res
     # ---- wavefunction file round trip (at most 32 bands: the file goes through the host filesystem)
     kc = s.lat.gcar(s.basis.kvec_d[s.ik])
     nb_io = min(s.nband, 32)
