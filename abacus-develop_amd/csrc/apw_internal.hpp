@@ -113,6 +113,11 @@ struct Prof {
     void prefill(size_t n);             // create events up front, so timed code creates none
     void resolve();
     void reset();
+    // A recorded event keeps the launch it was attached to alive, and with it that launch's
+    // device buffers: memory freed meanwhile is not returned until the event is recorded
+    // again or destroyed.  Before buffers that profiled launches used are freed, scrub
+    // resolves the pending timings and re-records every pooled event as a plain marker on s.
+    void scrub(hipStream_t s);
     ~Prof();
 };
 

@@ -55,6 +55,12 @@ void Prof::resolve() {
     }
 }
 
+void Prof::scrub(hipStream_t s) {
+    resolve();
+    for (hipEvent_t ev : pool) APW_HIP(hipEventRecord(ev, s));
+    APW_HIP(hipStreamSynchronize(s));
+}
+
 void Prof::reset() {
     resolve();
     entries.clear();

@@ -415,6 +415,7 @@ int apw_dav_subspace_destroy(apw_dav_subspace* dav) {
         if (!dav) return APW_OK;
         apw::DeviceGuard g(dav->d.ctx->device);
         (void)hipStreamSynchronize(dav->d.ctx->stream);
+        dav->d.ctx->prof.scrub(dav->d.ctx->stream);   // profiled launches must not pin the freed buffers
         delete dav;
         return APW_OK;
     });

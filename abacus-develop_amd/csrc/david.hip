@@ -491,6 +491,7 @@ int apw_david_destroy(apw_david* dav) {
         if (!dav) return APW_OK;
         apw::DeviceGuard g(dav->d.ctx->device);
         (void)hipStreamSynchronize(dav->d.ctx->stream);
+        dav->d.ctx->prof.scrub(dav->d.ctx->stream);   // profiled launches must not pin the freed buffers
         delete dav;
         return APW_OK;
     });

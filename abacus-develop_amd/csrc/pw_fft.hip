@@ -840,7 +840,8 @@ constexpr int odd_up(int n) { return n | 1; }
 
 template <class R, int N, int DIR>
 __global__ __launch_bounds__(zreg_threads(N)) void k_zreg(const double2* __restrict__ psi, int ld,
-                                                         const int* __restrict__ stick_ig, int ns,
+                                                         const int* __restrict__ stick_ig,
+                                                         const int4* __restrict__ zrun, int ns,
                                                          cplx<R>* __restrict__ S, double2* __restrict__ hpsi, int ldh,
                                                          const double* __restrict__ gk2, double tpiba2, int mode,
                                                          double scale, const cplx<R>* __restrict__ tw) {
@@ -855,14 +856,27 @@ __global__ __launch_bounds__(zreg_threads(N)) void k_zreg(const double2* __restr
     const bool sin = s < ns;
     const size_t band = blockIdx.y;
     const bool mid = g < NA;
-    const int* sig = stick_ig + (size_t)(sin ? s : 0) * N;
     C* Sl = S + (band * ns + (sin ? s : 0)) * (size_t)N;   // this stick's z line
     C v[NA], u[NB];
     unsigned po[NA];   // byte offsets into the band's sphere of z = NB n1 + g (BUF_OOB: no G)
-    static_for<0, NA>([&](auto n1) {
-        const int ig = sig[NB * APW_CI(n1) + g];
-        po[APW_CI(n1)] = (sin && ig >= 0) ? (unsigned)ig * 16u : BUF_OOB;
-    });
+    if (zrun) {
+        // the stick's plane waves as at most two runs of z, consecutive in the sphere:
+        // iz in [a0, a0 + la) holds ig = igA + iz - a0, iz in [b0, b0 + lb) holds igB + iz - b0
+        const int4 r = zrun[sin ? s : 0];
+        const int a0 = r.x & 0xffff, la = r.x >> 16, b0 = r.z & 0xffff, lb = r.z >> 16;
+        static_for<0, NA>([&](auto n1) {
+            const int iz = NB * APW_CI(n1) + g;
+            const int ig = (unsigned)(iz - a0) < (unsigned)la ? r.y + iz - a0
+                         : ((unsigned)(iz - b0) < (unsigned)lb ? r.w + iz - b0 : -1);
+            po[APW_CI(n1)] = (sin && ig >= 0) ? (unsigned)ig * 16u : BUF_OOB;
+        });
+    } else {
+        const int* sig = stick_ig + (size_t)(sin ? s : 0) * N;
+        static_for<0, NA>([&](auto n1) {
+            const int ig = sig[NB * APW_CI(n1) + g];
+            po[APW_CI(n1)] = (sin && ig >= 0) ? (unsigned)ig * 16u : BUF_OOB;
+        });
+    }
     double2 add[NA];   // DIR -1: what the result is added to (kinetic term or old hpsi)
     double kin[NA];
     const __amdgpu_buffer_rsrc_t hr = make_rsrc(hpsi + band * ldh, (unsigned)((size_t)ldh * 16));
@@ -1100,7 +1114,8 @@ void pass_forward_half(hipStream_t s, const PwGrid& g, R* S, R* X, int nb, doubl
 #define APW_ZREG_LAUNCH(N)                                                                                    \
     case N:                                                                                                  \
         APW_PLAUNCH(ps, (k_zreg<R, N, -1>), dim3((g.ns + ZL - 1) / ZL, nb), dim3(zreg_threads(N)), 0, s, psi, \
-                           ld, g.stick_ig.get(), g.ns, reinterpret_cast<C*>(S), hpsi, ldh, gk2, tpiba2, (int)mode, \
+                           ld, g.stick_ig.get(), g.zrun_ok ? g.zrun.get() : nullptr, g.ns, reinterpret_cast<C*>(S), \
+                           hpsi, ldh, gk2, tpiba2, (int)mode,                                               \
                            scale, tw_ptr<R>(g, g.twoff_z));                                                  \
         break;
     switch (zreg_len(g, ld, ldh, sched_of<R>(g))) {
@@ -1126,7 +1141,8 @@ void pass_backward_half(hipStream_t s, const PwGrid& g, const double2* psi, int 
 #define APW_ZREG_LAUNCH(N)                                                                                    \
     case N:                                                                                                  \
         APW_PLAUNCH(ps, (k_zreg<R, N, 1>), dim3((g.ns + ZL - 1) / ZL, nb), dim3(zreg_threads(N)), 0, s, psi,  \
-                           ld, g.stick_ig.get(), g.ns, reinterpret_cast<C*>(S), (double2*)nullptr, 0,          \
+                           ld, g.stick_ig.get(), g.zrun_ok ? g.zrun.get() : nullptr, g.ns, reinterpret_cast<C*>(S), \
+                           (double2*)nullptr, 0,                                                            \
                            (const double*)nullptr, 0.0, 0, 0.0, tw_ptr<R>(g, g.twoff_z));                    \
         break;
         switch (zreg_len(g, ld, 0, sched_of<R>(g))) {
@@ -1214,6 +1230,34 @@ void PwGrid::build(int nx_, int ny_, int nz_, int npw_, const int* ig2ixyz) {
         const int iz = (int)(idx % nz);
         APW_CHECK(h_ig[(size_t)s * nz + iz] < 0, "ig2ixyz maps two plane waves to box point %ld", idx);
         h_ig[(size_t)s * nz + iz] = ig;
+    }
+    // each stick's plane waves as runs of consecutive z that are consecutive in the sphere
+    // (the plane-wave order of PW_Basis_K gives one run, or two when z wraps through 0)
+    std::vector<int4> h_zrun(ns);
+    zrun_ok = nz <= 0x7fff;
+    for (int s = 0; s < ns && zrun_ok; ++s) {
+        int nrun = 0;
+        int st[2] = {0, 0}, ln[2] = {0, 0}, ig0[2] = {0, 0};
+        for (int iz = 0; iz < nz; ++iz) {
+            const int ig = h_ig[(size_t)s * nz + iz];
+            if (ig < 0) continue;
+            const bool cont = nrun > 0 && iz == st[nrun - 1] + ln[nrun - 1] && ig == ig0[nrun - 1] + ln[nrun - 1];
+            if (!cont) {
+                if (nrun == 2) {
+                    zrun_ok = false;
+                    break;
+                }
+                st[nrun] = iz;
+                ig0[nrun] = ig;
+                ++nrun;
+            }
+            ln[nrun - 1]++;
+        }
+        h_zrun[s] = make_int4(st[0] | (ln[0] << 16), ig0[0], st[1] | (ln[1] << 16), ig0[1]);
+    }
+    if (zrun_ok) {
+        zrun.reserve(ns);
+        APW_HIP(hipMemcpy(zrun.get(), h_zrun.data(), ns * sizeof(int4), hipMemcpyHostToDevice));
     }
     std::vector<int> h_slot_ix, h_slot_start, h_slot_cnt;
     for (int s = 0; s < ns; ++s) {

@@ -29,6 +29,9 @@ struct PwGrid {
     int nxr = 0;   // x slots (distinct ix of the sphere)
     FftPlan px, py, pz;
     DevBuf<int> stick_ig;     // ns * nz, -1 where no G
+    DevBuf<int4> zrun;        // ns: each stick's G as at most two runs of z consecutive in
+                              // the sphere (iz start | length << 16, ig of the start) x 2
+    bool zrun_ok = false;     // every stick is at most two such runs
     DevBuf<int> stick_iy;     // ns
     DevBuf<int> slot_ix;      // nxr
     DevBuf<int> slot_start;   // nxr (first stick of the slot; sticks sorted by ix, iy)

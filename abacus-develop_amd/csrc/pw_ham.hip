@@ -427,6 +427,7 @@ int apw_ham_destroy(apw_ham* ham) {
         if (!ham) return APW_OK;
         apw::DeviceGuard g(ham->h.ctx->device);
         (void)hipStreamSynchronize(ham->h.ctx->stream);
+        ham->h.ctx->prof.scrub(ham->h.ctx->stream);   // profiled launches must not pin the freed buffers
         delete ham;
         return APW_OK;
     });
