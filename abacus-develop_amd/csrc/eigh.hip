@@ -587,6 +587,284 @@ __global__ __launch_bounds__(NTR) void k_tridiag_reg(const double2* __restrict__
     }
 }
 
+// Householder tridiagonalisation (lower) for large n (up to 64 * NW * CH rows): the own
+// columns stay in registers for the whole reduction, their rows split over the NW waves of
+// the workgroup (wave w, lane l holds rows 64 (w + NW ch) + l, ch < CH, of every own column),
+// so the trailing matrix never leaves the register files of the grid (at n = 1920 it is
+// 59 MB, more than all LDS together).  Same ownership and ring as k_tridiag_reg; each lane
+// polls only its own rows of p_k and of column k + 1, and the sums over rows (v^H p, the
+// reflector norm, the projections onto v_{k+1}) are wave sums combined in LDS in a fixed
+// order.  Three workgroup barriers per column step.
+// LDS: y_k and v_k by row (2n), NC x NW projection partials, broadcast values, 2 NW sums.
+template <int CH, int NC, int NW>
+__global__ __launch_bounds__(NW * 64) void k_tridiag_rows(const double2* __restrict__ A, int n, int ncol,
+                                                         double2* __restrict__ V, double2* __restrict__ tau,
+                                                         double* __restrict__ d, double* __restrict__ e,
+                                                         double2* __restrict__ ring, unsigned* abort_flag) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    double2* yl = reinterpret_cast<double2*>(smem);   // y_k by row
+    double2* vl = yl + n;                              // v_k by row
+    double2* pp = vl + n;                              // pp[lc * NW + wave]: partial projections
+    double2* bc = pp + NC * NW;                        // v_k, p_k, column k+1 at row k+1; alpha
+    double* red = reinterpret_cast<double*>(bc + 4);   // [NW] v^H p partials, [NW] norm partials
+    const int nwg = (int)gridDim.x, bid = (int)blockIdx.x;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    double2 cols[NC][CH];
+#pragma unroll
+    for (int lc = 0; lc < NC; ++lc) {
+        const int c = bid + lc * nwg;
+#pragma unroll
+        for (int ch = 0; ch < CH; ++ch) {
+            const int i = 64 * (wave + NW * ch) + lane;
+            cols[lc][ch] = (lc < ncol && c < n && i < n) ? A[(size_t)c * n + i] : make_double2(0.0, 0.0);
+        }
+    }
+    double2 v[CH], y[CH], w[CH];   // own rows of v_k, y_k, and the column / v_{k+1}
+    // the owning lane of row i stores x at row i into *dst
+    auto own_put = [&](int i, const double2 (&x)[CH], double2* dst) {
+        const int j = i >> 6;
+        if (i < n && j % NW == wave && lane == (i & 63)) {
+#pragma unroll
+            for (int ch = 0; ch < CH; ++ch)
+                if (ch == j / NW) *dst = x[ch];
+        }
+    };
+    // reflector of w on rows > k0 (as in k_tridiag_reg); one workgroup barrier
+    auto reflector = [&](int k0, double2& tv, double& beta) {
+        double s2 = 0;
+#pragma unroll
+        for (int ch = 0; ch < CH; ++ch) {
+            const int i = 64 * (wave + NW * ch) + lane;
+            if (i > k0 + 1 && i < n) s2 += w[ch].x * w[ch].x + w[ch].y * w[ch].y;
+        }
+        s2 = wave_sum(s2);
+        if (lane == 0) red[NW + wave] = s2;
+        own_put(k0 + 1, w, bc + 3);
+        __syncthreads();
+        double xn2 = 0;
+        for (int q = 0; q < NW; ++q) xn2 += red[NW + q];
+        const double2 alpha = bc[3];
+        double2 sc;
+        if (xn2 == 0.0 && alpha.y == 0.0) {
+            tv = make_double2(0.0, 0.0);
+            beta = alpha.x;
+            sc = make_double2(0.0, 0.0);
+        } else {
+            const double nrm = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xn2);
+            beta = alpha.x >= 0 ? -nrm : nrm;
+            tv = make_double2((beta - alpha.x) / beta, -alpha.y / beta);
+            const double2 den = make_double2(alpha.x - beta, alpha.y);
+            const double dd = den.x * den.x + den.y * den.y;
+            sc = make_double2(den.x / dd, -den.y / dd);
+        }
+#pragma unroll
+        for (int ch = 0; ch < CH; ++ch) {
+            const int i = 64 * (wave + NW * ch) + lane;
+            w[ch] = (i == k0 + 1) ? make_double2(1.0, 0.0)
+                                  : ((i > k0 + 1 && i < n) ? cmul(w[ch], sc) : make_double2(0.0, 0.0));
+        }
+    };
+    // as in k_tridiag_reg, with (y_c, v_c) read from LDS; one workgroup barrier before the
+    // per-column sums over the waves are published
+    auto update_and_project = [&](int k, bool update) {
+        const __amdgpu_buffer_rsrc_t rs = ring_rsrc(ring + (size_t)(k + 1) * 2 * n, 2 * n);
+#pragma unroll
+        for (int lc = 0; lc < NC; ++lc) {
+            const int c = bid + lc * nwg;
+            if (lc < ncol && c > k + 1 && c < n) {   // uniform over the workgroup
+                double2 yc = make_double2(0.0, 0.0), vc = make_double2(0.0, 0.0);
+                if (update) {
+                    yc = yl[c];
+                    vc = vl[c];
+                }
+                double pr = 0, pi = 0;
+#pragma unroll
+                for (int ch = 0; ch < CH; ++ch) {
+                    const int i = 64 * (wave + NW * ch) + lane;
+                    if (i > k + 1 && i < n) {
+                        double2 a = cols[lc][ch];
+                        if (update) {
+                            const double2 a1 = cconjmul(yc, v[ch]);
+                            const double2 a2 = cconjmul(vc, y[ch]);
+                            a.x -= a1.x + a2.x;
+                            a.y -= a1.y + a2.y;
+                            cols[lc][ch] = a;
+                        }
+                        const double2 q = cconjmul(a, w[ch]);
+                        pr += q.x;
+                        pi += q.y;
+                    }
+                }
+                pr = wave_sum(pr);
+                pi = wave_sum(pi);
+                if (lane == 0) pp[lc * NW + wave] = make_double2(pr, pi);
+                if (c == k + 2) {
+#pragma unroll
+                    for (int ch = 0; ch < CH; ++ch) {
+                        const int i = 64 * (wave + NW * ch) + lane;
+                        if (i >= k + 2 && i < n) ring_put16(rs, n + (i - (k + 2)), cols[lc][ch]);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x < NC) {
+            const int lc = threadIdx.x, c = bid + lc * nwg;
+            if (lc < ncol && c > k + 1 && c < n) {
+                double2 s = make_double2(0.0, 0.0);
+                for (int q = 0; q < NW; ++q) {
+                    s.x += pp[lc * NW + q].x;
+                    s.y += pp[lc * NW + q].y;
+                }
+                ring_put16(rs, c - (k + 2), s);
+            }
+        }
+    };
+    // first reflector from column 0 (rows 1 .. n-1)
+    double2 t;
+    double beta;
+#pragma unroll
+    for (int ch = 0; ch < CH; ++ch) {
+        const int i = 64 * (wave + NW * ch) + lane;
+        w[ch] = (i >= 1 && i < n) ? A[i] : make_double2(0.0, 0.0);
+    }
+    reflector(0, t, beta);
+    if (bid == 0) {
+#pragma unroll
+        for (int ch = 0; ch < CH; ++ch) {
+            const int i = 64 * (wave + NW * ch) + lane;
+            if (i >= 1 && i < n) V[i] = w[ch];
+        }
+        if (threadIdx.x == 0) {
+            tau[0] = t;
+            d[0] = A[0].x;
+            e[0] = beta;
+        }
+    }
+    update_and_project(-1, false);
+#pragma unroll
+    for (int ch = 0; ch < CH; ++ch) v[ch] = w[ch];
+    for (int k = 0; k < n - 1; ++k) {
+        const int m = n - k - 1;
+        const __amdgpu_buffer_rsrc_t rs = ring_rsrc(ring + (size_t)k * 2 * n, 2 * n);
+        // ---- own rows of p_k (-> y) and of column k+1 (-> w) ----
+#pragma unroll
+        for (int ch = 0; ch < CH; ++ch) {
+            y[ch] = make_double2(0.0, 0.0);
+            w[ch] = make_double2(0.0, 0.0);
+        }
+        {
+            unsigned spins = 0, got = 0;
+            while (true) {
+                bool mine = true;
+#pragma unroll
+                for (int ch = 0; ch < CH; ++ch) {
+                    const int i = 64 * (wave + NW * ch) + lane;
+                    if (i > k && i < n) {
+                        double2 val;
+                        if (!((got >> (2 * ch)) & 1u)) {
+                            if (ring_get16(rs, i - (k + 1), val)) {
+                                y[ch] = val;
+                                got |= 1u << (2 * ch);
+                            } else {
+                                mine = false;
+                            }
+                        }
+                        if (!((got >> (2 * ch + 1)) & 1u)) {
+                            if (ring_get16(rs, n + (i - (k + 1)), val)) {
+                                w[ch] = val;
+                                got |= 1u << (2 * ch + 1);
+                            } else {
+                                mine = false;
+                            }
+                        }
+                    }
+                }
+                if (__all(mine)) break;
+                if ((++spins & 31u) == 0) {
+                    int st = 0;
+                    if (lane == 0) {
+                        st = __hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u;
+                        if (!st && spins > SPIN_LIMIT) {
+                            atomicExch(abort_flag, 1u);
+                            st = 1;
+                        }
+                    }
+                    if (__shfl(st, 0, 64)) break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+        // ---- mu = v^H p over the workgroup; y = tau p - (|tau|^2 mu / 2) v ----
+        double mr = 0;
+#pragma unroll
+        for (int ch = 0; ch < CH; ++ch) mr += cconjmul(v[ch], y[ch]).x;
+        mr = wave_sum(mr);
+        if (lane == 0) red[wave] = mr;
+        own_put(k + 1, v, bc + 0);
+        own_put(k + 1, y, bc + 1);
+        own_put(k + 1, w, bc + 2);
+        __syncthreads();
+        double mu = 0;
+        for (int q = 0; q < NW; ++q) mu += red[q];
+        const double c2 = 0.5 * (t.x * t.x + t.y * t.y) * mu;
+#pragma unroll
+        for (int ch = 0; ch < CH; ++ch) {
+            const double2 tp = cmul(t, y[ch]);
+            y[ch] = make_double2(tp.x - c2 * v[ch].x, tp.y - c2 * v[ch].y);
+        }
+        // ---- column k+1 after the rank-2 update ----
+        const double2 v0 = bc[0];
+        double2 y0 = cmul(t, bc[1]);
+        y0 = make_double2(y0.x - c2 * v0.x, y0.y - c2 * v0.y);
+#pragma unroll
+        for (int ch = 0; ch < CH; ++ch) {
+            const double2 a1 = cconjmul(v0, y[ch]);
+            const double2 a2 = cconjmul(y0, v[ch]);
+            w[ch].x -= a1.x + a2.x;
+            w[ch].y -= a1.y + a2.y;
+        }
+        double dk1;
+        {
+            const double2 a1 = cconjmul(v0, y0);
+            const double2 a2 = cconjmul(y0, v0);
+            dk1 = bc[2].x - (a1.x + a2.x);
+        }
+        if (m >= 2) {
+            // y_k and v_k by row, for the (y_c, v_c) of the own columns' updates
+#pragma unroll
+            for (int ch = 0; ch < CH; ++ch) {
+                const int i = 64 * (wave + NW * ch) + lane;
+                if (i > k && i < n) {
+                    yl[i] = y[ch];
+                    vl[i] = v[ch];
+                }
+            }
+            double2 tn;
+            double bn;
+            reflector(k + 1, tn, bn);
+            if (bid == 0) {
+#pragma unroll
+                for (int ch = 0; ch < CH; ++ch) {
+                    const int i = 64 * (wave + NW * ch) + lane;
+                    if (i >= k + 2 && i < n) V[(size_t)(k + 1) * n + i] = w[ch];
+                }
+                if (threadIdx.x == 0) {
+                    tau[k + 1] = tn;
+                    d[k + 1] = dk1;
+                    e[k + 1] = bn;
+                }
+            }
+            update_and_project(k, true);
+#pragma unroll
+            for (int ch = 0; ch < CH; ++ch) v[ch] = w[ch];
+            t = tn;
+        } else if (bid == 0 && threadIdx.x == 0) {
+            d[k + 1] = dk1;
+        }
+    }
+}
+
 // Sturm count: number of eigenvalues of T strictly below x, from the sign changes of the
 // characteristic polynomials of the leading blocks, p_i = (d_i - x) p_{i-1} - e_{i-1}^2 p_{i-2}
 // (division-free; the ratio q_i = p_i / p_{i-1} of the LAPACK form is never formed).  A zero
@@ -1347,6 +1625,61 @@ void heevx_lowest(hipStream_t s, EigWorkspace& ws, int num_cu, int n, int ldh, c
                 }
             }
             run_reg(w);
+        } else if (n <= 2048 && (n + 7) / 8 <= std::min(num_cu, 256)) {
+            // row-split register form: up to 8 own columns a workgroup in registers, 1024
+            // threads (16 waves x 2 row chunks) or 512 (8 waves x 4 chunks)
+            const int wmin = (n + 7) / 8, wmax = std::min(num_cu, 256);
+            int w = wmax, variant = 0;
+            const size_t nring = (size_t)2 * n * n;
+            ws.ring.reserve(2 * nring);
+            double2* ring = reinterpret_cast<double2*>(ws.ring.get());
+            auto run_rows = [&](int wg, int var) {
+                int ncol_r = (n + wg - 1) / wg;
+                const int nw = var == 0 ? 16 : 8;
+                const size_t lds = (size_t)2 * n * sizeof(double2) + (size_t)(8 * nw + 4) * sizeof(double2) +
+                                   (size_t)2 * nw * sizeof(double);
+                const void* fn = var == 0 ? reinterpret_cast<const void*>(k_tridiag_rows<2, 8, 16>)
+                                          : reinterpret_cast<const void*>(k_tridiag_rows<4, 8, 8>);
+                APW_CHECK(ncol_r <= 8 && n <= 64 * 32, "heevx: row-split form cannot take n=%d on %d workgroups", n, wg);
+                hipLaunchKernelGGL(k_fill_ring, dim3((unsigned)std::min<size_t>(1024, (nring + 255) / 256)), dim3(256),
+                                   0, s, ring, nring);
+                APW_LAUNCH_CHECK();
+                APW_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+                const double2* Ac = A;
+                unsigned* abortp = barp + 1;
+                void* args[] = {&Ac, &n, &ncol_r, &Vp, &taup, &dp, &ep, &ring, &abortp};
+                spin_launch(fn, dim3(wg), dim3(64 * nw), args, (unsigned)lds, s, num_cu);
+            };
+            const int tmode = tune_mode();
+            if (tmode > 0) {
+                char key[96];
+                std::snprintf(key, sizeof(key), "tridiag rows n=%d cu=%d", (n + 31) / 32 * 32, num_cu);
+                int a = 0, b = 0;
+                if (tune_find(key, a, b) && a >= wmin && a <= wmax && (b == 0 || b == 1)) {
+                    w = a;
+                    variant = b;
+                } else if (tmode == 2) {
+                    std::vector<int> cand;
+                    for (int c : {wmin, (n + 5) / 6, (n + 3) / 4, 192, 224, 256}) {
+                        c = std::max(wmin, std::min(c, wmax));
+                        if (std::find(cand.begin(), cand.end(), c) == cand.end()) cand.push_back(c);
+                    }
+                    double best = 1e300;
+                    for (int var = 0; var < 2; ++var)
+                        for (int c : cand) {
+                            const double us = time_on_stream(s, 2, [&] { run_rows(c, var); });
+                            if (us < best) {
+                                best = us;
+                                w = c;
+                                variant = var;
+                            }
+                        }
+                    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(256), 0, s, ws.bar.get(), 64 + 256);
+                    APW_LAUNCH_CHECK();
+                    tune_store(key, w, variant, best);
+                }
+            }
+            run_rows(w, variant);
         } else {
             // columns in LDS when the share fits, else in A itself (one column per
             // workgroup slot, L2/Infinity-Cache resident); the ring as above
