@@ -9,7 +9,7 @@
 //
 // Tile: 64 x 64, 128 x 64 or 64 x 128 (m x n) per workgroup, k step 8 (16 for the wide
 // tile) through two LDS stages, 4 waves in a 2 x 2 arrangement, each wave owning TM x TN MFMA 16x16 sub-tiles
-// with separate real and imaginary accumulators (a complex product = 4 real MFMAs).  Operand roles are swapped
+// with separate accumulators (a complex product = 3 real MFMAs in FP64, 4 in FP32).  Operand roles are swapped
 // (MFMA "A" <- op(B) indexed by n, MFMA "B" <- op(A) indexed by m) so that the 16 lanes of
 // a group hold 16 consecutive rows m of one column n: the column-major stores of C are then
 // 256-byte coalesced runs.  Accumulator register r of lane l holds D row drow(l, r) and
@@ -77,6 +77,10 @@ __global__ __launch_bounds__(NT) void zgemm_kernel(int M, int N, int K, int kchu
     using acc_t = typename Mfma<R>::acc_t;
     constexpr bool X32 = !std::is_same<R, S>::value;
     static_assert(!X32 || (std::is_same<R, float>::value && std::is_same<S, double>::value), "x32: f32 on f64");
+    // FP64: three real products per complex one (P1 = re re, P2 = im im, P3 = (re + im)(re + im);
+    // re = P1 - P2, im = P3 - P1 - P2), 3 MFMAs instead of 4; the error stays normwise, a few
+    // ulps of |a||b| per term
+    constexpr bool M3 = std::is_same<R, double>::value;
     constexpr int BM = 32 * TM, BN = 32 * TN;
     constexpr int NA = BM * BK / NT, NB = BN * BK / NT;
     constexpr int LA = BM + 1, LB = BN + 1;
@@ -120,7 +124,8 @@ __global__ __launch_bounds__(NT) void zgemm_kernel(int M, int N, int K, int kchu
     const int kbeg = sp * kchunk;
     const int kend = min(K, kbeg + kchunk);
 
-    acc_t acc_re[TN][TM], acc_im[TN][TM];
+    acc_t acc_re[TN][TM], acc_im[TN][TM];   // 4M: re, im; 3M: P1, P3
+    acc_t acc_p2[M3 ? TN : 1][M3 ? TM : 1];   // 3M: P2
     d4_t wide_re[X32 ? TN : 1][X32 ? TM : 1], wide_im[X32 ? TN : 1][X32 ? TM : 1];   // X32: FP64 sums
 #pragma unroll
     for (int a = 0; a < TN; ++a)
@@ -128,6 +133,7 @@ __global__ __launch_bounds__(NT) void zgemm_kernel(int M, int N, int K, int kchu
         for (int b = 0; b < TM; ++b) {
             acc_re[a][b] = acc_t{0, 0, 0, 0};
             acc_im[a][b] = acc_t{0, 0, 0, 0};
+            if constexpr (M3) acc_p2[a][b] = acc_t{0, 0, 0, 0};
             if constexpr (X32) {
                 wide_re[a][b] = d4_t{0, 0, 0, 0};
                 wide_im[a][b] = d4_t{0, 0, 0, 0};
@@ -235,15 +241,31 @@ __global__ __launch_bounds__(NT) void zgemm_kernel(int M, int N, int K, int kchu
                 yr[t] = As[st][0][kk][mm];
                 yi[t] = As[st][1][kk][mm];
             }
+            if constexpr (M3) {
+                R xs[TN], ys[TM];
 #pragma unroll
-            for (int tn = 0; tn < TN; ++tn)
+                for (int t = 0; t < TN; ++t) xs[t] = xr[t] + xi[t];
 #pragma unroll
-                for (int tm = 0; tm < TM; ++tm) {
-                    acc_re[tn][tm] = Mfma<R>::mma(xr[tn], yr[tm], acc_re[tn][tm]);
-                    acc_re[tn][tm] = Mfma<R>::mma(-xi[tn], yi[tm], acc_re[tn][tm]);
-                    acc_im[tn][tm] = Mfma<R>::mma(xr[tn], yi[tm], acc_im[tn][tm]);
-                    acc_im[tn][tm] = Mfma<R>::mma(xi[tn], yr[tm], acc_im[tn][tm]);
-                }
+                for (int t = 0; t < TM; ++t) ys[t] = yr[t] + yi[t];
+#pragma unroll
+                for (int tn = 0; tn < TN; ++tn)
+#pragma unroll
+                    for (int tm = 0; tm < TM; ++tm) {
+                        acc_re[tn][tm] = Mfma<R>::mma(xr[tn], yr[tm], acc_re[tn][tm]);
+                        acc_p2[tn][tm] = Mfma<R>::mma(xi[tn], yi[tm], acc_p2[tn][tm]);
+                        acc_im[tn][tm] = Mfma<R>::mma(xs[tn], ys[tm], acc_im[tn][tm]);
+                    }
+            } else {
+#pragma unroll
+                for (int tn = 0; tn < TN; ++tn)
+#pragma unroll
+                    for (int tm = 0; tm < TM; ++tm) {
+                        acc_re[tn][tm] = Mfma<R>::mma(xr[tn], yr[tm], acc_re[tn][tm]);
+                        acc_re[tn][tm] = Mfma<R>::mma(-xi[tn], yi[tm], acc_re[tn][tm]);
+                        acc_im[tn][tm] = Mfma<R>::mma(xr[tn], yi[tm], acc_im[tn][tm]);
+                        acc_im[tn][tm] = Mfma<R>::mma(xi[tn], yr[tm], acc_im[tn][tm]);
+                    }
+            }
         }
         // unconditional (one basic block with the MFMAs): past the end the masks are zero and
         // the loads read element 0
@@ -274,6 +296,10 @@ __global__ __launch_bounds__(NT) void zgemm_kernel(int M, int N, int K, int kchu
                 if constexpr (X32) {
                     xr = wide_re[tn][tm][r];
                     xi = wide_im[tn][tm][r];
+                } else if constexpr (M3) {
+                    const double p1 = acc_re[tn][tm][r], p2 = acc_p2[tn][tm][r];
+                    xr = p1 - p2;
+                    xi = (double)acc_im[tn][tm][r] - (p1 + p2);
                 } else {
                     xr = (double)acc_re[tn][tm][r];
                     xi = (double)acc_im[tn][tm][r];
