@@ -9,7 +9,7 @@
 //
 // Tile: 64 x 64, 128 x 64 or 64 x 128 (m x n) per workgroup, k step 8 (16 for the wide
 // tile) through two LDS stages, 4 waves in a 2 x 2 arrangement, each wave owning TM x TN MFMA 16x16 sub-tiles
-// with separate accumulators (a complex product = 3 real MFMAs in FP64, 4 in FP32).  Operand roles are swapped
+// with separate accumulators (a complex product = 3 real MFMAs; 4 in the x32 form).  Operand roles are swapped
 // (MFMA "A" <- op(B) indexed by n, MFMA "B" <- op(A) indexed by m) so that the 16 lanes of
 // a group hold 16 consecutive rows m of one column n: the column-major stores of C are then
 // 256-byte coalesced runs.  Accumulator register r of lane l holds D row drow(l, r) and
@@ -77,10 +77,10 @@ __global__ __launch_bounds__(NT) void zgemm_kernel(int M, int N, int K, int kchu
     using acc_t = typename Mfma<R>::acc_t;
     constexpr bool X32 = !std::is_same<R, S>::value;
     static_assert(!X32 || (std::is_same<R, float>::value && std::is_same<S, double>::value), "x32: f32 on f64");
-    // FP64: three real products per complex one (P1 = re re, P2 = im im, P3 = (re + im)(re + im);
+    // three real products per complex one (P1 = re re, P2 = im im, P3 = (re + im)(re + im);
     // re = P1 - P2, im = P3 - P1 - P2), 3 MFMAs instead of 4; the error stays normwise, a few
-    // ulps of |a||b| per term
-    constexpr bool M3 = std::is_same<R, double>::value;
+    // ulps of |a||b| per term.  The x32 form keeps four (its FP32 sums are folded into FP64)
+    constexpr bool M3 = !X32;
     constexpr int BM = 32 * TM, BN = 32 * TN;
     constexpr int NA = BM * BK / NT, NB = BN * BK / NT;
     constexpr int LA = BM + 1, LB = BN + 1;
