@@ -545,10 +545,13 @@ def test_david_kpoints_and_cells_mixed(torch, ctx, key):
     assert de < 1e-9
 
 
-@pytest.mark.parametrize("n,nband,ndim", [(200, 8, 4), (900, 150, 2)])
+@pytest.mark.parametrize("n,nband,ndim", [(200, 8, 4), (900, 150, 4)])
 def test_david_callback_dense(torch, ctx, n, nband, ndim):
-    """Reference-style test (diago_david_test.cpp): a dense random Hermitian H applied by a
-    user hpsi callback; eigenvalues vs LAPACK.  nband = 150 orthonormalises in two blocks."""
+    """Reference-style test (diago_david_test.cpp, david_ndim 4 as there): a dense random
+    Hermitian H applied by a user hpsi callback; eigenvalues vs LAPACK.  nband = 150
+    orthonormalises in two blocks.  (With david_ndim 2 the reference's stopping rule,
+    |delta e| < thr per band, stops early on a slowly converging top band for about one
+    random matrix in seven of this kind, whatever the rounding; with 4, none of 21 did.)"""
     from apw import pw
     rng = np.random.default_rng(100)
     H = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
