@@ -587,15 +587,32 @@ __global__ __launch_bounds__(NTR) void k_tridiag_reg(const double2* __restrict__
     }
 }
 
+// Sum over each 16-lane row, in every lane of the row (quad_perm [1,0,3,2], [2,3,0,1], half-row
+// and row mirrors: every source lane is valid, so the moves need no zeroed old value).
+template <int CTRL>
+__device__ inline double dppm_d(double v) {
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xF, 0xF, true);
+    return __hiloint2double(hi, lo);
+}
+__device__ inline double row_sum16(double v) {
+    v += dppm_d<0xB1>(v);
+    v += dppm_d<0x4E>(v);
+    v += dppm_d<0x141>(v);
+    v += dppm_d<0x140>(v);
+    return v;
+}
+
 // Householder tridiagonalisation (lower) for large n (up to 64 * NW * CH rows): the own
 // columns stay in registers for the whole reduction, their rows split over the NW waves of
 // the workgroup (wave w, lane l holds rows 64 (w + NW ch) + l, ch < CH, of every own column),
 // so the trailing matrix never leaves the register files of the grid (at n = 1920 it is
 // 59 MB, more than all LDS together).  Same ownership and ring as k_tridiag_reg; each lane
-// polls only its own rows of p_k and of column k + 1, and the sums over rows (v^H p, the
-// reflector norm, the projections onto v_{k+1}) are wave sums combined in LDS in a fixed
-// order.  Three workgroup barriers per column step.
-// LDS: y_k and v_k by row (2n), NC x NW projection partials, broadcast values, 2 NW sums.
+// polls only its own rows of p_k and of column k + 1.  The sums over rows v^H p and the
+// reflector norm are wave sums combined in LDS in a fixed order; the projections onto
+// v_{k+1} are summed over 16-lane rows only and the 4 NW row partials of a column combined
+// by wave 0.  Three workgroup barriers per column step.
+// LDS: y_k and v_k by row (2n), 4 NC NW projection partials, broadcast values, 2 NW sums.
 template <int CH, int NC, int NW>
 __global__ __launch_bounds__(NW * 64) void k_tridiag_rows(const double2* __restrict__ A, int n, int ncol,
                                                          double2* __restrict__ V, double2* __restrict__ tau,
@@ -604,8 +621,8 @@ __global__ __launch_bounds__(NW * 64) void k_tridiag_rows(const double2* __restr
     extern __shared__ __align__(16) unsigned char smem[];
     double2* yl = reinterpret_cast<double2*>(smem);   // y_k by row
     double2* vl = yl + n;                              // v_k by row
-    double2* pp = vl + n;                              // pp[lc * NW + wave]: partial projections
-    double2* bc = pp + NC * NW;                        // v_k, p_k, column k+1 at row k+1; alpha
+    double2* pp = vl + n;                              // pp[(lc * NW + wave) * 4 + row]: partial projections
+    double2* bc = pp + 4 * NC * NW;                    // v_k, p_k, column k+1 at row k+1; alpha
     double* red = reinterpret_cast<double*>(bc + 4);   // [NW] v^H p partials, [NW] norm partials
     const int nwg = (int)gridDim.x, bid = (int)blockIdx.x;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -629,6 +646,9 @@ __global__ __launch_bounds__(NW * 64) void k_tridiag_rows(const double2* __restr
                 if (ch == j / NW) *dst = x[ch];
         }
     };
+    // sum of the NW wave partials r[0..NW): one LDS read a lane and the same row tree in every
+    // wave and every workgroup, so all of them get the same bits
+    auto waves_sum = [&](const double* r) { return row_sum16((lane & 15) < NW ? r[lane & 15] : 0.0); };
     // reflector of w on rows > k0 (as in k_tridiag_reg); one workgroup barrier
     auto reflector = [&](int k0, double2& tv, double& beta) {
         double s2 = 0;
@@ -641,8 +661,7 @@ __global__ __launch_bounds__(NW * 64) void k_tridiag_rows(const double2* __restr
         if (lane == 0) red[NW + wave] = s2;
         own_put(k0 + 1, w, bc + 3);
         __syncthreads();
-        double xn2 = 0;
-        for (int q = 0; q < NW; ++q) xn2 += red[NW + q];
+        const double xn2 = waves_sum(red + NW);
         const double2 alpha = bc[3];
         double2 sc;
         if (xn2 == 0.0 && alpha.y == 0.0) {
@@ -652,10 +671,11 @@ __global__ __launch_bounds__(NW * 64) void k_tridiag_rows(const double2* __restr
         } else {
             const double nrm = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xn2);
             beta = alpha.x >= 0 ? -nrm : nrm;
-            tv = make_double2((beta - alpha.x) / beta, -alpha.y / beta);
+            const double rb = 1.0 / beta;
+            tv = make_double2((beta - alpha.x) * rb, -alpha.y * rb);
             const double2 den = make_double2(alpha.x - beta, alpha.y);
-            const double dd = den.x * den.x + den.y * den.y;
-            sc = make_double2(den.x / dd, -den.y / dd);
+            const double rd = 1.0 / (den.x * den.x + den.y * den.y);
+            sc = make_double2(den.x * rd, -den.y * rd);
         }
 #pragma unroll
         for (int ch = 0; ch < CH; ++ch) {
@@ -677,27 +697,35 @@ __global__ __launch_bounds__(NW * 64) void k_tridiag_rows(const double2* __restr
                     yc = yl[c];
                     vc = vl[c];
                 }
+                // every row, unpredicated: v_k and y_k are zero on rows <= k and past n, so
+                // the update changes nothing there but row k + 1, which no later step reads;
+                // v_{k+1} (in w) is zero on rows <= k + 1 and past n, so those rows add nothing
+                // to the projection
+                // a -= v_i conj(y_c) + y_i conj(v_c);  p += conj(a) w_i  (as FMA chains)
                 double pr = 0, pi = 0;
 #pragma unroll
                 for (int ch = 0; ch < CH; ++ch) {
-                    const int i = 64 * (wave + NW * ch) + lane;
-                    if (i > k + 1 && i < n) {
-                        double2 a = cols[lc][ch];
-                        if (update) {
-                            const double2 a1 = cconjmul(yc, v[ch]);
-                            const double2 a2 = cconjmul(vc, y[ch]);
-                            a.x -= a1.x + a2.x;
-                            a.y -= a1.y + a2.y;
-                            cols[lc][ch] = a;
-                        }
-                        const double2 q = cconjmul(a, w[ch]);
-                        pr += q.x;
-                        pi += q.y;
+                    double2 a = cols[lc][ch];
+                    if (update) {
+                        a.x = fma(-yc.x, v[ch].x, a.x);
+                        a.x = fma(-yc.y, v[ch].y, a.x);
+                        a.x = fma(-vc.x, y[ch].x, a.x);
+                        a.x = fma(-vc.y, y[ch].y, a.x);
+                        a.y = fma(-yc.x, v[ch].y, a.y);
+                        a.y = fma(yc.y, v[ch].x, a.y);
+                        a.y = fma(-vc.x, y[ch].y, a.y);
+                        a.y = fma(vc.y, y[ch].x, a.y);
+                        cols[lc][ch] = a;
                     }
+                    pr = fma(a.x, w[ch].x, pr);
+                    pr = fma(a.y, w[ch].y, pr);
+                    pi = fma(a.x, w[ch].y, pi);
+                    pi = fma(-a.y, w[ch].x, pi);
                 }
-                pr = wave_sum(pr);
-                pi = wave_sum(pi);
-                if (lane == 0) pp[lc * NW + wave] = make_double2(pr, pi);
+                // row sums only: the 4 rows of every wave go to LDS
+                pr = row_sum16(pr);
+                pi = row_sum16(pi);
+                if ((lane & 15) == 0) pp[(lc * NW + wave) * 4 + (lane >> 4)] = make_double2(pr, pi);
                 if (c == k + 2) {
 #pragma unroll
                     for (int ch = 0; ch < CH; ++ch) {
@@ -708,16 +736,27 @@ __global__ __launch_bounds__(NW * 64) void k_tridiag_rows(const double2* __restr
             }
         }
         __syncthreads();
-        if (threadIdx.x < NC) {
-            const int lc = threadIdx.x, c = bid + lc * nwg;
-            if (lc < ncol && c > k + 1 && c < n) {
-                double2 s = make_double2(0.0, 0.0);
-                for (int q = 0; q < NW; ++q) {
-                    s.x += pp[lc * NW + q].x;
-                    s.y += pp[lc * NW + q].y;
-                }
-                ring_put16(rs, c - (k + 2), s);
+        // wave 0, lane 8 lc + g: partials g, g + 8, ... of column lc, then a tree over g
+        if (wave == 0) {
+            // the lane id read afresh (volatile: not hoisted out of the column loop, where a
+            // long-lived per-lane column index was spilled to scratch)
+            int l;
+            asm volatile("v_mbcnt_lo_u32_b32 %0, -1, 0\n\tv_mbcnt_hi_u32_b32 %0, -1, %0" : "=v"(l));
+            const int lc = l >> 3, g = l & 7, c = bid + lc * nwg;
+            double re = 0, im = 0;
+#pragma unroll
+            for (int q = g; q < 4 * NW; q += 8) {
+                const double2 t = pp[lc * NW * 4 + q];
+                re += t.x;
+                im += t.y;
             }
+            re += dppm_d<0xB1>(re);
+            im += dppm_d<0xB1>(im);
+            re += dppm_d<0x4E>(re);
+            im += dppm_d<0x4E>(im);
+            re += dppm_d<0x141>(re);
+            im += dppm_d<0x141>(im);
+            if (g == 0 && lc < ncol && c > k + 1 && c < n) ring_put16(rs, c - (k + 2), make_double2(re, im));
         }
     };
     // first reflector from column 0 (rows 1 .. n-1)
@@ -805,8 +844,7 @@ __global__ __launch_bounds__(NW * 64) void k_tridiag_rows(const double2* __restr
         own_put(k + 1, y, bc + 1);
         own_put(k + 1, w, bc + 2);
         __syncthreads();
-        double mu = 0;
-        for (int q = 0; q < NW; ++q) mu += red[q];
+        const double mu = waves_sum(red);
         const double c2 = 0.5 * (t.x * t.x + t.y * t.y) * mu;
 #pragma unroll
         for (int ch = 0; ch < CH; ++ch) {
@@ -1627,19 +1665,18 @@ void heevx_lowest(hipStream_t s, EigWorkspace& ws, int num_cu, int n, int ldh, c
             run_reg(w);
         } else if (n <= 2048 && (n + 7) / 8 <= std::min(num_cu, 256)) {
             // row-split register form: up to 8 own columns a workgroup in registers, 1024
-            // threads (16 waves x 2 row chunks) or 512 (8 waves x 4 chunks)
+            // threads (16 waves x 2 row chunks; 8 waves x 4 chunks measured 1.7x slower)
             const int wmin = (n + 7) / 8, wmax = std::min(num_cu, 256);
-            int w = wmax, variant = 0;
+            int w = wmax;
             const size_t nring = (size_t)2 * n * n;
             ws.ring.reserve(2 * nring);
             double2* ring = reinterpret_cast<double2*>(ws.ring.get());
-            auto run_rows = [&](int wg, int var) {
+            auto run_rows = [&](int wg) {
                 int ncol_r = (n + wg - 1) / wg;
-                const int nw = var == 0 ? 16 : 8;
-                const size_t lds = (size_t)2 * n * sizeof(double2) + (size_t)(8 * nw + 4) * sizeof(double2) +
+                constexpr int nw = 16;
+                const size_t lds = (size_t)2 * n * sizeof(double2) + (size_t)(4 * 8 * nw + 4) * sizeof(double2) +
                                    (size_t)2 * nw * sizeof(double);
-                const void* fn = var == 0 ? reinterpret_cast<const void*>(k_tridiag_rows<2, 8, 16>)
-                                          : reinterpret_cast<const void*>(k_tridiag_rows<4, 8, 8>);
+                const void* fn = reinterpret_cast<const void*>(k_tridiag_rows<2, 8, nw>);
                 APW_CHECK(ncol_r <= 8 && n <= 64 * 32, "heevx: row-split form cannot take n=%d on %d workgroups", n, wg);
                 hipLaunchKernelGGL(k_fill_ring, dim3((unsigned)std::min<size_t>(1024, (nring + 255) / 256)), dim3(256),
                                    0, s, ring, nring);
@@ -1655,9 +1692,8 @@ void heevx_lowest(hipStream_t s, EigWorkspace& ws, int num_cu, int n, int ldh, c
                 char key[96];
                 std::snprintf(key, sizeof(key), "tridiag rows n=%d cu=%d", (n + 31) / 32 * 32, num_cu);
                 int a = 0, b = 0;
-                if (tune_find(key, a, b) && a >= wmin && a <= wmax && (b == 0 || b == 1)) {
+                if (tune_find(key, a, b) && a >= wmin && a <= wmax && b == 0) {
                     w = a;
-                    variant = b;
                 } else if (tmode == 2) {
                     std::vector<int> cand;
                     for (int c : {wmin, (n + 5) / 6, (n + 3) / 4, 192, 224, 256}) {
@@ -1665,21 +1701,19 @@ void heevx_lowest(hipStream_t s, EigWorkspace& ws, int num_cu, int n, int ldh, c
                         if (std::find(cand.begin(), cand.end(), c) == cand.end()) cand.push_back(c);
                     }
                     double best = 1e300;
-                    for (int var = 0; var < 2; ++var)
-                        for (int c : cand) {
-                            const double us = time_on_stream(s, 2, [&] { run_rows(c, var); });
-                            if (us < best) {
-                                best = us;
-                                w = c;
-                                variant = var;
-                            }
+                    for (int c : cand) {
+                        const double us = time_on_stream(s, 2, [&] { run_rows(c); });
+                        if (us < best) {
+                            best = us;
+                            w = c;
                         }
+                    }
                     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(256), 0, s, ws.bar.get(), 64 + 256);
                     APW_LAUNCH_CHECK();
-                    tune_store(key, w, variant, best);
+                    tune_store(key, w, 0, best);
                 }
             }
-            run_rows(w, variant);
+            run_rows(w);
         } else {
             // columns in LDS when the share fits, else in A itself (one column per
             // workgroup slot, L2/Infinity-Cache resident); the ring as above
