@@ -93,3 +93,60 @@ def test_tuned_fft_hpsi_matches_oracle(torch, tuner):
     ham.close()
     ham2.close()
     ctx.close()
+
+
+# every GEMM plan the autotuner can choose (tile/map variant 0..9, with and without a
+# k split), forced through the table, against numpy: FP64 and FP32 three-product forms and
+# the x32 form (which never takes the 128 x 64 tiles, variants 2, 3 and 8)
+NVAR = 10
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32", "x32"])
+@pytest.mark.parametrize("ta,m,n,k", [("N", 3001, 70, 100), ("C", 70, 90, 5003)])
+def test_every_gemm_plan(torch, tuner, tmp_path, prec, ta, m, n, k):
+    import json
+    pw = tuner
+    ctx = pw.Context(0)
+    rng = np.random.default_rng(11)
+    rows, cols = (k, m) if ta == "C" else (m, k)
+    A = rng.standard_normal((cols, rows)) + 1j * rng.standard_normal((cols, rows))
+    B = rng.standard_normal((n, k)) + 1j * rng.standard_normal((n, k))
+    Cm = rng.standard_normal((n, m)) + 1j * rng.standard_normal((n, m))
+    dt = torch.complex64 if prec == "f32" else torch.complex128
+    if prec == "f32":
+        A, B, Cm = (x.astype(np.complex64).astype(np.complex128) for x in (A, B, Cm))
+    dA, dB = torch.from_numpy(A).to(dt).cuda(), torch.from_numpy(B).to(dt).cuda()
+    alpha, beta = 0.7 - 0.2j, 0.3 + 0.1j
+    opa = A.T.conj().T if ta == "C" else A.T
+    want = alpha * (opa @ B.T) + beta * Cm.T
+    mag = np.abs(alpha) * (np.abs(opa) @ np.abs(B.T)) + np.abs(beta) * np.abs(Cm.T)
+    eps = 6e-8 if prec in ("f32", "x32") else np.finfo(float).eps
+    bound = 8 * np.sqrt(k) * eps * mag + 1e-13
+
+    def run():
+        out = torch.from_numpy(Cm).to(dt).cuda()
+        pw.gemm_op(ctx, ta, "N", m, n, k, alpha, dA, rows, dB, k, beta, out, m, x32=(prec == "x32"))
+        ctx.synchronize()
+        return out.cpu().numpy().astype(np.complex128).T
+
+    pw.tune_set_mode(pw.TUNE_MEASURE)
+    run()
+    (key, _, _), = pw.tune_entries()        # the table key of this shape
+    path = str(tmp_path / "plan.json")
+    tried = 0
+    for v in range(NVAR):
+        if prec == "x32" and v in (2, 3, 8):
+            continue
+        for occ in (0, 2):
+            with open(path, "w") as f:
+                f.write('{"version": 1, "entries": [\n' + json.dumps({"key": key, "choice": [v, occ], "us": 1.0}) + "\n]}\n")
+            pw.tune_clear()
+            assert pw.tune_load(path) == 1
+            pw.tune_set_mode(pw.TUNE_CACHE)
+            got = run()
+            assert pw.tune_stats()["hits"] >= 1
+            err = np.abs(got - want)
+            assert (err <= bound).all(), (prec, ta, v, occ, (err / bound).max())
+            tried += 1
+    print(prec, ta, key, tried, "plans")
+    ctx.close()
