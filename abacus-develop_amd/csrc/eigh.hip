@@ -352,6 +352,29 @@ __global__ __launch_bounds__(NT) void k_tridiag_fused(double2* __restrict__ A, i
     }
 }
 
+// Sum over each 16-lane row, in every lane of the row (quad_perm [1,0,3,2], [2,3,0,1], half-row
+// and row mirrors: every source lane is valid, so the moves need no zeroed old value).
+template <int CTRL>
+__device__ inline double dppm_d(double v) {
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xF, 0xF, true);
+    return __hiloint2double(hi, lo);
+}
+__device__ inline double row_sum16(double v) {
+    v += dppm_d<0xB1>(v);
+    v += dppm_d<0x4E>(v);
+    v += dppm_d<0x141>(v);
+    v += dppm_d<0x140>(v);
+    return v;
+}
+
+// sum over the 64 lanes, in every lane: the row tree above, then the four row sums read out
+// (fixed order: deterministic)
+__device__ inline double wave_sum_rows(double v) {
+    v = row_sum16(v);
+    return (lane_bcast(v, 0) + lane_bcast(v, 16)) + (lane_bcast(v, 32) + lane_bcast(v, 48));
+}
+
 // Value of a per-lane register array at absolute row i (lane i % 64, slot i / 64),
 // broadcast to the whole wave.
 template <int R>
@@ -437,21 +460,27 @@ __global__ __launch_bounds__(NTR) void k_tridiag_reg(const double2* __restrict__
             for (int r = 0; r < R; ++r) {
                 const int i = lane + 64 * r;
                 if (i > k + 1 && i < n) {
+                    // a -= v_i conj(y_c) + y_i conj(v_c);  p += conj(a) w_i  (as FMA chains)
                     double2 a = col[i];
                     if (update) {
-                        const double2 a1 = cconjmul(yc, v[r]);   // v_i conj(y_c)
-                        const double2 a2 = cconjmul(vc, y[r]);   // y_i conj(v_c)
-                        a.x -= a1.x + a2.x;
-                        a.y -= a1.y + a2.y;
+                        a.x = fma(-yc.x, v[r].x, a.x);
+                        a.x = fma(-yc.y, v[r].y, a.x);
+                        a.x = fma(-vc.x, y[r].x, a.x);
+                        a.x = fma(-vc.y, y[r].y, a.x);
+                        a.y = fma(-yc.x, v[r].y, a.y);
+                        a.y = fma(yc.y, v[r].x, a.y);
+                        a.y = fma(-vc.x, y[r].y, a.y);
+                        a.y = fma(vc.y, y[r].x, a.y);
                         col[i] = a;
                     }
-                    const double2 q = cconjmul(a, w[r]);
-                    pr += q.x;
-                    pi += q.y;
+                    pr = fma(a.x, w[r].x, pr);
+                    pr = fma(a.y, w[r].y, pr);
+                    pi = fma(a.x, w[r].y, pi);
+                    pi = fma(-a.y, w[r].x, pi);
                 }
             }
-            pr = wave_sum(pr);
-            pi = wave_sum(pi);
+            pr = wave_sum_rows(pr);
+            pi = wave_sum_rows(pi);
             if (k + 2 < n && lane == 0) ring_put16(rs, c - (k + 2), make_double2(pr, pi));
             if (c == k + 2) {
                 // owner: publish the updated column k + 2 (rows k+2 .. n-1)
@@ -585,22 +614,6 @@ __global__ __launch_bounds__(NTR) void k_tridiag_reg(const double2* __restrict__
             d[k + 1] = dk1;
         }
     }
-}
-
-// Sum over each 16-lane row, in every lane of the row (quad_perm [1,0,3,2], [2,3,0,1], half-row
-// and row mirrors: every source lane is valid, so the moves need no zeroed old value).
-template <int CTRL>
-__device__ inline double dppm_d(double v) {
-    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xF, 0xF, true);
-    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xF, 0xF, true);
-    return __hiloint2double(hi, lo);
-}
-__device__ inline double row_sum16(double v) {
-    v += dppm_d<0xB1>(v);
-    v += dppm_d<0x4E>(v);
-    v += dppm_d<0x141>(v);
-    v += dppm_d<0x140>(v);
-    return v;
 }
 
 // Householder tridiagonalisation (lower) for large n (up to 64 * NW * CH rows): the own
