@@ -244,6 +244,14 @@ constexpr int HEEVX_MAX = 2000;
 // eig: device double[nband]; vcc: device c128 (ld ldv), columns 0..nband-1 rows 0..n-1.
 void heevx_lowest(hipStream_t s, EigWorkspace& ws, int num_cu, int n, int ldh, const double* hcc,
                   int nband, double* eig, double* vcc, int ldv, Prof* prof = nullptr);
+// The same in two parts: the eigenvalues (reduction and bisection), then the eigenvectors
+// (inverse iteration and back-transform) of the bands j with need[j] != 0 (need: device int
+// [nband], nullptr = all; a cluster of close eigenvalues is computed whole when any of its
+// bands is needed).  Columns not computed keep their previous contents.
+void heevx_values(hipStream_t s, EigWorkspace& ws, int num_cu, int n, int ldh, const double* hcc, int nband,
+                  double* eig, Prof* prof = nullptr);
+void heevx_vectors(hipStream_t s, EigWorkspace& ws, int n, int nband, const double* eig, double* vcc, int ldv,
+                   const int* need, Prof* prof = nullptr);
 // Sizes ws for every n <= nmax, so heevx_lowest allocates nothing (a growing workspace
 // frees and allocates, and hipFree waits for the device).
 void heevx_reserve(EigWorkspace& ws, int nmax, int nband);

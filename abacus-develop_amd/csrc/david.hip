@@ -138,7 +138,7 @@ struct David {
     apw_ctx* ctx = nullptr;
     int nband = 0, dim = 0, ndim = 0, nbx = 0;
     DevBuf<double> pre, basis, hp, sp, hcc, vcc, eig, tmp, vc, lmat, resid;
-    DevBuf<int> unconv_d, status;
+    DevBuf<int> unconv_d, need_d, status;
     EigWorkspace ews;
     GemmWorkspace& gws() { return ctx->gws; }   // shared by the context: sized once
 
@@ -156,13 +156,15 @@ struct David {
     double last_resid = -1.0;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> hev;
     // pinned host block: [status 2 int][barrier words 2 uint][eig nband][resid nband][unconv nband int]
+    // [need nband int]
     unsigned char* hbuf = nullptr;
-    size_t hbuf_bytes() const { return 16 + (size_t)2 * nband * sizeof(double) + (size_t)nband * sizeof(int); }
+    size_t hbuf_bytes() const { return 16 + (size_t)2 * nband * sizeof(double) + (size_t)2 * nband * sizeof(int); }
     int* h_status() { return reinterpret_cast<int*>(hbuf); }
     unsigned* h_bar() { return reinterpret_cast<unsigned*>(hbuf + 8); }
     double* h_eig() { return reinterpret_cast<double*>(hbuf + 16); }
     double* h_res() { return h_eig() + nband; }
     int* h_unconv() { return reinterpret_cast<int*>(h_res() + nband); }
+    int* h_need() { return h_unconv() + nband; }
     ~David() {
         if (hbuf) (void)hipHostFree(hbuf);
     }
@@ -195,6 +197,7 @@ struct David {
         lmat.reserve((size_t)2 * nbx * nband);
         resid.reserve(nbx);
         unconv_d.reserve(nband);
+        need_d.reserve(nband);
         status.reserve(4);
         heevx_reserve(ews, nbx, nband);
         gemm_reserve(ctx->gws, ctx->num_cu);
@@ -340,6 +343,27 @@ struct David {
         ews.single_max = ctx->eig_single_max;
         heevx_lowest(st(), ews, ctx->num_cu, nbase, nbx, hcc.get(), nband, eig.get(), vcc.get(), nbx, &ctx->prof);
     }
+    // the eigenvalues only; the eigenvectors follow from solve_vectors once the host knows
+    // which bands are unconverged
+    void solve_values(int nbase) {
+        ews.single_max = ctx->eig_single_max;
+        heevx_values(st(), ews, ctx->num_cu, nbase, nbx, hcc.get(), nband, eig.get(), &ctx->prof);
+    }
+    // all eigenvectors (every column of vcc is read: psi = basis vcc and refresh), or only those
+    // of the unconverged bands (cal_grad reads vcc[:, unconv] alone)
+    void solve_vectors(int nbase, bool all, const std::vector<int>& unconv, int notconv) {
+        const int* need = nullptr;
+        if (!all) {
+            // the pinned need block is rewritten only after the next fetch_iteration has
+            // synchronised the stream, so this copy has completed by then
+            int* hn = h_need();
+            std::fill(hn, hn + nband, 0);
+            for (int m = 0; m < notconv; ++m) hn[unconv[m]] = 1;
+            APW_HIP(hipMemcpyAsync(need_d.get(), hn, nband * sizeof(int), hipMemcpyHostToDevice, st()));
+            need = need_d.get();
+        }
+        heevx_vectors(st(), ews, nbase, nband, eig.get(), vcc.get(), nbx, need, &ctx->prof);
+    }
 
     // hcc[nbase .. nbase+nc, 0 .. nbase+nc) = basis_new^H hpsi   (cal_elem, diago_david.cpp:626-717)
     void cal_elem(int nbase, int nc) {
@@ -395,7 +419,7 @@ struct David {
             cal_elem(nbase, notconv);
             const int nc_grad = notconv;
             nbase += notconv;
-            solve_subspace(nbase);
+            solve_values(nbase);
             fetch_iteration(switch_thr > 0 ? nc_grad : 0);
             const double* ehost = h_eig();
             double rmax = 0;
@@ -408,7 +432,9 @@ struct David {
                 eig_host[m] = ehost[m];
             }
             const bool leave_mixed = switch_thr > 0 && (rmax < switch_thr || notconv == 0);
-            if (!notconv || nbase + notconv > nbx || it == maxiter || leave_mixed) {
+            const bool full = !notconv || nbase + notconv > nbx || it == maxiter || leave_mixed;
+            solve_vectors(nbase, full, unconv, notconv);
+            if (full) {
                 // psi = basis vcc (:292-306)
                 gemm('N', 'N', dim, nband, nbase, 1.0, basis.get(), dim, vcc.get(), nbx, 0.0, psi, ldpsi);
                 if (ldpsi > dim) {

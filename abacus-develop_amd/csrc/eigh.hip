@@ -1038,7 +1038,7 @@ __device__ inline unsigned hash_u32(unsigned x) {
 // each), reduction scratch, the pivots (n ints).
 __global__ __launch_bounds__(NT) void k_invit(const double* __restrict__ d, const double* __restrict__ e, int n,
                                               int nband, const double* __restrict__ w, double* __restrict__ Z,
-                                              double ctol_rel) {
+                                              double ctol_rel, const int* __restrict__ need) {
     extern __shared__ __align__(16) unsigned char smem[];
     double* ds = reinterpret_cast<double*>(smem);
     double* es = ds + n;
@@ -1070,6 +1070,11 @@ __global__ __launch_bounds__(NT) void k_invit(const double* __restrict__ d, cons
     if (j0 > 0 && w[j0] - w[j0 - 1] <= ctol) return;   // not a cluster head (uniform)
     int j1 = j0;
     while (j1 + 1 < nband && w[j1 + 1] - w[j1] <= ctol) ++j1;
+    if (need) {   // the cluster is computed whole when any of its bands is needed (uniform)
+        bool any = false;
+        for (int j = j0; j <= j1; ++j) any = any || need[j] != 0;
+        if (!any) return;
+    }
     const double eps = DBL_EPSILON;
     const double pertol = 10.0 * eps * tn;
     const double tiny = eps * tn;
@@ -1247,11 +1252,12 @@ __global__ __launch_bounds__(NT) void k_invit(const double* __restrict__ d, cons
 // vcc[:, j] = Q z_j, Q = P_0 P_1 ... P_{n-2}; one workgroup per column.
 __global__ __launch_bounds__(NT) void k_backtransform(int n, const double2* __restrict__ V,
                                                       const double2* __restrict__ tau, const double* __restrict__ Z,
-                                                      double2* __restrict__ vcc, int ldv) {
+                                                      double2* __restrict__ vcc, int ldv, const int* __restrict__ need) {
     extern __shared__ __align__(16) unsigned char smem[];
     double2* z = reinterpret_cast<double2*>(smem);
     double* red = reinterpret_cast<double*>(z + n);
     const int j = blockIdx.x;
+    if (need && !need[j]) return;   // whole workgroup
     for (int i = threadIdx.x; i < n; i += NT) z[i] = make_double2(Z[(size_t)j * n + i], 0.0);
     __syncthreads();
     for (int k = n - 2; k >= 0; --k) {
@@ -1284,10 +1290,10 @@ template <int RPL, int D>
 __global__ __launch_bounds__(NT) void k_backtransform_w(int n, int nband, const double2* __restrict__ V,
                                                         const double2* __restrict__ tau,
                                                         const double* __restrict__ Z, double2* __restrict__ vcc,
-                                                        int ldv) {
+                                                        int ldv, const int* __restrict__ need) {
     const int lane = threadIdx.x & 63;
     const int j = blockIdx.x * (NT / 64) + (threadIdx.x >> 6);
-    if (j >= nband) return;   // whole wave; the kernel has no workgroup barrier
+    if (j >= nband || (need && !need[j])) return;   // whole wave; the kernel has no workgroup barrier
     double2 z[RPL], vb[D][RPL], tb[D];
 #pragma unroll
     for (int r = 0; r < RPL; ++r) {
@@ -1542,7 +1548,14 @@ void heevx_reserve(EigWorkspace& ws, int nmax, int nband) {
 
 void heevx_lowest(hipStream_t s, EigWorkspace& ws, int num_cu, int n, int ldh, const double* hcc, int nband,
                   double* eig, double* vcc, int ldv, Prof* prof) {
-    APW_CHECK(n >= 1 && nband >= 1 && nband <= n && ldh >= n && ldv >= n, "heevx: bad sizes n=%d nband=%d", n, nband);
+    APW_CHECK(ldv >= n, "heevx: ldv=%d below n=%d", ldv, n);
+    heevx_values(s, ws, num_cu, n, ldh, hcc, nband, eig, prof);
+    heevx_vectors(s, ws, n, nband, eig, vcc, ldv, nullptr, prof);
+}
+
+void heevx_values(hipStream_t s, EigWorkspace& ws, int num_cu, int n, int ldh, const double* hcc, int nband,
+                  double* eig, Prof* prof) {
+    APW_CHECK(n >= 1 && nband >= 1 && nband <= n && ldh >= n, "heevx: bad sizes n=%d nband=%d", n, nband);
     APW_CHECK(n <= HEEVX_MAX, "heevx: n=%d above %d (LDS of the inverse-iteration kernel)", n, HEEVX_MAX);
     ws.a.reserve((size_t)2 * n * n);
     ws.v.reserve((size_t)2 * n * n);
@@ -1759,13 +1772,18 @@ void heevx_lowest(hipStream_t s, EigWorkspace& ws, int num_cu, int n, int ldh, c
                        ws.e.get(), n, nband, eig, n > 1 ? ws.bar.get() + 1 : nullptr);
     APW_LAUNCH_CHECK();
     }
+}
+
+void heevx_vectors(hipStream_t s, EigWorkspace& ws, int n, int nband, const double* eig, double* vcc, int ldv,
+                   const int* need, Prof* prof) {
+    APW_CHECK(n >= 1 && nband >= 1 && nband <= n && ldv >= n, "heevx: bad sizes n=%d nband=%d", n, nband);
     {
     ProfScope ps(prof, s, "eig_invit", 0.0, 0.0);
     const size_t lds_inv = (size_t)8 * n * sizeof(double) + 64 * sizeof(double) + (size_t)n * sizeof(int);
     APW_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_invit), hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)lds_inv));
     hipLaunchKernelGGL(k_invit, dim3(nband), dim3(NT), lds_inv, s, ws.d.get(), ws.e.get(), n, nband, eig, ws.z.get(),
-                       1e-7);
+                       1e-7, need);
     APW_LAUNCH_CHECK();
     }
     ProfScope pb(prof, s, "eig_backtransform", 16.0 * n * (double)n / 2.0 + 16.0 * n * nband,
@@ -1775,26 +1793,26 @@ void heevx_lowest(hipStream_t s, EigWorkspace& ws, int num_cu, int n, int ldh, c
     double2* out = reinterpret_cast<double2*>(vcc);
     const dim3 gw((nband + NT / 64 - 1) / (NT / 64));
     if (n < 2) {
-        hipLaunchKernelGGL((k_backtransform_w<1, 4>), gw, dim3(NT), 0, s, n, nband, Vc, tc, ws.z.get(), out, ldv);
+        hipLaunchKernelGGL((k_backtransform_w<1, 4>), gw, dim3(NT), 0, s, n, nband, Vc, tc, ws.z.get(), out, ldv, need);
     } else if (n <= 128) {
-        hipLaunchKernelGGL((k_backtransform_w<2, 4>), gw, dim3(NT), 0, s, n, nband, Vc, tc, ws.z.get(), out, ldv);
+        hipLaunchKernelGGL((k_backtransform_w<2, 4>), gw, dim3(NT), 0, s, n, nband, Vc, tc, ws.z.get(), out, ldv, need);
     } else if (n <= 256) {
-        hipLaunchKernelGGL((k_backtransform_w<4, 4>), gw, dim3(NT), 0, s, n, nband, Vc, tc, ws.z.get(), out, ldv);
+        hipLaunchKernelGGL((k_backtransform_w<4, 4>), gw, dim3(NT), 0, s, n, nband, Vc, tc, ws.z.get(), out, ldv, need);
     } else if (n <= 384) {
-        hipLaunchKernelGGL((k_backtransform_w<6, 4>), gw, dim3(NT), 0, s, n, nband, Vc, tc, ws.z.get(), out, ldv);
+        hipLaunchKernelGGL((k_backtransform_w<6, 4>), gw, dim3(NT), 0, s, n, nband, Vc, tc, ws.z.get(), out, ldv, need);
     } else if (n <= 512) {
-        hipLaunchKernelGGL((k_backtransform_w<8, 4>), gw, dim3(NT), 0, s, n, nband, Vc, tc, ws.z.get(), out, ldv);
+        hipLaunchKernelGGL((k_backtransform_w<8, 4>), gw, dim3(NT), 0, s, n, nband, Vc, tc, ws.z.get(), out, ldv, need);
     } else if (n <= 640) {
-        hipLaunchKernelGGL((k_backtransform_w<10, 4>), gw, dim3(NT), 0, s, n, nband, Vc, tc, ws.z.get(), out, ldv);
+        hipLaunchKernelGGL((k_backtransform_w<10, 4>), gw, dim3(NT), 0, s, n, nband, Vc, tc, ws.z.get(), out, ldv, need);
     } else if (n <= 768) {
-        hipLaunchKernelGGL((k_backtransform_w<12, 3>), gw, dim3(NT), 0, s, n, nband, Vc, tc, ws.z.get(), out, ldv);
+        hipLaunchKernelGGL((k_backtransform_w<12, 3>), gw, dim3(NT), 0, s, n, nband, Vc, tc, ws.z.get(), out, ldv, need);
     } else if (n <= 1024) {
-        hipLaunchKernelGGL((k_backtransform_w<16, 2>), gw, dim3(NT), 0, s, n, nband, Vc, tc, ws.z.get(), out, ldv);
+        hipLaunchKernelGGL((k_backtransform_w<16, 2>), gw, dim3(NT), 0, s, n, nband, Vc, tc, ws.z.get(), out, ldv, need);
     } else {
         const size_t lds_bt = (size_t)n * sizeof(double2) + 64 * sizeof(double);
         APW_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_backtransform),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bt));
-        hipLaunchKernelGGL(k_backtransform, dim3(nband), dim3(NT), lds_bt, s, n, Vc, tc, ws.z.get(), out, ldv);
+        hipLaunchKernelGGL(k_backtransform, dim3(nband), dim3(NT), lds_bt, s, n, Vc, tc, ws.z.get(), out, ldv, need);
     }
     APW_LAUNCH_CHECK();
 }
