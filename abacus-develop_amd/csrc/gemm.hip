@@ -58,8 +58,8 @@ constexpr int BKMAX = 16, NT = 256;
 // op codes: 0 = N, 1 = T, 2 = C.  Each of the 4 waves (2 x 2) owns (16 TM) x (16 TN) of the
 // (32 TM) x (32 TN) tile.  Two LDS stages: while the MFMAs consume stage s, the k-tile
 // after the next is loaded into registers and the next one is written to stage s^1, so a
-// k step costs one workgroup barrier.  LDS rows are padded by one element so that the
-// k-fastest operand stores (16 lanes down a column) fall in distinct banks.
+// k step costs one workgroup barrier.  LDS rows are padded (16 / BK elements) so that the
+// k-fastest operand stores fall in distinct banks.
 // S: element type of A, B, C and the split-K partials.  S = R is the plain form.  R = float
 // with S = double is the FP32-MFMA form of an FP64 product: operands rounded to FP32 as
 // they are staged in LDS, FP32 accumulators folded into FP64 ones every XFOLD k steps
@@ -83,7 +83,11 @@ __global__ __launch_bounds__(NT) void zgemm_kernel(int M, int N, int K, int kchu
     constexpr bool M3 = !X32;
     constexpr int BM = 32 * TM, BN = 32 * TN;
     constexpr int NA = BM * BK / NT, NB = BN * BK / NT;
-    constexpr int LA = BM + 1, LB = BN + 1;
+    // row padding 16 / BK elements: a k-fastest staging store puts 16 lanes on BK consecutive
+    // k rows of 16 / BK columns, which then fall on 16 distinct bank pairs (with one element of
+    // padding and BK = 8, lanes 8..15 hit the banks of lanes 0..7); the m- or n-fastest stores
+    // and the MFMA operand reads stay conflict-free
+    constexpr int LA = BM + 16 / BK, LB = BN + 16 / BK;
     __shared__ R As[2][2][BK][LA];   // [stage][re, im][k][m]
     __shared__ R Bs[2][2][BK][LB];   // [stage][re, im][k][n]
 
