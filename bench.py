@@ -385,6 +385,37 @@ def measure_batched(pw, nk=64, ncontexts=4):
     return res
 
 
+def run_gemm_shapes():
+    """The tall update (basis x vcc: 228487 x 256 x 1024, NN) and the inner product
+    (basis^H hpsi: 1024 x 256 x 228487, CN) of the 216-atom solve at their largest, with the
+    tuned plans; prints one JSON line with each shape's mean time and algorithmic TFLOP/s."""
+    import torch
+    from apw import pw
+    if os.path.exists(pw.TUNE_CACHE_FILE):
+        pw.tune_load()
+        pw.tune_set_mode(pw.TUNE_CACHE)
+    ctx = pw.Context(0)
+    npw = 228487
+    out = {}
+    for ta, m, n, k in (("N", npw, 256, 1024), ("C", 1024, 256, npw)):
+        rows = m if ta == "N" else k
+        gen = torch.Generator(device="cuda").manual_seed(7)
+        A = torch.randn(k if ta == "N" else m, rows, dtype=torch.complex128, device="cuda", generator=gen)
+        B = torch.randn(n, k, dtype=torch.complex128, device="cuda", generator=gen)
+        C = torch.zeros(n, m, dtype=torch.complex128, device="cuda")
+        pw.gemm_op(ctx, ta, "N", m, n, k, 1.0, A, rows, B, k, 0.0, C, m)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(4):
+            pw.gemm_op(ctx, ta, "N", m, n, k, 1.0, A, rows, B, k, 0.0, C, m)
+        ctx.synchronize()
+        dt = (time.perf_counter() - t0) / 4
+        out[f"{ta}N {m}x{n}x{k}"] = {"ms": round(1e3 * dt, 3), "tflops_algorithmic": round(8.0 * m * n * k / dt / 1e12, 2)}
+        del A, B, C
+    ctx.close()
+    print(json.dumps({"gemm_shapes": out}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -397,6 +428,9 @@ def main():
     ap.add_argument("--no-next", action="store_true", help="skip the getvnl / psiToRho / DavSubspace / wfc-file lines")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (the reference for HBM-byte counters) and exit")
+    ap.add_argument("--gemm-shapes", action="store_true",
+                    help="only run the two largest FP64 GEMM shapes of the 216-atom solve with the tuned plans "
+                         "(4 launches each; for matrix-core counters) and exit")
     args = ap.parse_args()
     if args.copy_calibration:
         import torch
@@ -408,6 +442,9 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_calibration": {"bytes_read_per_copy": 8 * n, "bytes_written_per_copy": 8 * n,
                                                "copies": 3}}), flush=True)
+        return
+    if args.gemm_shapes:
+        run_gemm_shapes()
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
