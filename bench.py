@@ -55,7 +55,7 @@ def run_solve(pw, ctx, ham, s, psi0_dev, precision):
 
 # library profiler name -> kernel-name prefixes in the rocprofv3 traffic summary
 _PMC_FAMILY = {   # regular expressions over the kernel names of the committed traffic summary
-    "eig_tridiag": r"apw::k_tridiag_(reg|fused|lds)",
+    "eig_tridiag": r"apw::k_tridiag_(reg|rows|fused|lds)",
     # GEMMs: the path's updates are op(A) = N (tiled), its inner products op(A) = C (split-K)
     # (template arguments: MFMA type, storage type, op(A), ...)
     "zgemm_f64_tiled": r"apw::zgemm_kernel<double, double, 0, ",
