@@ -448,7 +448,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(k-points/cells shard across GPUs with no exchange inside Davidson, see DESIGN.md)",
+              "(k-points and cells would shard across GPUs with no exchange inside Davidson)",
               file=sys.stderr)
         sys.exit(2)
 
