@@ -454,14 +454,14 @@ void gemm_impl(hipStream_t s, GemmWorkspace& ws, int num_cu, char ta, char tb, i
     const bool inner = tiles_of(64, 64) < 2 * cus && k / 64 > 1;
     // Fixed rule (measured on the Davidson shapes): inner products (k >> m n) split over k
     // with 64 x 32 tiles when they cover n with fewer columns than 64 x 64 ones (n <= 32,
-    // 65..96), else 64 x 64; tall updates 128 x 64 (n a multiple of 64) or 128 x 32 in the
-    // XCD-aware order; wide 64 x 128; else 64 x 64.  FP32-MFMA on FP64 storage cannot hold
-    // the 128 x 64 tile's two accumulator sets: 128 x 32 instead.
+    // 65..96), else 64 x 64; tall updates 128 x 32 in the XCD-aware order (with three
+    // accumulator sets the 128 x 64 tile runs one wave per SIMD and lost: 7.1 vs 6.6 ms on
+    // 228487 x 256 x 1024; 64 x 64 made the 216-atom solve 3 % slower); wide 64 x 128; else
+    // 64 x 64.
     int variant, occ = 0;
     if (inner) variant = (n <= 32 || (n > 64 && n <= 96)) ? 1 : 0;
-    else if (tall) variant = n % 64 == 0 ? 3 : 4;
+    else if (tall) variant = 4;
     else variant = wide ? 5 : 0;
-    if (X32 && (variant == 2 || variant == 3)) variant = 4;
     if (inner) occ = VOCC[variant];
     const char* prec_tag = X32 ? "x32" : (std::is_same<R, double>::value ? "f64" : "f32");
     auto need_of = [&](int v, int oc) {
