@@ -334,7 +334,7 @@ def test_dnevx_vs_lapack(torch, ctx, n, nband, ld):
     assert np.abs(V.conj().T @ V - np.eye(nband)).max() < 1e-11
 
 
-@pytest.mark.parametrize("n,nband", [(2, 1), (48, 24), (192, 48), (384, 96), (640, 160)])
+@pytest.mark.parametrize("n,nband", [(2, 1), (48, 24), (97, 40), (98, 40), (192, 48), (384, 96), (640, 160)])
 def test_dnevx_single_workgroup(torch, n, nband):
     """The one-workgroup reduction (ordinary launch, for concurrent small solves): eigenpairs
     vs LAPACK, and the same eigenvectors (phases included) as the cooperative form."""
