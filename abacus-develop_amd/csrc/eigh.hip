@@ -1595,6 +1595,10 @@ void heevx_values(hipStream_t s, EigWorkspace& ws, int num_cu, int n, int ldh, c
             }
         }
         const int R = (n + 63) / 64;
+        // the row-split register form from n = 705 (measured faster than the LDS-column register
+        // form there: 4.3 vs 4.5 ms at n = 768, 6.2 vs 8.0 ms at n = 1024) when n / 8
+        // workgroups fit on the GPU; the LDS-column form below that, or to n = 1024 otherwise
+        const bool rows_fit = n <= 2048 && (n + 7) / 8 <= std::min(num_cu, 256);
         if (n <= ws.single_max) {
             // one workgroup owns every column (kept in the copy A) and exchanges with itself:
             // an ordinary launch, not ordered against other streams' reductions, so many small
@@ -1612,7 +1616,7 @@ void heevx_values(hipStream_t s, EigWorkspace& ws, int num_cu, int n, int ldh, c
             hipLaunchKernelGGL(k_tridiag_fused<true>, dim3(1), dim3(NT), lds, s, A, n, ncol1, Vp, taup, dp, ep, ring,
                                barp + 1);
             APW_LAUNCH_CHECK();
-        } else if (R <= 16) {
+        } else if (R <= 11 || (R <= 16 && !rows_fit)) {
             // register form: LDS = own columns + two gather buffers of 2n
             const size_t fixed_r = (size_t)4 * n * sizeof(double2);
             int w = std::max(1, std::min(n / 8, 16)), nc = 0;
@@ -1689,7 +1693,7 @@ void heevx_values(hipStream_t s, EigWorkspace& ws, int num_cu, int n, int ldh, c
                 }
             }
             run_reg(w);
-        } else if (n <= 2048 && (n + 7) / 8 <= std::min(num_cu, 256)) {
+        } else if (rows_fit) {
             // row-split register form: up to 8 own columns a workgroup in registers, 1024
             // threads (16 waves x 2 row chunks; 8 waves x 4 chunks measured 1.7x slower)
             const int wmin = (n + 7) / 8, wmax = std::min(num_cu, 256);
