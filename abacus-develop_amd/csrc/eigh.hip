@@ -5,9 +5,10 @@
 // (rocm/dngvd_op.hip.cu:146-170).  Same algorithm family as zheevx, all on the device:
 //   1. Householder reduction of the lower triangle to a real symmetric tridiagonal T
 //      (zhetrd 'L' form, unblocked) in one kernel: the trailing matrix is partitioned by
-//      columns over workgroups that are all resident at once and held in their registers
-//      and LDS (global memory for n too large for that), one exchange through a ring per
-//      column (or in one workgroup, for many small concurrent solves);
+//      columns over workgroups that are all resident at once, one exchange through a ring
+//      per column.  n <= 704: columns in LDS, vectors in registers; up to 2048: columns in
+//      registers, rows split over the waves of a workgroup; global memory only where neither
+//      fits the GPU (or in one workgroup, for many small concurrent solves);
 //   2. bisection with Sturm counts for eigenvalues 0..nband-1 of T (dstebz form);
 //   3. inverse iteration with partial-pivoting tridiagonal LU and Gram-Schmidt inside
 //      clusters of close eigenvalues (dstein form);
