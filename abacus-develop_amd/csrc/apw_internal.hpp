@@ -152,6 +152,10 @@ struct EigWorkspace {
     DevBuf<double> d, e;   // tridiagonal
     DevBuf<double> ring;   // 2n^2 c128 exchange ring of the reduction
     DevBuf<double> z;      // n x nband real eigenvectors of T
+    // blocked back-transform: the needed columns as c128 (n x nband), the Gram partials,
+    // triangular factors and two nb x nband products of the WY blocks, the column list
+    DevBuf<double> zc, wy;
+    DevBuf<int> cols;
     DevBuf<unsigned> bar;  // [1]: abort word of the exchange polls
     // n <= single_max: the reduction runs in one workgroup (ordinary launch, not ordered
     // against other streams); 0 = always the multi-workgroup cooperative forms
@@ -242,16 +246,18 @@ void gather_cols(hipStream_t s, int nbase, int nc, const double2* vcc, int nbx, 
 // Lowest nband eigenpairs of Hermitian hcc (n x n, lower triangle, ld ldh), n <= HEEVX_MAX.
 constexpr int HEEVX_MAX = 2000;
 // eig: device double[nband]; vcc: device c128 (ld ldv), columns 0..nband-1 rows 0..n-1.
-void heevx_lowest(hipStream_t s, EigWorkspace& ws, int num_cu, int n, int ldh, const double* hcc,
-                  int nband, double* eig, double* vcc, int ldv, Prof* prof = nullptr);
+// gws: split-K partials of the back-transform's products (large n).
+void heevx_lowest(hipStream_t s, EigWorkspace& ws, GemmWorkspace& gws, int num_cu, int n, int ldh,
+                  const double* hcc, int nband, double* eig, double* vcc, int ldv, Prof* prof = nullptr);
 // The same in two parts: the eigenvalues (reduction and bisection), then the eigenvectors
 // (inverse iteration and back-transform) of the bands j with need[j] != 0 (need: device int
-// [nband], nullptr = all; a cluster of close eigenvalues is computed whole when any of its
-// bands is needed).  Columns not computed keep their previous contents.
+// [nband], nullptr = all, with nneed the number of non-zero flags; a cluster of close
+// eigenvalues is computed whole when any of its bands is needed).  Columns not computed keep
+// their previous contents.
 void heevx_values(hipStream_t s, EigWorkspace& ws, int num_cu, int n, int ldh, const double* hcc, int nband,
                   double* eig, Prof* prof = nullptr);
-void heevx_vectors(hipStream_t s, EigWorkspace& ws, int n, int nband, const double* eig, double* vcc, int ldv,
-                   const int* need, Prof* prof = nullptr);
+void heevx_vectors(hipStream_t s, EigWorkspace& ws, GemmWorkspace& gws, int num_cu, int n, int nband,
+                   const double* eig, double* vcc, int ldv, const int* need, int nneed, Prof* prof = nullptr);
 // Sizes ws for every n <= nmax, so heevx_lowest allocates nothing (a growing workspace
 // frees and allocates, and hipFree waits for the device).
 void heevx_reserve(EigWorkspace& ws, int nmax, int nband);

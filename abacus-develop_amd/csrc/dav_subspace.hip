@@ -290,7 +290,8 @@ struct DavSub {
             APW_LAUNCH_CHECK();
         } else if (init) {
             ews.single_max = ctx->eig_single_max;
-            heevx_lowest(st(), ews, ctx->num_cu, nbase, nbx, hcc.get(), nband, eig.get(), vcc.get(), nbx, &ctx->prof);
+            heevx_lowest(st(), ews, gws(), ctx->num_cu, nbase, nbx, hcc.get(), nband, eig.get(), vcc.get(), nbx,
+                         &ctx->prof);
         } else {
             zero_status();
             ews.single_max = ctx->eig_single_max;

@@ -341,7 +341,8 @@ struct David {
 
     void solve_subspace(int nbase) {
         ews.single_max = ctx->eig_single_max;
-        heevx_lowest(st(), ews, ctx->num_cu, nbase, nbx, hcc.get(), nband, eig.get(), vcc.get(), nbx, &ctx->prof);
+        heevx_lowest(st(), ews, gws(), ctx->num_cu, nbase, nbx, hcc.get(), nband, eig.get(), vcc.get(), nbx,
+                     &ctx->prof);
     }
     // the eigenvalues only; the eigenvectors follow from solve_vectors once the host knows
     // which bands are unconverged
@@ -362,7 +363,8 @@ struct David {
             APW_HIP(hipMemcpyAsync(need_d.get(), hn, nband * sizeof(int), hipMemcpyHostToDevice, st()));
             need = need_d.get();
         }
-        heevx_vectors(st(), ews, nbase, nband, eig.get(), vcc.get(), nbx, need, &ctx->prof);
+        heevx_vectors(st(), ews, gws(), ctx->num_cu, nbase, nband, eig.get(), vcc.get(), nbx, need, all ? nband : notconv,
+                      &ctx->prof);
     }
 
     // hcc[nbase .. nbase+nc, 0 .. nbase+nc) = basis_new^H hpsi   (cal_elem, diago_david.cpp:626-717)

@@ -12,7 +12,9 @@
 //   2. bisection with Sturm counts for eigenvalues 0..nband-1 of T (dstebz form);
 //   3. inverse iteration with partial-pivoting tridiagonal LU and Gram-Schmidt inside
 //      clusters of close eigenvalues (dstein form);
-//   4. back-transformation z -> Q z with the stored reflectors (zunmtr form).
+//   4. back-transformation z -> Q z with the stored reflectors (zunmtr form): one wave per
+//      vector for n <= 512, else blocks of 64 reflectors in compact WY form (zlarft), two
+//      matrix-core products per block.
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
@@ -1250,39 +1252,6 @@ __global__ __launch_bounds__(NT) void k_invit(const double* __restrict__ d, cons
     }
 }
 
-// vcc[:, j] = Q z_j, Q = P_0 P_1 ... P_{n-2}; one workgroup per column.
-__global__ __launch_bounds__(NT) void k_backtransform(int n, const double2* __restrict__ V,
-                                                      const double2* __restrict__ tau, const double* __restrict__ Z,
-                                                      double2* __restrict__ vcc, int ldv, const int* __restrict__ need) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    double2* z = reinterpret_cast<double2*>(smem);
-    double* red = reinterpret_cast<double*>(z + n);
-    const int j = blockIdx.x;
-    if (need && !need[j]) return;   // whole workgroup
-    for (int i = threadIdx.x; i < n; i += NT) z[i] = make_double2(Z[(size_t)j * n + i], 0.0);
-    __syncthreads();
-    for (int k = n - 2; k >= 0; --k) {
-        const double2 t = tau[k];
-        if (t.x == 0.0 && t.y == 0.0) continue;
-        const double2* vk = V + (size_t)k * n;   // rows k+1..n-1
-        double sr = 0, si = 0;
-        for (int i = k + 1 + threadIdx.x; i < n; i += NT) {
-            const double2 q = cconjmul(vk[i], z[i]);
-            sr += q.x;
-            si += q.y;
-        }
-        const double2 dot = block_sum2(sr, si, red);
-        const double2 f = cmul(t, dot);
-        for (int i = k + 1 + threadIdx.x; i < n; i += NT) {
-            const double2 q = cmul(vk[i], f);
-            z[i].x -= q.x;
-            z[i].y -= q.y;
-        }
-        __syncthreads();
-    }
-    for (int i = threadIdx.x; i < n; i += NT) vcc[(size_t)j * ldv + i] = z[i];
-}
-
 // vcc[:, j] = Q z_j with one wave per column, no workgroup barriers: lane l holds rows
 // l + 64 r (r < RPL) of z in registers; reflectors are read straight from V into a ring of
 // D register sets, D - 1 reflectors ahead of their use, so each load has D - 1 reflector
@@ -1354,6 +1323,211 @@ __global__ __launch_bounds__(NT) void k_backtransform_w(int n, int nband, const 
     for (int r = 0; r < RPL; ++r) {
         const int i = lane + 64 * r;
         if (i < n) vcc[(size_t)j * ldv + i] = z[r];
+    }
+}
+
+// ---- blocked back-transform (zunmtr with compact WY blocks of zlarft's 'F','C' form) ----
+// Q = P_0 P_1 ... P_{n-2} is taken WY_NB reflectors at a time.  Block b holds reflectors
+// k0 .. k0 + nbk - 1 (k0 = b WY_NB): P_k0 ... P_{k0+nbk-1} = I - V_b T_b V_b^H, where V_b is
+// rows k0 + 1 .. n - 1 of those columns of V (zero above each reflector's first row) and
+// T_b is upper triangular:
+//   T(c, c) = tau_c,   T(r, c) = -tau_c sum_{q = r}^{c - 1} T(r, q) G(q, c)   (r < c),
+// with G = V_b^H V_b.  Q Z is Z <- Z - V_b (T_b (V_b^H Z)) for b from the last block down to
+// 0: two matrix-core products per block (with U_b = V_b T_b^H, W = U_b^H Z, Z -= V_b W)
+// instead of a chain of n dependent reflections.
+constexpr int WY_NB = 64;
+constexpr int WY_ROWS = 256;   // rows of V per workgroup of the Gram partials
+constexpr int WY_MIN_N = 513;   // from this n the blocked form is faster than one wave per column
+
+// cols[0 .. nneed) = the indices j with need[j] != 0, in order (need == nullptr: all);
+// entries up to mc that no flag fills are set to -1 (one wave).
+__global__ __launch_bounds__(64) void k_wy_cols(const int* __restrict__ need, int nband, int mc,
+                                                int* __restrict__ cols) {
+    const int lane = threadIdx.x;
+    int base = 0;
+    for (int j0 = 0; j0 < nband; j0 += 64) {
+        const int j = j0 + lane;
+        const bool f = j < nband && (need == nullptr || need[j] != 0);
+        const unsigned long long m = __ballot(f);
+        const int pos = base + (int)__popcll(m & ((1ull << lane) - 1ull));
+        if (f && pos < mc) cols[pos] = j;
+        base += (int)__popcll(m);
+    }
+    for (int c = base + lane; c < mc; c += 64) cols[c] = -1;
+}
+
+// zc[:, c] = (Z[:, cols[c]], 0)
+__global__ void k_wy_gather(int n, const int* __restrict__ cols, const double* __restrict__ Z,
+                            double2* __restrict__ zc) {
+    const int c = blockIdx.y, i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = cols[c];
+    if (i < n) zc[(size_t)c * n + i] = make_double2(j >= 0 ? Z[(size_t)j * n + i] : 0.0, 0.0);
+}
+
+// vcc[:, cols[c]] = zc[:, c]
+__global__ void k_wy_scatter(int n, const int* __restrict__ cols, const double2* __restrict__ zc,
+                             double2* __restrict__ vcc, int ldv) {
+    const int c = blockIdx.y, i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = cols[c];
+    if (i < n && j >= 0) vcc[(size_t)j * ldv + i] = zc[(size_t)c * n + i];
+}
+
+// rows k0 + 1 .. k of column k of V (above the reflector's first row, inside its block's
+// panel) set to zero; one workgroup per column k = 0 .. n - 2
+__global__ __launch_bounds__(WY_NB) void k_wy_zero(int n, double2* __restrict__ V) {
+    const int k = blockIdx.x;
+    const int i = k / WY_NB * WY_NB + 1 + (int)threadIdx.x;
+    if (i <= k) V[(size_t)k * n + i] = make_double2(0.0, 0.0);
+}
+
+// partial Gram matrices of the blocks: gp[b S + s] = sum over rows r0 + s WY_ROWS ... of
+// conj(V_b(i, q)) V_b(i, c), NB x NB row-major.  Thread (tq, tc) of a 16 x 16 grid keeps the
+// 4 x 4 entries (4 tq + q, 4 tc + c); 32 rows of the panel at a time are staged in LDS.
+template <int NB>
+__global__ __launch_bounds__(256) void k_wy_gram(int n, int S, const double2* __restrict__ V,
+                                                 double2* __restrict__ gp) {
+    static_assert(NB == 64, "the 16 x 16 thread grid covers 64 x 64");
+    __shared__ double2 tile[32][NB + 1];
+    const int b = blockIdx.x, s = blockIdx.y;
+    const int k0 = b * NB, nbk = min(NB, n - 1 - k0);
+    const int rs = k0 + 1 + s * WY_ROWS, re = min(n, rs + WY_ROWS);
+    const int tq = threadIdx.x >> 4, tc = threadIdx.x & 15;
+    double2 acc[4][4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[q][c] = make_double2(0.0, 0.0);
+    for (int i0 = rs; i0 < re; i0 += 32) {
+        for (int e = threadIdx.x; e < 32 * NB; e += 256) {
+            const int c = e >> 5, rr = e & 31, i = i0 + rr;
+            tile[rr][c] = (i < re && c < nbk && i > k0 + c) ? V[(size_t)(k0 + c) * n + i] : make_double2(0.0, 0.0);
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int rr = 0; rr < 32; ++rr) {
+            double2 a[4], bb[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) a[q] = tile[rr][4 * tq + q];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) bb[c] = tile[rr][4 * tc + c];
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    acc[q][c].x = fma(a[q].x, bb[c].x, fma(a[q].y, bb[c].y, acc[q][c].x));
+                    acc[q][c].y = fma(a[q].x, bb[c].y, fma(-a[q].y, bb[c].x, acc[q][c].y));
+                }
+        }
+        __syncthreads();
+    }
+    double2* g = gp + (size_t)(b * S + s) * NB * NB;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) g[(4 * tq + q) * NB + 4 * tc + c] = acc[q][c];
+}
+
+// T_b (NB x NB upper triangular) from the Gram partials, built by merging neighbouring
+// groups of reflectors: for groups 1 (rows a..a+s) and 2 (rows c = a+s .. c+s) with
+// factors T1 and T2, the merged factor has T12 = -T1 G12 T2, G12 = V1^H V2.  Six levels
+// (s = 1 .. 32) from T = diag(tau), two barriers each.  Result in LDS (Ts, ld NB + 1).
+template <int NB>
+__device__ void wy_tfactor(int b, int S, int nbk, int k0, const double2* __restrict__ gp,
+                           const double2* __restrict__ tau, double2 (*G)[NB + 1], double2 (*Ts)[NB + 1],
+                           double2* X) {
+    const int tid = threadIdx.x, nt = blockDim.x;
+    for (int e = tid; e < NB * NB; e += nt) {
+        double gx = 0, gy = 0;
+        for (int q = 0; q < S; ++q) {
+            const double2 t = gp[(size_t)(b * S + q) * NB * NB + e];
+            gx += t.x;
+            gy += t.y;
+        }
+        const int i = e / NB, j = e % NB;
+        G[i][j] = make_double2(gx, gy);
+        Ts[i][j] = (i == j && i < nbk) ? tau[k0 + i] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    for (int s = 1; s < NB; s *= 2) {
+        const int cnt = (NB / (2 * s)) * s * s;
+        // X_p = G12 T2: X(i, j) = sum_{q <= j} G(a + i, c + q) T(c + q, c + j)
+        for (int e = tid; e < cnt; e += nt) {
+            const int p = e / (s * s), i = (e / s) % s, j = e % s, a = 2 * s * p, c = a + s;
+            double x = 0, y = 0;
+#pragma unroll 4
+            for (int q = 0; q <= j; ++q) {
+                const double2 t = cmul(G[a + i][c + q], Ts[c + q][c + j]);
+                x += t.x;
+                y += t.y;
+            }
+            X[e] = make_double2(x, y);
+        }
+        __syncthreads();
+        // T12 = -T1 X: T(a + i, c + j) = -sum_{q >= i} T(a + i, a + q) X(q, j)
+        for (int e = tid; e < cnt; e += nt) {
+            const int p = e / (s * s), i = (e / s) % s, j = e % s, a = 2 * s * p, c = a + s;
+            double x = 0, y = 0;
+#pragma unroll 4
+            for (int q = i; q < s; ++q) {
+                const double2 t = cmul(Ts[a + i][a + q], X[p * s * s + q * s + j]);
+                x += t.x;
+                y += t.y;
+            }
+            Ts[a + i][c + j] = make_double2(-x, -y);
+        }
+        __syncthreads();
+    }
+}
+
+// T_b of every block (one workgroup each) into T (NB x NB per block, row-major)
+template <int NB>
+__global__ __launch_bounds__(256) void k_wy_tfac(int n, int S, const double2* __restrict__ gp,
+                                                 const double2* __restrict__ tau, double2* __restrict__ T) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    double2(*G)[NB + 1] = reinterpret_cast<double2(*)[NB + 1]>(smem);
+    double2(*Ts)[NB + 1] = G + NB;
+    double2* X = reinterpret_cast<double2*>(Ts + NB);   // NB * NB / 4 entries (32 s at s = NB / 2)
+    const int b = blockIdx.x, k0 = b * NB, nbk = min(NB, n - 1 - k0);
+    wy_tfactor<NB>(b, S, nbk, k0, gp, tau, G, Ts, X);
+    double2* Tb = T + (size_t)b * NB * NB;
+    for (int e = threadIdx.x; e < NB * NB; e += blockDim.x) Tb[e] = Ts[e / NB][e % NB];
+}
+
+// U_b = V_b T_b^H (rows k0 + 1 .. n - 1, stored where V_b is in V's layout), so that
+// T_b V_b^H Z = U_b^H Z.  Grid (blocks, 64-row chunks).  Thread (row i, wave w) computes
+// U(i, q) for the 16 q of wave w: T(q, c) is then wave-uniform in LDS, and a wave's loads
+// of V(i, c) are 64 consecutive rows of one column.
+template <int NB>
+__global__ __launch_bounds__(256) void k_wy_u(int n, const double2* __restrict__ T, const double2* __restrict__ V,
+                                              double2* __restrict__ U) {
+    static_assert(NB == 64, "four waves of 16 q each");
+    __shared__ double2 Ts[NB][NB + 1];
+    const int b = blockIdx.x;
+    const int k0 = b * NB, nbk = min(NB, n - 1 - k0), r0 = k0 + 1;
+    const int i0 = r0 + blockIdx.y * 64;
+    if (i0 >= n) return;   // whole workgroup: no barrier reached
+    const double2* Tb = T + (size_t)b * NB * NB;
+    for (int e = threadIdx.x; e < NB * NB; e += 256) Ts[e / NB][e % NB] = Tb[e];
+    __syncthreads();
+    const int r = threadIdx.x & 63, w = threadIdx.x >> 6, i = i0 + r;
+    const int ii = i < n ? i : n - 1;   // rows past n load a valid row, their results are not stored
+    double2 acc[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) acc[q] = make_double2(0.0, 0.0);
+#pragma unroll 4
+    for (int c = 16 * w; c < nbk; ++c) {
+        const double2 v = V[(size_t)(k0 + c) * n + ii];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const double2 t = Ts[16 * w + q][c];   // zero for c < q: T is upper triangular
+            acc[q].x = fma(v.x, t.x, fma(v.y, t.y, acc[q].x));
+            acc[q].y = fma(v.y, t.x, fma(-v.x, t.y, acc[q].y));
+        }
+    }
+    if (i < n) {
+#pragma unroll
+        for (int q = 0; q < 16; ++q)
+            if (16 * w + q < nbk) U[(size_t)(k0 + 16 * w + q) * n + i] = acc[q];
     }
 }
 
@@ -1535,6 +1709,68 @@ void spin_launch(const void* fn, dim3 grid, dim3 block, void** args, unsigned ld
 }
 }  // namespace
 
+// sizes of the blocked back-transform's work arrays (doubles) for n x mc
+struct WySizes {
+    int nblk, S;
+    size_t gp, t, w;
+    WySizes(int n, int mc) {
+        nblk = (n - 1 + WY_NB - 1) / WY_NB;
+        S = (n - 1 + WY_ROWS - 1) / WY_ROWS;
+        gp = (size_t)2 * nblk * S * WY_NB * WY_NB;
+        t = (size_t)2 * nblk * WY_NB * WY_NB;
+        w = (size_t)2 * WY_NB * mc;
+    }
+    size_t total() const { return gp + t + w; }
+};
+
+void wy_reserve(EigWorkspace& ws, int n, int mc) {
+    ws.a.reserve((size_t)2 * n * n);
+    ws.zc.reserve((size_t)2 * n * mc);
+    ws.wy.reserve(WySizes(n, mc).total());
+    ws.cols.reserve(mc);
+}
+
+// vcc[:, j] = Q z_j for the mc columns j with need[j] != 0 (all when need is null), by WY
+// blocks: gather, Gram partials, T_b, U_b = V_b T_b^H, two products per block, scatter.
+void backtransform_wy(hipStream_t s, EigWorkspace& ws, GemmWorkspace& gws, int num_cu, int n, int nband,
+                      const int* need, int mc, double* vcc, int ldv) {
+    wy_reserve(ws, n, mc);
+    const WySizes z(n, mc);
+    double2* V = reinterpret_cast<double2*>(ws.v.get());
+    const double2* tau = reinterpret_cast<const double2*>(ws.tau.get());
+    double2* zc = reinterpret_cast<double2*>(ws.zc.get());
+    double2* gp = reinterpret_cast<double2*>(ws.wy.get());
+    double2* U = reinterpret_cast<double2*>(ws.a.get());   // the reduction's copy of hcc is spent
+    double2* T = reinterpret_cast<double2*>(ws.wy.get() + z.gp);
+    double* W = ws.wy.get() + z.gp + z.t;
+    int* cols = ws.cols.get();
+    hipLaunchKernelGGL(k_wy_cols, dim3(1), dim3(64), 0, s, need, nband, mc, cols);
+    const dim3 gcol((n + NT - 1) / NT, mc);
+    hipLaunchKernelGGL(k_wy_gather, gcol, dim3(NT), 0, s, n, cols, ws.z.get(), zc);
+    hipLaunchKernelGGL(k_wy_zero, dim3(n - 1), dim3(WY_NB), 0, s, n, V);
+    hipLaunchKernelGGL(k_wy_gram<WY_NB>, dim3(z.nblk, z.S), dim3(256), 0, s, n, z.S, V, gp);
+    const size_t lds_t = (size_t)2 * WY_NB * (WY_NB + 1) * sizeof(double2) + (size_t)WY_NB * WY_NB / 4 * sizeof(double2);
+    APW_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_wy_tfac<WY_NB>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_t));
+    hipLaunchKernelGGL(k_wy_tfac<WY_NB>, dim3(z.nblk), dim3(256), lds_t, s, n, z.S, gp, tau, T);
+    hipLaunchKernelGGL(k_wy_u<WY_NB>, dim3(z.nblk, (n - 1 + 63) / 64), dim3(256), 0, s, n, T, V, U);
+    APW_LAUNCH_CHECK();
+    const double* Vd = ws.v.get();
+    const double* Ud = ws.a.get();
+    double* zd = ws.zc.get();
+    for (int b = z.nblk - 1; b >= 0; --b) {
+        const int k0 = b * WY_NB, nbk = std::min(WY_NB, n - 1 - k0), r0 = k0 + 1, L = n - r0;
+        const size_t off = 2 * ((size_t)k0 * n + r0);
+        // W = U_b^H Z = T_b V_b^H Z, then Z -= V_b W over rows r0 .. n - 1
+        gemm<double>(s, gws, num_cu, 'C', 'N', nbk, mc, L, 1.0, 0.0, Ud + off, n, zd + 2 * (size_t)r0, n, 0.0, 0.0,
+                     W, WY_NB);
+        gemm<double>(s, gws, num_cu, 'N', 'N', L, mc, nbk, -1.0, 0.0, Vd + off, n, W, WY_NB, 1.0, 0.0,
+                     zd + 2 * (size_t)r0, n);
+    }
+    hipLaunchKernelGGL(k_wy_scatter, gcol, dim3(NT), 0, s, n, cols, zc, reinterpret_cast<double2*>(vcc), ldv);
+    APW_LAUNCH_CHECK();
+}
+
 void heevx_reserve(EigWorkspace& ws, int nmax, int nband) {
     const size_t n = (size_t)std::max(nmax, 1);
     ws.a.reserve(2 * n * n);
@@ -1545,13 +1781,14 @@ void heevx_reserve(EigWorkspace& ws, int nmax, int nband) {
     ws.z.reserve(n * (size_t)std::max(nband, 1));
     ws.bar.reserve(64 + 256);
     ws.ring.reserve(2 * (2 * n * n));   // the larger of the two exchange rings below
+    if ((int)n >= WY_MIN_N) wy_reserve(ws, (int)n, std::max(nband, 1));
 }
 
-void heevx_lowest(hipStream_t s, EigWorkspace& ws, int num_cu, int n, int ldh, const double* hcc, int nband,
-                  double* eig, double* vcc, int ldv, Prof* prof) {
+void heevx_lowest(hipStream_t s, EigWorkspace& ws, GemmWorkspace& gws, int num_cu, int n, int ldh,
+                  const double* hcc, int nband, double* eig, double* vcc, int ldv, Prof* prof) {
     APW_CHECK(ldv >= n, "heevx: ldv=%d below n=%d", ldv, n);
     heevx_values(s, ws, num_cu, n, ldh, hcc, nband, eig, prof);
-    heevx_vectors(s, ws, n, nband, eig, vcc, ldv, nullptr, prof);
+    heevx_vectors(s, ws, gws, num_cu, n, nband, eig, vcc, ldv, nullptr, nband, prof);
 }
 
 void heevx_values(hipStream_t s, EigWorkspace& ws, int num_cu, int n, int ldh, const double* hcc, int nband,
@@ -1779,9 +2016,11 @@ void heevx_values(hipStream_t s, EigWorkspace& ws, int num_cu, int n, int ldh, c
     }
 }
 
-void heevx_vectors(hipStream_t s, EigWorkspace& ws, int n, int nband, const double* eig, double* vcc, int ldv,
-                   const int* need, Prof* prof) {
+void heevx_vectors(hipStream_t s, EigWorkspace& ws, GemmWorkspace& gws, int num_cu, int n, int nband,
+                   const double* eig, double* vcc, int ldv, const int* need, int nneed, Prof* prof) {
     APW_CHECK(n >= 1 && nband >= 1 && nband <= n && ldv >= n, "heevx: bad sizes n=%d nband=%d", n, nband);
+    const int mc = need ? nneed : nband;
+    APW_CHECK(mc >= 0 && mc <= nband, "heevx: %d needed eigenvectors of %d", mc, nband);
     {
     ProfScope ps(prof, s, "eig_invit", 0.0, 0.0);
     const size_t lds_inv = (size_t)8 * n * sizeof(double) + 64 * sizeof(double) + (size_t)n * sizeof(int);
@@ -1791,8 +2030,12 @@ void heevx_vectors(hipStream_t s, EigWorkspace& ws, int n, int nband, const doub
                        1e-7, need);
     APW_LAUNCH_CHECK();
     }
-    ProfScope pb(prof, s, "eig_backtransform", 16.0 * n * (double)n / 2.0 + 16.0 * n * nband,
-                 8.0 * n * (double)n * nband);
+    ProfScope pb(prof, s, "eig_backtransform", 16.0 * n * (double)n / 2.0 + 16.0 * n * mc,
+                 8.0 * n * (double)n * mc);
+    if (n >= WY_MIN_N) {
+        if (mc > 0) backtransform_wy(s, ws, gws, num_cu, n, nband, need, mc, vcc, ldv);
+        return;
+    }
     const double2* Vc = reinterpret_cast<const double2*>(ws.v.get());
     const double2* tc = reinterpret_cast<const double2*>(ws.tau.get());
     double2* out = reinterpret_cast<double2*>(vcc);
@@ -1805,19 +2048,9 @@ void heevx_vectors(hipStream_t s, EigWorkspace& ws, int n, int nband, const doub
         hipLaunchKernelGGL((k_backtransform_w<4, 4>), gw, dim3(NT), 0, s, n, nband, Vc, tc, ws.z.get(), out, ldv, need);
     } else if (n <= 384) {
         hipLaunchKernelGGL((k_backtransform_w<6, 4>), gw, dim3(NT), 0, s, n, nband, Vc, tc, ws.z.get(), out, ldv, need);
-    } else if (n <= 512) {
-        hipLaunchKernelGGL((k_backtransform_w<8, 4>), gw, dim3(NT), 0, s, n, nband, Vc, tc, ws.z.get(), out, ldv, need);
-    } else if (n <= 640) {
-        hipLaunchKernelGGL((k_backtransform_w<10, 4>), gw, dim3(NT), 0, s, n, nband, Vc, tc, ws.z.get(), out, ldv, need);
-    } else if (n <= 768) {
-        hipLaunchKernelGGL((k_backtransform_w<12, 3>), gw, dim3(NT), 0, s, n, nband, Vc, tc, ws.z.get(), out, ldv, need);
-    } else if (n <= 1024) {
-        hipLaunchKernelGGL((k_backtransform_w<16, 2>), gw, dim3(NT), 0, s, n, nband, Vc, tc, ws.z.get(), out, ldv, need);
     } else {
-        const size_t lds_bt = (size_t)n * sizeof(double2) + 64 * sizeof(double);
-        APW_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_backtransform),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bt));
-        hipLaunchKernelGGL(k_backtransform, dim3(nband), dim3(NT), lds_bt, s, n, Vc, tc, ws.z.get(), out, ldv, need);
+        static_assert(WY_MIN_N <= 513, "one wave per column holds at most 512 rows");
+        hipLaunchKernelGGL((k_backtransform_w<8, 4>), gw, dim3(NT), 0, s, n, nband, Vc, tc, ws.z.get(), out, ldv, need);
     }
     APW_LAUNCH_CHECK();
 }
@@ -1853,7 +2086,8 @@ extern "C" int apw_dnevx_op(apw_ctx* ctx, int nstart, int ldh, const double* hcc
         apw::DeviceGuard g(ctx->device);
         apw::EigWorkspace& ws = ctx->ews;
         ws.single_max = ctx->eig_single_max;
-        apw::heevx_lowest(ctx->stream, ws, ctx->num_cu, nstart, ldh, hcc, nbands, eigenvalue, vcc, ldh, &ctx->prof);
+        apw::heevx_lowest(ctx->stream, ws, ctx->gws, ctx->num_cu, nstart, ldh, hcc, nbands, eigenvalue, vcc, ldh,
+                          &ctx->prof);
         APW_HIP(hipStreamSynchronize(ctx->stream));
         unsigned flag[2] = {0, 0};
         if (nstart > 1) APW_HIP(hipMemcpy(flag, ws.bar.get(), sizeof(flag), hipMemcpyDeviceToHost));

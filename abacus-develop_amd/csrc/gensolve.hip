@@ -81,7 +81,7 @@ void hegvx_lowest(hipStream_t s, GemmWorkspace& gws, EigWorkspace& ews, GenWorks
     gemm<double>(s, gws, num_cu, 'N', 'N', n, n, n, 1.0, 0.0, hcc, ldh, w.x.get(), n, 0.0, 0.0, w.c.get(), n, prof);
     w.t2.reserve(2 * nn);
     gemm<double>(s, gws, num_cu, 'C', 'N', n, n, n, 1.0, 0.0, w.x.get(), n, w.c.get(), n, 0.0, 0.0, w.t2.get(), n, prof);
-    heevx_lowest(s, ews, num_cu, n, n, w.t2.get(), nband, eig, w.y.get(), n, prof);
+    heevx_lowest(s, ews, gws, num_cu, n, n, w.t2.get(), nband, eig, w.y.get(), n, prof);
     // x = U^-1 y
     gemm<double>(s, gws, num_cu, 'N', 'N', n, nband, n, 1.0, 0.0, w.x.get(), n, w.y.get(), n, 0.0, 0.0, vcc, ldv, prof);
 }

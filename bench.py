@@ -68,14 +68,16 @@ _PMC_FAMILY = {   # regular expressions over the kernel names of the committed t
     "fft_z_bwd_scatter": r"apw::k_zreg<\w+, \d+, 1>|apw::k_zbwd",
     "fft_z_fwd_gather": r"apw::k_zreg<\w+, \d+, -1>|apw::k_zfwd",
     "eig_bisect": r"apw::k_bisect", "eig_invit": r"apw::k_invit",
-    "eig_backtransform": r"apw::k_backtransform", "chol_inv": r"apw::k_chol_inv",
+    "eig_backtransform": r"apw::k_(backtransform|wy_)", "chol_inv": r"apw::k_chol_inv",
 }
 # kernels whose time is set by dependent latency chains, not by a byte or flop roof
 _LATENCY_BOUND = {
     "eig_tridiag": "latency-bound: one cross-workgroup exchange per column of the reduction",
     "eig_bisect": "latency-bound: sequential Sturm recurrences", "eig_invit": "latency-bound: sequential solves",
-    "eig_backtransform": "latency-bound: one wave reduction per reflector", "chol_inv": "latency-bound",
+    "chol_inv": "latency-bound",
 }
+_NOTES = dict(_LATENCY_BOUND, eig_backtransform="n > 512: WY blocks of 64 reflectors, two FP64 matrix-core "
+              "products each (8 n^2 flops per vector); n <= 512: one wave per vector")
 
 
 WORKLOAD = "si216"
@@ -100,11 +102,11 @@ def _pmc_traffic(name):
 
 def _roofline(name, e):
     avg_s = e["ms"] * 1e-3 / max(1, e["launches"])
-    if name.startswith(("zgemm", "cgemm")):
+    if name.startswith(("zgemm", "cgemm")) or name == "eig_backtransform":
         bound, unit = "mfma", "TFLOP/s"
         achieved = e["flops"] / max(1, e["launches"]) / avg_s / 1e12
         # zgemm_x32: c128 data on the FP32 matrix cores, priced against the FP32 peak
-        peak = PEAK_TFLOPS["f64" if name.startswith("zgemm_f64") else "f32"]
+        peak = PEAK_TFLOPS["f64" if name.startswith(("zgemm_f64", "eig_")) else "f32"]
     else:
         bound, unit = "hbm", "GB/s"
         achieved = e["bytes"] / max(1, e["launches"]) / avg_s / 1e9
@@ -112,8 +114,8 @@ def _roofline(name, e):
     out = {"bound": bound, "achieved": round(achieved, 3), "peak": peak, "unit": unit,
            "frac": round(achieved / peak, 4), "traffic": _pmc_traffic(name), "kernel": name,
            "avg_launch_us": round(avg_s * 1e6, 2), "launches": e["launches"]}
-    if name in _LATENCY_BOUND:
-        out["note"] = _LATENCY_BOUND[name]
+    if name in _NOTES:
+        out["note"] = _NOTES[name]
     return out
 
 

@@ -194,7 +194,8 @@ def test_fft_chain_reference_fixture(torch, ctx):
 # ------------------------------------------------------------------ GEMM
 @pytest.mark.parametrize("ta,tb,m,n,k", [("N", "N", 70, 33, 19), ("C", "N", 64, 48, 3001), ("N", "T", 300, 40, 64),
                                          ("T", "C", 17, 65, 129), ("C", "N", 160, 640, 51627),
-                                         ("N", "N", 51627, 160, 640), ("N", "C", 5, 7, 0)])
+                                         ("N", "N", 51627, 160, 640), ("N", "C", 5, 7, 0), ("C", "N", 1, 30, 1),
+                                         ("N", "N", 1, 30, 1)])
 def test_zgemm_vs_numpy(torch, ctx, ta, tb, m, n, k):
     from apw import pw
     rng = np.random.default_rng(m * 7 + n * 3 + k)
@@ -311,11 +312,13 @@ def test_psi_to_rho_vs_oracle(torch, ctx, name, nb):
 
 # ------------------------------------------------------------------ eigensolver
 @pytest.mark.parametrize("n,nband,ld", [(1, 1, 1), (2, 1, 3), (48, 24, 48), (200, 60, 256), (640, 160, 640),
-                                        (900, 225, 900), (1024, 64, 1024), (1100, 50, 1100), (1300, 40, 1300),
-                                        (1920, 480, 1920)])
+                                        (900, 225, 900), (1024, 64, 1024), (1026, 30, 1026), (1089, 100, 1100),
+                                        (1100, 50, 1100), (1300, 40, 1300), (1920, 480, 1920)])
 def test_dnevx_vs_lapack(torch, ctx, n, nband, ld):
     """Eigenpairs vs LAPACK, at every size class of the solver, up to the 216-atom
-    workload's full subspace (n = 1920, 480 bands)."""
+    workload's full subspace (n = 1920, 480 bands).  From n = 1025 the back-transform runs
+    in blocks of 64 reflectors: n = 1026 ends on a block of one reflector, n = 1089 on a
+    full block."""
     from apw import pw
     rng = np.random.default_rng(n)
     H = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
