@@ -68,8 +68,11 @@ _PMC_FAMILY = {   # regular expressions over the kernel names of the committed t
     "fft_z_bwd_scatter": r"apw::k_zreg<\w+, \d+, 1>|apw::k_zbwd",
     "fft_z_fwd_gather": r"apw::k_zreg<\w+, \d+, -1>|apw::k_zfwd",
     "eig_bisect": r"apw::k_bisect", "eig_invit": r"apw::k_invit",
-    "eig_backtransform": r"apw::k_(backtransform|wy_)", "chol_inv": r"apw::k_chol_inv",
+    "eig_backtransform": r"apw::k_(backtransform|wy_)|small-grid apw::(zgemm_kernel<double|splitk_reduce<double)",
+    "chol_inv": r"apw::k_chol_inv",
 }
+# entries made of several dispatches per launch: the dispatches that occur once per launch
+_PMC_CALLS = {"eig_backtransform": r"apw::k_(backtransform_w|wy_cols)"}
 # kernels whose time is set by dependent latency chains, not by a byte or flop roof
 _LATENCY_BOUND = {
     "eig_tridiag": "latency-bound: one cross-workgroup exchange per column of the reduction",
@@ -97,6 +100,8 @@ def _pmc_traffic(name):
         if re.match(_PMC_FAMILY[name], k):
             tot += (v["fetch_bytes"] + v["write_bytes"]) * v["dispatches"]
             cnt += v["dispatches"]
+    if name in _PMC_CALLS:
+        cnt = sum(v["dispatches"] for k, v in ks.items() if re.match(_PMC_CALLS[name], k))
     return round(tot / cnt) if cnt else None
 
 
