@@ -431,14 +431,16 @@ void gemm_impl(hipStream_t s, GemmWorkspace& ws, int num_cu, char ta, char tb, i
     // 0: 64 x 64, m-major; 1: 64 x 32, m-major; 2: 128 x 64, n-major; 3: 128 x 64 and 4: 128 x 32,
     // XCD-aware (each XCD takes whole m-tiles); 5: 64 x 128 with k step 16; 6: 64 x 64 and
     // 7: 64 x 32, XCD-aware split-K (the tiles of one k split share one XCD's L2); 8: 128 x 64
-    // and 9: 128 x 32 with whole m-tiles per XCD and a k step of 16 (half the barriers).
+    // and 9: 128 x 32 with whole m-tiles per XCD and a k step of 16 (half the barriers); 10: 64 x 64
+    // with a k step of 16 and 11: the same XCD-aware (split-K candidates: each staged column
+    // of a k-contiguous operand is 256 bytes instead of 128).
     // occ > 0 splits k so that every workgroup of the launch is resident at once (occ
     // workgroups per CU: floor of the resident slots over the tiles, at least 64 of k per
     // split; a launch with a partial last round of workgroups ran up to 1.5x longer); occ = 0
     // does not split.
-    static const int VBM[10] = {64, 64, 128, 128, 128, 64, 64, 64, 128, 128};
-    static const int VBN[10] = {64, 32, 64, 64, 32, 128, 64, 32, 64, 32};
-    static const int VOCC[10] = {3, 6, 3, 3, 3, 2, 3, 6, 1, 1};   // resident workgroups per CU
+    static const int VBM[12] = {64, 64, 128, 128, 128, 64, 64, 64, 128, 128, 64, 64};
+    static const int VBN[12] = {64, 32, 64, 64, 32, 128, 64, 32, 64, 32, 64, 64};
+    static const int VOCC[12] = {3, 6, 3, 3, 3, 2, 3, 6, 1, 1, 2, 2};   // resident workgroups per CU
     const int cus = std::max(num_cu, 64);
     auto tiles_of = [&](int bm, int bn) { return (long)((m + bm - 1) / bm) * ((n + bn - 1) / bn); };
     auto splits_of = [&](int v, int occ) {
@@ -517,6 +519,10 @@ void gemm_impl(hipStream_t s, GemmWorkspace& ws, int num_cu, char ta, char tb, i
                                                 alpha_im, b_re, b_im, part); break;                                 \
             case 9: launch<R, S, X, Y, 4, 1, 16, 2>(ps, s, splits, m, n, k, kchunk, A, lda, B, ldb, Cp, ldcp, alpha_re, \
                                                 alpha_im, b_re, b_im, part); break;                                 \
+            case 10: launch<R, S, X, Y, 2, 2, 16, 0>(ps, s, splits, m, n, k, kchunk, A, lda, B, ldb, Cp, ldcp, alpha_re, \
+                                                 alpha_im, b_re, b_im, part); break;                                \
+            case 11: launch<R, S, X, Y, 2, 2, 16, 3>(ps, s, splits, m, n, k, kchunk, A, lda, B, ldb, Cp, ldcp, alpha_re, \
+                                                 alpha_im, b_re, b_im, part); break;                                \
             default: launch<R, S, X, Y, 2, 2, 8, 0>(ps, s, splits, m, n, k, kchunk, A, lda, B, ldb, Cp, ldcp, alpha_re, \
                                                 alpha_im, b_re, b_im, part); break;                                 \
         }                                                                                                           \
@@ -547,7 +553,7 @@ void gemm_impl(hipStream_t s, GemmWorkspace& ws, int num_cu, char ta, char tb, i
         std::snprintf(key, sizeof(key), "gemm %s %c%c m=%d n=%d k=%d cu=%d", prec_tag, "NTC"[oa], "NTC"[ob],
                       bucket(m), bucket(n), bucket(k), cus);
         int a = 0, b = 0;
-        if (tune_find(key, a, b) && a >= 0 && a < 10 && b >= 0 && b <= 6 && !(X32 && (a == 2 || a == 3 || a == 8))) {
+        if (tune_find(key, a, b) && a >= 0 && a < 12 && b >= 0 && b <= 6 && !(X32 && (a == 2 || a == 3 || a == 8))) {
             variant = a;
             occ = b;
         } else if (tmode == 2) {
@@ -555,7 +561,7 @@ void gemm_impl(hipStream_t s, GemmWorkspace& ws, int num_cu, char ta, char tb, i
             // split rule; timed into a scratch C (beta 0) so the caller's C is untouched
             std::vector<std::pair<int, int>> cand;
             if (inner) {
-                for (int v : {0, 1, 6, 7})
+                for (int v : {0, 1, 6, 7, 10, 11})
                     for (int oc : {1, 2, 3, 4, 6}) cand.emplace_back(v, oc);
             } else if (tall) {
                 for (int v : {3, 4, 0, 2, 8, 9}) cand.emplace_back(v, 0);
