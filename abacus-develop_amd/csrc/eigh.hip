@@ -1037,8 +1037,10 @@ __device__ inline unsigned hash_u32(unsigned x) {
 // and the two triangular solves are sequential (thread 0); their recurrences carry the
 // running values in registers and read the next operands from LDS one step ahead, so the
 // chain waits on arithmetic, not on LDS round trips.
-// LDS: d, e, the iterate xv, the LU factors A (diagonal), IA = 1/A, B, C, DD (n doubles
-// each), reduction scratch, the pivots (n ints).
+// LDS: d and e, which the LU overwrites in place with IA = 1 / pivot and B (step k writes
+// slot k after its reads of slots k + 1 and k + 2; reloaded for the next member of a
+// cluster), the iterate xv, the LU factors C and DD (n doubles each), reduction scratch and
+// the pivots (n bytes): under 80 KB up to n = 1980, two workgroups per CU.
 __global__ __launch_bounds__(NT) void k_invit(const double* __restrict__ d, const double* __restrict__ e, int n,
                                               int nband, const double* __restrict__ w, double* __restrict__ Z,
                                               double ctol_rel, const int* __restrict__ need) {
@@ -1046,13 +1048,12 @@ __global__ __launch_bounds__(NT) void k_invit(const double* __restrict__ d, cons
     double* ds = reinterpret_cast<double*>(smem);
     double* es = ds + n;
     double* xv = es + n;
-    double* A = xv + n;
-    double* IA = A + n;
-    double* B = IA + n;
-    double* C = B + n;
+    double* C = xv + n;
     double* DD = C + n;
     double* red = DD + n;
-    int* piv = reinterpret_cast<int*>(red + 64);
+    unsigned char* piv = reinterpret_cast<unsigned char*>(red + 64);
+    double* IA = ds;   // written over d by the LU
+    double* B = es;    // written over e by the LU
     const int j0 = blockIdx.x;
     // ||T||_1 bound (Gershgorin), block-parallel
     double tn = 0;
@@ -1086,6 +1087,13 @@ __global__ __launch_bounds__(NT) void k_invit(const double* __restrict__ d, cons
         double lam = w[j];
         if (j > j0 && lam - lam_prev < pertol) lam = lam_prev + pertol;
         lam_prev = lam;
+        if (j > j0) {   // the previous member's LU overwrote d and e (uniform)
+            for (int i = threadIdx.x; i < n; i += NT) {
+                ds[i] = d[i];
+                es[i] = i < n - 1 ? e[i] : 0.0;
+            }
+            __syncthreads();
+        }
         // LU of T - lam I with partial pivoting (thread 0, sequential); ak, bk carry the
         // running diagonal and superdiagonal of the active row
         if (threadIdx.x == 0) {
@@ -1105,7 +1113,6 @@ __global__ __launch_bounds__(NT) void k_invit(const double* __restrict__ d, cons
                 const double ia = 1.0 / pv;
                 const double mult = (keep ? ck : akk) * ia;
                 piv[k] = keep ? 0 : 1;
-                A[k] = pv;
                 IA[k] = ia;
                 B[k] = keep ? bk : an;
                 C[k] = mult;
@@ -1118,7 +1125,6 @@ __global__ __launch_bounds__(NT) void k_invit(const double* __restrict__ d, cons
                 bn = bn2;
             }
             if (fabs(ak) < tiny) ak = (ak >= 0 ? tiny : -tiny);
-            A[n - 1] = ak;
             IA[n - 1] = 1.0 / ak;
         }
         // start vector
@@ -2023,7 +2029,7 @@ void heevx_vectors(hipStream_t s, EigWorkspace& ws, GemmWorkspace& gws, int num_
     APW_CHECK(mc >= 0 && mc <= nband, "heevx: %d needed eigenvectors of %d", mc, nband);
     {
     ProfScope ps(prof, s, "eig_invit", 0.0, 0.0);
-    const size_t lds_inv = (size_t)8 * n * sizeof(double) + 64 * sizeof(double) + (size_t)n * sizeof(int);
+    const size_t lds_inv = (size_t)5 * n * sizeof(double) + 64 * sizeof(double) + (size_t)n;
     APW_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_invit), hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)lds_inv));
     hipLaunchKernelGGL(k_invit, dim3(nband), dim3(NT), lds_inv, s, ws.d.get(), ws.e.get(), n, nband, eig, ws.z.get(),
