@@ -1100,10 +1100,12 @@ __global__ __launch_bounds__(NT) void k_invit(const double* __restrict__ d, cons
             double ak = ds[0] - lam, bk = n > 1 ? es[0] : 0.0;
             double an = n > 1 ? ds[1] - lam : 0.0, ck = es[0], bn = n > 2 ? es[1] : 0.0;
             for (int k = 0; k < n - 1; ++k) {
-                // operands of step k+1, independent of this step
-                const double an2 = k + 2 < n ? ds[k + 2] - lam : 0.0;
-                const double ck2 = k + 1 < n - 1 ? es[k + 1] : 0.0;
-                const double bn2 = k + 3 < n ? es[k + 2] : 0.0;
+                // operands of step k+1, independent of this step: unconditional loads at
+                // clamped indices (e[n-1] is 0, and d past n - 1 only feeds the unused
+                // value after the last step), shift by lam applied where it is consumed
+                const double dn2 = ds[min(k + 2, n - 1)];
+                const double ck2 = es[min(k + 1, n - 1)];
+                const double bn2 = es[min(k + 2, n - 1)];
                 // branch-free (selects keep the exec mask whole on the sequential chain):
                 // no interchange when |c_k| <= |a_k| (a tiny a_k is replaced by +-tiny), else
                 // rows k and k+1 are swapped
@@ -1120,7 +1122,7 @@ __global__ __launch_bounds__(NT) void k_invit(const double* __restrict__ d, cons
                 const double akn = keep ? fma(-mult, bk, an) : fma(-mult, an, bk);
                 bk = keep ? bn : -mult * bn;
                 ak = akn;
-                an = an2;
+                an = dn2 - lam;
                 ck = ck2;
                 bn = bn2;
             }
@@ -1154,7 +1156,7 @@ __global__ __launch_bounds__(NT) void k_invit(const double* __restrict__ d, cons
                     for (int q = 0; q < 8; ++q) {
                         c8[q] = C[k + q];
                         p8[q] = piv[k + q];
-                        x8[q] = k + q + 2 < n ? xv[k + q + 2] : 0.0;
+                        x8[q] = xv[min(k + q + 2, n - 1)];   // past n - 1: fed to no step
                     }
 #pragma unroll
                     for (int q = 0; q < 8; ++q) fwd(c8[q], p8[q], x8[q], k + q);
@@ -1178,7 +1180,7 @@ __global__ __launch_bounds__(NT) void k_invit(const double* __restrict__ d, cons
                         const int kk = k - q;
                         xx[q] = xv[kk];
                         bb[q] = B[kk];
-                        dd[q] = kk < n - 2 ? DD[kk] : 0.0;
+                        dd[q] = DD[kk];   // DD[n - 2] = 0: the LU's last step has no second superdiagonal
                         ia[q] = IA[kk];
                     }
 #pragma unroll
