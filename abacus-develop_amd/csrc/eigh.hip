@@ -928,22 +928,17 @@ __global__ __launch_bounds__(NW * 64) void k_tridiag_rows(const double2* __restr
 // lane of a wave reads the same address.
 __device__ int sturm(const double* d, const double* e2, int n, double x) {
     double pm = 1.0, p = d[0] - x;
-    int c;
-    double sg;   // sign of the last p, zeros resolved as above
-    if (p == 0.0) {
-        c = 1;
-        sg = -1.0;
-    } else {
-        c = p < 0.0;
-        sg = p < 0.0 ? -1.0 : 1.0;
-    }
-    auto step = [&](double di, double fi) {   // branch-free: the sign logic stays off the p chain
+    // the last p negative, zeros resolved as above: a per-lane boolean, so the sign logic
+    // is a few lane-mask operations beside the p chain rather than selects of doubles
+    bool neg = p == 0.0 || p < 0.0;
+    int c = neg;
+    auto step = [&](double di, double fi) {
         const double pn = fma(di - x, p, -fi * pm);
         pm = p;
         p = pn;
-        const double sn = p == 0.0 ? -sg : (p < 0.0 ? -1.0 : 1.0);
-        c += sn != sg;
-        sg = sn;
+        const bool z = p == 0.0, lt = p < 0.0;
+        c += z | (lt ^ neg);              // a zero always changes the sign
+        neg = (z & !neg) | (!z & lt);     // bitwise: lane-mask logic, no branches
     };
     int i = 1;
     for (; i + 7 < n; i += 8) {
