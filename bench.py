@@ -68,7 +68,7 @@ _PMC_FAMILY = {   # regular expressions over the kernel names of the committed t
     "fft_z_bwd_scatter": r"apw::k_zreg<\w+, \d+, 1>|apw::k_zbwd",
     "fft_z_fwd_gather": r"apw::k_zreg<\w+, \d+, -1>|apw::k_zfwd",
     "eig_bisect": r"apw::k_bisect", "eig_invit": r"apw::k_invit",
-    "eig_backtransform": r"apw::k_(backtransform|wy_)|small-grid apw::(zgemm_kernel<double|splitk_reduce<double)",
+    "eig_backtransform": r"apw::k_(backtransform|wy_)|back-transform apw::(zgemm_kernel<double|splitk_reduce<double)",
     "chol_inv": r"apw::k_chol_inv",
 }
 # entries made of several dispatches per launch: the dispatches that occur once per launch
